@@ -1,0 +1,209 @@
+"""ctypes binding of the CPU FP64 oracle (test infrastructure only).
+
+Only tests/, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+bench.py import this module; it is the checker, never the product path.
+The C sources are ``exwave_oracle.{c,h}`` next to this file; ``build()``
+compiles them with gcc into ``oracle/_build/``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_SO = os.path.join(_HERE, "_build", "libexwave_oracle.so")
+
+EXPL_EULER, CLRK4, LSRK45R2, LSRK33R2, LSRK59R2, ADER = range(6)
+INTEGRATORS = {"ExplEuler": EXPL_EULER, "clRK4": CLRK4, "LSRK45R2": LSRK45R2,
+               "LSRK33R2": LSRK33R2, "LSRK59R2": LSRK59R2, "ADER": ADER}
+TRANSFORM_NONE, TRANSFORM_REFERENCE_SINE, TRANSFORM_AXIS_2PI, TRANSFORM_PRODUCT_2PI = range(4)
+
+
+def build(force: bool = False) -> str:
+    if force or not os.path.exists(_SO):
+        subprocess.check_call(["make", "-s", "-C", _HERE])
+    return _SO
+
+
+class _Mesh(C.Structure):
+    _fields_ = [("dim", C.c_int), ("n_cells", C.c_int),
+                ("vertices", C.POINTER(C.c_double)), ("neighbor", C.POINTER(C.c_int)),
+                ("rho", C.POINTER(C.c_double)), ("speed", C.POINTER(C.c_double)),
+                ("min_vertex_distance_over_c", C.c_double)]
+
+
+class Exact(C.Structure):
+    _fields_ = [("dim", C.c_int), ("n_modes", C.c_int), ("amp", C.c_double * 8),
+                ("m", (C.c_int * 3) * 8), ("rho", C.c_double), ("c", C.c_double)]
+
+    @staticmethod
+    def make(dim, modes, amps=None, rho=1.0, c=1.0):
+        e = Exact()
+        e.dim = dim
+        e.n_modes = len(modes)
+        for k, mm in enumerate(modes):
+            e.amp[k] = 1.0 if amps is None else amps[k]
+            for d in range(dim):
+                e.m[k][d] = int(mm[d])
+        e.rho, e.c = rho, c
+        return e
+
+    @staticmethod
+    def reference(dim, membrane_modes):
+        """input_exact_solution, include/input_parameters.h:94-128"""
+        return Exact.make(dim, [[membrane_modes] * dim])
+
+
+class Problem(C.Structure):
+    _fields_ = [("dim", C.c_int), ("degree", C.c_int), ("n_intervals", C.c_int),
+                ("n_refinements", C.c_int), ("transform", C.c_int),
+                ("transform_factor", C.c_double), ("boundary_id", C.c_int),
+                ("integrator", C.c_int), ("cfl_number", C.c_double),
+                ("max_time_steps", C.c_int), ("final_time", C.c_double),
+                ("output_every_time", C.c_double), ("ader_recon", C.c_int),
+                ("exact", Exact)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = C.CDLL(build())
+        L = _lib
+        dp = C.POINTER(C.c_double)
+        L.ox_mesh_hypercube.restype = C.POINTER(_Mesh)
+        L.ox_mesh_hypercube.argtypes = [C.c_int] * 5 + [C.c_double, C.c_int]
+        L.ox_mesh_hypercube.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int]
+        L.ox_mesh_free.argtypes = [C.POINTER(_Mesh)]
+        L.ox_mesh_set_material.argtypes = [C.POINTER(_Mesh), dp, dp]
+        L.ox_apply.argtypes = [C.POINTER(_Mesh), C.c_int, dp, dp, C.c_double]
+        L.ox_rk_update.argtypes = [C.c_long, C.c_double, C.c_double, C.c_int, dp, dp, dp]
+        L.ox_time_step.argtypes = [C.POINTER(_Mesh), C.c_int, C.c_int, C.c_double, dp, dp, dp,
+                                   dp, C.c_int]
+        L.ox_ader_ck.argtypes = [C.POINTER(_Mesh), C.c_int, dp, dp, C.c_double, C.c_double,
+                                 C.c_double, dp]
+        L.ox_apply_ader.argtypes = [C.POINTER(_Mesh), C.c_int, dp, dp, C.c_double, C.c_int, dp]
+        L.ox_project.argtypes = [C.POINTER(_Mesh), C.c_int, C.POINTER(Exact), C.c_double, dp]
+        L.ox_l2_errors.argtypes = [C.POINTER(_Mesh), C.c_int, C.POINTER(Exact), C.c_double, dp,
+                                   dp]
+        L.ox_energy.restype = C.c_double
+        L.ox_energy.argtypes = [C.POINTER(_Mesh), C.c_int, dp]
+        L.ox_run.restype = C.c_int
+        L.ox_run.argtypes = [C.POINTER(Problem), C.c_char_p, C.c_int, dp]
+        L.ox_n_dofs.restype = C.c_long
+        L.ox_n_dofs.argtypes = [C.POINTER(_Mesh), C.c_int]
+        L.ox_gauss.argtypes = [C.c_int, dp, dp]
+        L.ox_lagrange.argtypes = [C.c_int, dp, C.c_double, dp, dp]
+    return _lib
+
+
+def _p(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Mesh:
+    """A hypercube mesh exactly as the reference builds it (see exwave_oracle.c)."""
+
+    def __init__(self, dim, n_intervals, n_refinements=0, transform=0, factor=0.0,
+                 boundary_id=2):
+        self.dim = dim
+        self._m = lib().ox_mesh_hypercube(dim, n_intervals, n_refinements, transform, factor,
+                                          boundary_id)
+        self.n_cells = self._m.contents.n_cells
+
+    def __del__(self):
+        if getattr(self, "_m", None) is not None and _lib is not None:
+            _lib.ox_mesh_free(self._m)
+            self._m = None
+
+    @property
+    def vertices(self):
+        nv = 1 << self.dim
+        return np.ctypeslib.as_array(self._m.contents.vertices,
+                                     shape=(self.n_cells, nv, self.dim)).copy()
+
+    @property
+    def neighbor(self):
+        return np.ctypeslib.as_array(self._m.contents.neighbor,
+                                     shape=(self.n_cells, 2 * self.dim)).copy()
+
+    @property
+    def rho(self):
+        return np.ctypeslib.as_array(self._m.contents.rho, shape=(self.n_cells,)).copy()
+
+    @property
+    def speed(self):
+        return np.ctypeslib.as_array(self._m.contents.speed, shape=(self.n_cells,)).copy()
+
+    @property
+    def min_vertex_distance_over_c(self):
+        return self._m.contents.min_vertex_distance_over_c
+
+    def set_material(self, rho, speed):
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        speed = np.ascontiguousarray(speed, dtype=np.float64)
+        lib().ox_mesh_set_material(self._m, _p(rho), _p(speed))
+
+    def n_dofs(self, degree):
+        return lib().ox_n_dofs(self._m, degree)
+
+    def apply(self, degree, src, sign=1.0):
+        dst = np.empty_like(src)
+        lib().ox_apply(self._m, degree, _p(src), _p(dst), sign)
+        return dst
+
+    def time_step(self, degree, integrator, dt, vec_n, vec_np, ader_recon=0):
+        t1 = np.zeros_like(vec_n)
+        t2 = np.zeros_like(vec_n)
+        lib().ox_time_step(self._m, degree, integrator, dt, _p(vec_n), _p(vec_np), _p(t1),
+                           _p(t2), ader_recon)
+
+    def ader_ck(self, degree, src, recon, t2, t1, te):
+        dst = np.empty_like(src)
+        rp = _p(recon) if recon is not None else None
+        lib().ox_ader_ck(self._m, degree, _p(src), rp, t2, t1, te, _p(dst))
+        return dst
+
+    def apply_ader(self, degree, src, dt, recon=0):
+        dst = np.empty_like(src)
+        scr = np.empty(2 * src.size)
+        lib().ox_apply_ader(self._m, degree, _p(src), _p(dst), dt, recon, _p(scr))
+        return dst
+
+    def project(self, degree, exact, t=0.0):
+        dst = np.empty(self.n_dofs(degree))
+        lib().ox_project(self._m, degree, C.byref(exact), t, _p(dst))
+        return dst
+
+    def l2_errors(self, degree, exact, t, u):
+        out = np.zeros(3)
+        lib().ox_l2_errors(self._m, degree, C.byref(exact), t, _p(u), _p(out))
+        return out
+
+    def energy(self, degree, u):
+        return lib().ox_energy(self._m, degree, _p(u))
+
+
+def run(problem: Problem):
+    """Run a whole solve; returns (n_steps, [(step, time, err_p, err_v, mag_p)], energy)."""
+    buf = C.create_string_buffer(1 << 16)
+    en = C.c_double(0.0)
+    steps = lib().ox_run(C.byref(problem), buf, len(buf), C.byref(en))
+    rows = []
+    for line in buf.value.decode().strip().splitlines():
+        s, t, ep, ev, mp = line.split()
+        rows.append((int(s), float(t), float(ep), float(ev), float(mp)))
+    return steps, rows, en.value
+
+
+def gauss(n):
+    x = np.zeros(n)
+    w = np.zeros(n)
+    lib().ox_gauss(n, _p(x), _p(w))
+    return x, w

@@ -1,0 +1,943 @@
+/*
+ * exwave_oracle.c -- CPU FP64 restatement of ExWave's hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP path (tests/, smoke(),
+ * bench.py's cpu_baseline).  See exwave_oracle.h for the layout and for the
+ * list of reference functions restated here; each function below cites the
+ * reference lines it follows.
+ */
+#include "exwave_oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* 1D building blocks                                                        */
+/* ------------------------------------------------------------------------ */
+
+/* Gauss-Legendre rule with n points on [0,1], ascending (deal.II QGauss<1>). */
+void ox_gauss(int n, double *x, double *w) {
+  for (int i = 0; i < (n + 1) / 2; ++i) {
+    double z = cos(M_PI * (i + 0.75) / (n + 0.5));
+    double pp = 0.0;
+    for (int it = 0; it < 100; ++it) {
+      double p1 = 1.0, p2 = 0.0;
+      for (int j = 1; j <= n; ++j) {
+        double p3 = p2;
+        p2 = p1;
+        p1 = ((2.0 * j - 1.0) * z * p2 - (j - 1.0) * p3) / j;
+      }
+      pp = n * (z * p1 - p2) / (z * z - 1.0);
+      double z1 = z;
+      z = z1 - p1 / pp;
+      if (fabs(z - z1) < 1e-16) {
+        /* one more evaluation for the derivative at the converged point */
+        p1 = 1.0;
+        p2 = 0.0;
+        for (int j = 1; j <= n; ++j) {
+          double p3 = p2;
+          p2 = p1;
+          p1 = ((2.0 * j - 1.0) * z * p2 - (j - 1.0) * p3) / j;
+        }
+        pp = n * (z * p1 - p2) / (z * z - 1.0);
+        break;
+      }
+    }
+    /* z is the i-th largest root on [-1,1] */
+    double wi = 2.0 / ((1.0 - z * z) * pp * pp);
+    x[n - 1 - i] = 0.5 * (1.0 + z);
+    x[i] = 0.5 * (1.0 - z);
+    w[i] = w[n - 1 - i] = 0.5 * wi;
+  }
+  if (n % 2 == 1) x[n / 2] = 0.5;
+}
+
+void ox_lagrange(int n, const double *nodes, double x, double *val, double *der) {
+  for (int i = 0; i < n; ++i) {
+    double v = 1.0;
+    for (int j = 0; j < n; ++j)
+      if (j != i) v *= (x - nodes[j]) / (nodes[i] - nodes[j]);
+    if (val) val[i] = v;
+    if (der) {
+      double d = 0.0;
+      for (int m = 0; m < n; ++m) {
+        if (m == i) continue;
+        double t = 1.0 / (nodes[i] - nodes[m]);
+        for (int j = 0; j < n; ++j)
+          if (j != i && j != m) t *= (x - nodes[j]) / (nodes[i] - nodes[j]);
+        d += t;
+      }
+      der[i] = d;
+    }
+  }
+}
+
+void ox_basis_init(ox_basis *b, int degree) {
+  int n = degree + 1;
+  b->n = n;
+  ox_gauss(n, b->x, b->w);
+  for (int q = 0; q < n; ++q) ox_lagrange(n, b->x, b->x[q], NULL, b->D + q * n);
+  ox_lagrange(n, b->x, 0.0, b->face[0], NULL);
+  ox_lagrange(n, b->x, 1.0, b->face[1], NULL);
+}
+
+static int ipow(int a, int e) {
+  int r = 1;
+  while (e-- > 0) r *= a;
+  return r;
+}
+
+/* Apply the 1D matrix A (n_out x n_in, row-major) along direction `dir` of the
+ * tensor `in` (sizes sz[0..dim-1], xi_0 fastest).  transpose: use A^T (then A is
+ * n_in x n_out).  add: accumulate into out. */
+static void tensor_apply(int dim, const int *sz, int dir, const double *A, int n_out,
+                         const double *in, double *out, int transpose, int add) {
+  int n_in = sz[dir];
+  int stride = 1;
+  for (int d = 0; d < dir; ++d) stride *= sz[d];
+  int outer = 1;
+  for (int d = dir + 1; d < dim; ++d) outer *= sz[d];
+  for (int o = 0; o < outer; ++o)
+    for (int s = 0; s < stride; ++s) {
+      const double *ip = in + (long)o * stride * n_in + s;
+      double *op = out + (long)o * stride * n_out + s;
+      for (int r = 0; r < n_out; ++r) {
+        double acc = 0.0;
+        for (int c = 0; c < n_in; ++c)
+          acc += (transpose ? A[c * n_out + r] : A[r * n_in + c]) * ip[c * stride];
+        if (add)
+          op[r * stride] += acc;
+        else
+          op[r * stride] = acc;
+      }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* geometry: multilinear cells (the reference's cells are multilinear images  */
+/* of their 2^dim vertices, include/input_parameters.h:66-87 with a flat      */
+/* manifold and MappingQGeneric(k) reproducing the multilinear map)           */
+/* ------------------------------------------------------------------------ */
+
+static void multilinear_point(int dim, const double *V, const double *xi, double *x) {
+  int nv = 1 << dim;
+  for (int i = 0; i < dim; ++i) x[i] = 0.0;
+  for (int v = 0; v < nv; ++v) {
+    double s = 1.0;
+    for (int d = 0; d < dim; ++d) s *= ((v >> d) & 1) ? xi[d] : 1.0 - xi[d];
+    for (int i = 0; i < dim; ++i) x[i] += s * V[v * dim + i];
+  }
+}
+
+/* J[i*dim+e] = dx_i/dxi_e, Jinv[e*dim+i] = dxi_e/dx_i; returns det J */
+static double multilinear_jacobian(int dim, const double *V, const double *xi, double *J,
+                                   double *Jinv) {
+  int nv = 1 << dim;
+  for (int k = 0; k < dim * dim; ++k) J[k] = 0.0;
+  for (int v = 0; v < nv; ++v)
+    for (int e = 0; e < dim; ++e) {
+      double s = ((v >> e) & 1) ? 1.0 : -1.0;
+      for (int d = 0; d < dim; ++d)
+        if (d != e) s *= ((v >> d) & 1) ? xi[d] : 1.0 - xi[d];
+      for (int i = 0; i < dim; ++i) J[i * dim + e] += s * V[v * dim + i];
+    }
+  double det;
+  if (dim == 2) {
+    det = J[0] * J[3] - J[1] * J[2];
+    Jinv[0] = J[3] / det;
+    Jinv[1] = -J[1] / det;
+    Jinv[2] = -J[2] / det;
+    Jinv[3] = J[0] / det;
+  } else {
+    double a = J[0], b = J[1], c = J[2], d = J[3], e = J[4], f = J[5], g = J[6], h = J[7],
+           k = J[8];
+    double A = e * k - f * h, B = -(d * k - f * g), C = d * h - e * g;
+    det = a * A + b * B + c * C;
+    Jinv[0] = A / det;
+    Jinv[3] = B / det;
+    Jinv[6] = C / det;
+    Jinv[1] = -(b * k - c * h) / det;
+    Jinv[4] = (a * k - c * g) / det;
+    Jinv[7] = -(a * h - b * g) / det;
+    Jinv[2] = (b * f - c * e) / det;
+    Jinv[5] = -(a * f - c * d) / det;
+    Jinv[8] = (a * e - b * d) / det;
+  }
+  return det;
+}
+
+/* ------------------------------------------------------------------------ */
+/* mesh: GridGenerator::subdivided_hyper_cube(n,0,1) + transform of the       */
+/* coarse vertices + refine_global (include/input_parameters.h:59-88)         */
+/* ------------------------------------------------------------------------ */
+
+static void transform_vertex(int dim, int type, double f, double *x) {
+  double in[3] = {x[0], x[1], dim > 2 ? x[2] : 0.0};
+  if (type == OX_TRANSFORM_REFERENCE_SINE) {
+    /* include/input_parameters.h:41-50 */
+    double s = f;
+    for (int d = 0; d < dim; ++d) s *= sin(M_PI * in[d]);
+    x[0] = in[0] + s;
+  } else if (type == OX_TRANSFORM_AXIS_2PI) {
+    for (int i = 0; i < dim; ++i) {
+      double s = f;
+      for (int e = 0; e < dim; ++e)
+        if (e != i) s *= sin(2.0 * M_PI * in[e]);
+      x[i] = in[i] + s;
+    }
+  } else if (type == OX_TRANSFORM_PRODUCT_2PI) {
+    double s = f;
+    for (int e = 0; e < dim; ++e) s *= sin(2.0 * M_PI * in[e]);
+    for (int i = 0; i < dim; ++i) x[i] = in[i] + s;
+  }
+}
+
+ox_mesh *ox_mesh_hypercube(int dim, int n_intervals, int n_refinements, int transform,
+                           double factor, int boundary_id) {
+  ox_mesh *m = (ox_mesh *)calloc(1, sizeof(ox_mesh));
+  int N = n_intervals, R = 1 << n_refinements, Nf = N * R;
+  int nv = 1 << dim;
+  m->dim = dim;
+  m->n_cells = ipow(Nf, dim);
+  m->vertices = (double *)malloc(sizeof(double) * m->n_cells * nv * dim);
+  m->neighbor = (int *)malloc(sizeof(int) * m->n_cells * 2 * dim);
+  m->rho = (double *)malloc(sizeof(double) * m->n_cells);
+  m->speed = (double *)malloc(sizeof(double) * m->n_cells);
+  double mind = 1e300;
+  for (int c = 0; c < m->n_cells; ++c) {
+    int idx[3] = {0, 0, 0};
+    int t = c;
+    for (int d = 0; d < dim; ++d) {
+      idx[d] = t % Nf;
+      t /= Nf;
+    }
+    /* coarse parent vertices (transformed) */
+    double PV[8 * 3];
+    for (int v = 0; v < nv; ++v) {
+      double x[3] = {0.0, 0.0, 0.0};
+      for (int d = 0; d < dim; ++d) x[d] = (double)(idx[d] / R + ((v >> d) & 1)) / N;
+      transform_vertex(dim, transform, factor, x);
+      for (int d = 0; d < dim; ++d) PV[v * dim + d] = x[d];
+    }
+    /* child vertices: multilinear map of the parent at dyadic points */
+    for (int v = 0; v < nv; ++v) {
+      double xi[3];
+      for (int d = 0; d < dim; ++d) xi[d] = (double)(idx[d] % R + ((v >> d) & 1)) / R;
+      multilinear_point(dim, PV, xi, m->vertices + ((long)c * nv + v) * dim);
+    }
+    /* neighbours */
+    for (int d = 0; d < dim; ++d)
+      for (int s = 0; s < 2; ++s) {
+        int nidx = idx[d] + (s ? 1 : -1);
+        int f = 2 * d + s;
+        if (nidx < 0 || nidx >= Nf)
+          m->neighbor[c * 2 * dim + f] = -boundary_id;
+        else {
+          int stride = ipow(Nf, d);
+          m->neighbor[c * 2 * dim + f] = c + (s ? stride : -stride);
+        }
+      }
+    m->rho[c] = 1.0;
+    m->speed[c] = 1.0;
+    /* TriaAccessor::minimum_vertex_distance: min over all vertex pairs */
+    const double *V = m->vertices + (long)c * nv * dim;
+    for (int a = 0; a < nv; ++a)
+      for (int b = a + 1; b < nv; ++b) {
+        double s = 0.0;
+        for (int d = 0; d < dim; ++d) s += (V[a * dim + d] - V[b * dim + d]) * (V[a * dim + d] - V[b * dim + d]);
+        if (s < mind) mind = s;
+      }
+  }
+  m->min_vertex_distance_over_c = sqrt(mind);
+  return m;
+}
+
+void ox_mesh_set_material(ox_mesh *m, const double *rho, const double *speed) {
+  int nv = 1 << m->dim;
+  double mind = 1e300;
+  for (int c = 0; c < m->n_cells; ++c) {
+    m->rho[c] = rho[c];
+    m->speed[c] = speed[c];
+    const double *V = m->vertices + (long)c * nv * m->dim;
+    for (int a = 0; a < nv; ++a)
+      for (int b = a + 1; b < nv; ++b) {
+        double s = 0.0;
+        for (int d = 0; d < m->dim; ++d)
+          s += (V[a * m->dim + d] - V[b * m->dim + d]) * (V[a * m->dim + d] - V[b * m->dim + d]);
+        s = sqrt(s) / speed[c]; /* source/explicit_wave.cc:537-540 */
+        if (s < mind) mind = s;
+      }
+  }
+  m->min_vertex_distance_over_c = mind;
+}
+
+void ox_mesh_free(ox_mesh *m) {
+  if (!m) return;
+  free(m->vertices);
+  free(m->neighbor);
+  free(m->rho);
+  free(m->speed);
+  free(m);
+}
+
+long ox_n_dofs(const ox_mesh *m, int degree) {
+  return (long)m->n_cells * (m->dim + 1) * ipow(degree + 1, m->dim);
+}
+
+/* ------------------------------------------------------------------------ */
+/* DG operator                                                               */
+/* ------------------------------------------------------------------------ */
+
+/* numerical flux on one face quadrature point.
+ * source/wave_equation_operations.cc:480-551 (inner) and 569-629 (boundary).
+ * up/um: (v_0..v_{dim-1}, p) on this (plus) side and on the neighbour;
+ * nb_bid < 0 selects the boundary branch with id -nb_bid.  out: the test
+ * values submitted on the plus side, multiplied by boundary_fac. */
+static void face_flux(int dim, const double *nrm, const double *up, const double *um,
+                      double rho_p, double c_p, double rho_m, double c_m, int boundary_id,
+                      double fac, double *out) {
+  double nvp = 0.0;
+  for (int d = 0; d < dim; ++d) nvp += up[d] * nrm[d];
+  double pp = up[dim];
+  double tau_p = 1.0 / c_p / rho_p;
+  double lambda;
+  if (boundary_id == 0) {
+    double nvm = 0.0;
+    for (int d = 0; d < dim; ++d) nvm -= um[d] * nrm[d];
+    double tau_m = 1.0 / c_m / rho_m;
+    double tau_inv = 1.0 / (tau_p + tau_m);
+    lambda = tau_inv * (nvp + nvm + tau_p * pp + tau_m * um[dim]);
+  } else if (boundary_id == 1) {
+    lambda = 1.0 / tau_p * nvp + pp;
+  } else if (boundary_id == 2) {
+    lambda = 0.0;
+  } else {
+    double t2 = 1.0 / c_p / rho_p;
+    lambda = tau_p / (tau_p + t2) * pp + 1.0 / (tau_p + t2) * nvp;
+  }
+  for (int d = 0; d < dim; ++d) out[d] = fac * (pp - lambda) / rho_p * nrm[d];
+  out[dim] = fac * c_p * c_p * rho_p * (-nvp + tau_p * (lambda - pp));
+}
+
+typedef struct work {
+  double *r, *g, *tmp;
+} work;
+
+/* One cell of M^-1 A(src): evaluate_cell + the faces of this cell +
+ * inverse mass (GAUSS_POINTS_VECTOR_OPERATION form,
+ * source/wave_equation_operations.cc:640-657). */
+static void apply_cell(const ox_mesh *m, const ox_basis *b, int c, const double *src,
+                       double *dst, double sign, work *W) {
+  const int dim = m->dim, n = b->n, nf = dim + 1;
+  const int nd = ipow(n, dim), nv = 1 << dim;
+  const int sz[3] = {n, n, n};
+  const double *V = m->vertices + (long)c * nv * dim;
+  const double *u = src + (long)c * nf * nd;
+  const double rho = m->rho[c], cc = m->speed[c];
+  double *r = W->r, *g = W->g, *tmp = W->tmp;
+  memset(r, 0, sizeof(double) * nf * nd);
+
+  /* cell integral, evaluate_cell (:424-456) */
+  const double *p = u + dim * nd;
+  for (int e = 0; e < dim; ++e) tensor_apply(dim, sz, e, b->D, n, p, tmp + e * nd, 0, 0);
+  double *jxw = g + dim * nd;
+  for (int q = 0; q < nd; ++q) {
+    double xi[3], J[9], Ji[9];
+    int t = q;
+    double wq = 1.0;
+    for (int d = 0; d < dim; ++d) {
+      xi[d] = b->x[t % n];
+      wq *= b->w[t % n];
+      t /= n;
+    }
+    double det = multilinear_jacobian(dim, V, xi, J, Ji);
+    double dw = det * wq;
+    jxw[q] = dw;
+    for (int i = 0; i < dim; ++i) {
+      double gp = 0.0;
+      for (int e = 0; e < dim; ++e) gp += Ji[e * dim + i] * tmp[e * nd + q];
+      r[i * nd + q] = -gp / rho * dw;
+    }
+    for (int e = 0; e < dim; ++e) {
+      double s = 0.0;
+      for (int i = 0; i < dim; ++i) s += Ji[e * dim + i] * u[i * nd + q];
+      g[e * nd + q] = rho * cc * cc * dw * s;
+    }
+  }
+  for (int e = 0; e < dim; ++e) tensor_apply(dim, sz, e, b->D, n, g + e * nd, r + dim * nd, 1, 1);
+
+  /* faces of this cell (:480-551, :569-629), each evaluated from this side */
+  const int nt = ipow(n, dim - 1);
+  for (int f = 0; f < 2 * dim; ++f) {
+    const int d = f >> 1, s = f & 1;
+    const int nb = m->neighbor[c * 2 * dim + f];
+    const int stride_d = ipow(n, d);
+    const double *un = nb >= 0 ? src + (long)nb * nf * nd : NULL;
+    for (int t = 0; t < nt; ++t) {
+      int base = 0, tt = t;
+      double xi[3], wt = 1.0;
+      xi[d] = (double)s;
+      for (int e = 0; e < dim; ++e) {
+        if (e == d) continue;
+        int te = tt % n;
+        tt /= n;
+        base += te * ipow(n, e);
+        xi[e] = b->x[te];
+        wt *= b->w[te];
+      }
+      double J[9], Ji[9];
+      double det = multilinear_jacobian(dim, V, xi, J, Ji);
+      double nrm[3], na = 0.0;
+      for (int i = 0; i < dim; ++i) {
+        nrm[i] = Ji[d * dim + i];
+        na += nrm[i] * nrm[i];
+      }
+      na = sqrt(na);
+      for (int i = 0; i < dim; ++i) nrm[i] = (s ? 1.0 : -1.0) * nrm[i] / na;
+      double jxw_f = fabs(det) * na * wt;
+      double up[4], um[4] = {0, 0, 0, 0}, val[4];
+      for (int k = 0; k < nf; ++k) {
+        double a = 0.0, bb = 0.0;
+        for (int i = 0; i < n; ++i) {
+          a += b->face[s][i] * u[k * nd + base + i * stride_d];
+          if (un) bb += b->face[1 - s][i] * un[k * nd + base + i * stride_d];
+        }
+        up[k] = a;
+        um[k] = bb;
+      }
+      face_flux(dim, nrm, up, um, rho, cc, nb >= 0 ? m->rho[nb] : rho,
+                nb >= 0 ? m->speed[nb] : cc, nb >= 0 ? 0 : -nb, 1.0, val);
+      for (int k = 0; k < nf; ++k)
+        for (int i = 0; i < n; ++i)
+          r[k * nd + base + i * stride_d] += b->face[s][i] * val[k] * jxw_f;
+    }
+  }
+
+  /* inverse mass: collocation at Gauss points -> divide by JxW */
+  double *o = dst + (long)c * nf * nd;
+  for (int k = 0; k < nf; ++k)
+    for (int q = 0; q < nd; ++q) o[k * nd + q] = sign * r[k * nd + q] / jxw[q];
+}
+
+static work work_alloc(int dim, int n) {
+  int nd = ipow(n, dim);
+  work W;
+  W.r = (double *)malloc(sizeof(double) * 4 * nd);
+  W.g = (double *)malloc(sizeof(double) * 4 * nd);
+  W.tmp = (double *)malloc(sizeof(double) * 16 * nd);
+  return W;
+}
+static void work_free(work *W) {
+  free(W->r);
+  free(W->g);
+  free(W->tmp);
+}
+
+/* WaveEquationOperation::apply (source/wave_equation_operations.cc:665-685)
+ * multiplied by `sign` (-1 reproduces local_apply_secondader_domain +
+ * the ADER face sweeps with boundary_fac = -1, :1265-1342). */
+void ox_apply(const ox_mesh *m, int degree, const double *src, double *dst, double sign) {
+  ox_basis b;
+  ox_basis_init(&b, degree);
+#pragma omp parallel
+  {
+    work W = work_alloc(m->dim, b.n);
+#pragma omp for schedule(static)
+    for (int c = 0; c < m->n_cells; ++c) apply_cell(m, &b, c, src, dst, sign, &W);
+    work_free(&W);
+  }
+}
+
+/* RKVectorUpdater::apply_to_subrange (source/time_integrators.cc:128-155) */
+void ox_rk_update(long n, double f1, double f2, int is_last, const double *v1, double *v2,
+                  double *v3) {
+  if (is_last) {
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; ++i) v2[i] = v3[i] + f1 * v1[i];
+  } else {
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; ++i) {
+      double up = v1[i];
+      v2[i] += f1 * up;
+      v3[i] = v2[i] + f2 * up;
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* ADER Cauchy-Kowalevski (source/wave_equation_operations.cc:961-1262)      */
+/* ------------------------------------------------------------------------ */
+
+typedef struct ck_ctx {
+  int dim, degree;
+  const double *V;
+  double rho, c2rho, rho_inv;
+  double tstart, tend;
+} ck_ctx;
+
+/* integrate_taylor_cauchykovalewski_step<step_no, add_into_contrib>
+ * (:1063-1262) with spectral_evaluation = true.  s holds (dim+1) fields at
+ * the (my_degree+1)^dim Gauss points of this step's degree. */
+static void ck_step(const ck_ctx *X, int step_no, int add, double *s, double *contrib) {
+  const int dim = X->dim, k = X->degree;
+  double fac = -0.5;
+  for (int kk = 2; kk <= step_no + 2; ++kk) fac /= -(double)(kk + 1);
+  const double fac_t = fac * (pow(X->tend, step_no + 3) - pow(X->tstart, step_no + 3));
+  const int reduce_step = step_no / 2;
+  const int at_reduce_step = (step_no - 2 * reduce_step) == 1;
+  const int my_degree = reduce_step > 0 ? k - 2 * reduce_step : k;
+  const int n = my_degree + 1, nd = ipow(n, dim);
+  const int sz[3] = {n, n, n};
+  double x[OX_MAX_N], w[OX_MAX_N], D[OX_MAX_N * OX_MAX_N];
+  ox_gauss(n, x, w);
+  for (int q = 0; q < n; ++q) ox_lagrange(n, x, x[q], NULL, D + q * n);
+
+  /* reference-coordinate gradients of every field */
+  double *gr = (double *)malloc(sizeof(double) * (dim + 1) * dim * nd);
+  for (int f = 0; f <= dim; ++f)
+    for (int e = 0; e < dim; ++e)
+      tensor_apply(dim, sz, e, D, n, s + f * nd, gr + (f * dim + e) * nd, 0, 0);
+  for (int q = 0; q < nd; ++q) {
+    double xi[3], J[9], Ji[9];
+    int t = q;
+    for (int d = 0; d < dim; ++d) {
+      xi[d] = x[t % n];
+      t /= n;
+    }
+    multilinear_jacobian(dim, X->V, xi, J, Ji);
+    double div = 0.0, pg[3];
+    for (int d = 0; d < dim; ++d)
+      for (int e = 0; e < dim; ++e) div += gr[(d * dim + e) * nd + q] * Ji[e * dim + d];
+    for (int d = 0; d < dim; ++d) {
+      pg[d] = 0.0;
+      for (int e = 0; e < dim; ++e) pg[d] += gr[(dim * dim + e) * nd + q] * Ji[e * dim + d];
+    }
+    for (int d = 0; d < dim; ++d) {
+      double val = X->rho_inv * pg[d];
+      contrib[d * nd + q] = (add ? contrib[d * nd + q] : 0.0) + fac_t * val;
+      s[d * nd + q] = val;
+    }
+    double val = X->c2rho * div;
+    contrib[dim * nd + q] = (add ? contrib[dim * nd + q] : 0.0) + fac_t * val;
+    s[dim * nd + q] = val;
+  }
+  free(gr);
+
+  if (step_no < k - 2) {
+    if (at_reduce_step) {
+      /* weighted L2 projection degree my_degree -> next_degree on the unit
+       * cell (setup :287-313, do_backward :1244) */
+      const int next_degree = my_degree >= 2 ? my_degree - 2 : 1;
+      const int nl = next_degree + 1, ndl = ipow(nl, dim);
+      double xl[OX_MAX_N], wl[OX_MAX_N];
+      ox_gauss(nl, xl, wl);
+      double Vlh[OX_MAX_N * OX_MAX_N]; /* Vlh[q*nl+i] = l^low_i(x^high_q) */
+      double P[OX_MAX_N * OX_MAX_N];   /* P[i*n+q] = Vlh[q][i] w_q / wl_i */
+      for (int q = 0; q < n; ++q) ox_lagrange(nl, xl, x[q], Vlh + q * nl, NULL);
+      for (int i = 0; i < nl; ++i)
+        for (int q = 0; q < n; ++q) P[i * n + q] = Vlh[q * nl + i] * w[q] / wl[i];
+      double *a = (double *)malloc(sizeof(double) * nd);
+      double *bb = (double *)malloc(sizeof(double) * nd);
+      double *slow = (double *)malloc(sizeof(double) * (dim + 1) * ndl);
+      double *clow = (double *)malloc(sizeof(double) * (dim + 1) * ndl);
+      for (int f = 0; f <= dim; ++f) {
+        /* apply P along each direction, shrinking one direction at a time */
+        int szc[3] = {n, n, n};
+        const double *in = s + f * nd;
+        double *bufs[2] = {a, bb};
+        for (int e = 0; e < dim; ++e) {
+          double *out = (e == dim - 1) ? slow + f * ndl : bufs[e & 1];
+          tensor_apply(dim, szc, e, P, nl, in, out, 0, 0);
+          szc[e] = nl;
+          in = out;
+        }
+      }
+      ck_step(X, step_no + 1, 0, slow, clow);
+      /* interpolate back (do_forward :1252) and add */
+      for (int f = 0; f <= dim; ++f) {
+        int szc[3] = {nl, nl, nl};
+        const double *in = clow + f * ndl;
+        double *bufs[2] = {a, bb};
+        for (int e = 0; e < dim; ++e) {
+          double *out = bufs[e & 1];
+          tensor_apply(dim, szc, e, Vlh, n, in, out, 0, 0);
+          szc[e] = n;
+          in = out;
+        }
+        for (int q = 0; q < nd; ++q) contrib[f * nd + q] += in[q];
+      }
+      free(a);
+      free(bb);
+      free(slow);
+      free(clow);
+    } else {
+      ck_step(X, step_no + 1, 1, s, contrib);
+    }
+  }
+}
+
+/* integrate_taylor_cauchykovalewski (:961-1059) for all cells */
+void ox_ader_ck(const ox_mesh *m, int degree, const double *src, const double *recon, double t2,
+                double t1, double te, double *tempsrc) {
+  ox_basis b;
+  ox_basis_init(&b, degree);
+  const int dim = m->dim, n = b.n, nf = dim + 1, nd = ipow(n, dim), nv = 1 << dim;
+  const int sz[3] = {n, n, n};
+#pragma omp parallel
+  {
+    double *s = (double *)malloc(sizeof(double) * nf * nd);
+    double *contrib = (double *)malloc(sizeof(double) * nf * nd);
+    double *gr = (double *)malloc(sizeof(double) * nf * dim * nd);
+#pragma omp for schedule(static)
+    for (int c = 0; c < m->n_cells; ++c) {
+      const double *u = src + (long)c * nf * nd;
+      ck_ctx X;
+      X.dim = dim;
+      X.degree = degree;
+      X.V = m->vertices + (long)c * nv * dim;
+      X.rho = m->rho[c];
+      X.rho_inv = 1.0 / m->rho[c];
+      X.c2rho = m->speed[c] * m->speed[c] * m->rho[c];
+      X.tstart = t1 - te;
+      X.tend = t2 - te;
+      const double c1 = 0.5 * ((t2 - te) * (t2 - te) - (t1 - te) * (t1 - te));
+      for (int i = 0; i < nf * nd; ++i) contrib[i] = (t2 - t1) * u[i];
+      if (recon) {
+        const double *r = recon + (long)c * nf * nd;
+        for (int i = 0; i < nf * nd; ++i) {
+          contrib[i] += c1 * r[i];
+          s[i] = -r[i];
+        }
+      } else {
+        for (int f = 0; f < nf; ++f)
+          for (int e = 0; e < dim; ++e)
+            tensor_apply(dim, sz, e, b.D, n, u + f * nd, gr + (f * dim + e) * nd, 0, 0);
+        for (int q = 0; q < nd; ++q) {
+          double xi[3], J[9], Ji[9];
+          int t = q;
+          for (int d = 0; d < dim; ++d) {
+            xi[d] = b.x[t % n];
+            t /= n;
+          }
+          multilinear_jacobian(dim, X.V, xi, J, Ji);
+          double div = 0.0;
+          for (int d = 0; d < dim; ++d) {
+            double pg = 0.0;
+            for (int e = 0; e < dim; ++e) {
+              pg += gr[(dim * dim + e) * nd + q] * Ji[e * dim + d];
+              div += gr[(d * dim + e) * nd + q] * Ji[e * dim + d];
+            }
+            double nx = -X.rho_inv * pg;
+            contrib[d * nd + q] += c1 * nx;
+            s[d * nd + q] = -nx;
+          }
+          double np = -X.c2rho * div;
+          contrib[dim * nd + q] += c1 * np;
+          s[dim * nd + q] = -np;
+        }
+      }
+      if (degree >= 2) ck_step(&X, 0, 1, s, contrib);
+      memcpy(tempsrc + (long)c * nf * nd, contrib, sizeof(double) * nf * nd);
+    }
+    free(s);
+    free(contrib);
+    free(gr);
+  }
+}
+
+/* WaveEquationOperationADER::apply_ader (:1346-1381); scratch: 2 vectors */
+void ox_apply_ader(const ox_mesh *m, int degree, const double *src, double *dst, double dt,
+                   int recon, double *scratch) {
+  long N = ox_n_dofs(m, degree);
+  double *tempsrc = scratch, *rec = scratch + N;
+  if (recon) ox_apply(m, degree, src, rec, 1.0);
+  ox_ader_ck(m, degree, src, recon ? rec : NULL, dt, 0.0, 0.0, tempsrc);
+  ox_apply(m, degree, tempsrc, dst, -1.0);
+}
+
+/* ------------------------------------------------------------------------ */
+/* time integrators (source/time_integrators.cc)                             */
+/* ------------------------------------------------------------------------ */
+
+static void lsrk_2reg(const ox_mesh *m, int degree, int S, const double *a, const double *bcoef,
+                      double dt, double *vec_n, double *vec_np, double *tmp1) {
+  long N = ox_n_dofs(m, degree);
+  /* stage 1: apply(vec_n) ; update(v2 = vec_n, v3 = vec_np) then the
+   * registers swap roles every stage (:260-280) */
+  double *src = vec_n, *v2 = vec_n, *v3 = vec_np;
+  for (int s = 0; s < S; ++s) {
+    ox_apply(m, degree, src, tmp1, 1.0);
+    if (s < S - 1) {
+      ox_rk_update(N, a[s] * dt, (bcoef[s] - a[s]) * dt, 0, tmp1, v2, v3);
+      /* next stage input is v2, accumulator v3 */
+      double *nv2 = v3, *nv3 = v2;
+      src = v2;
+      v2 = nv2;
+      v3 = nv3;
+    } else {
+      /* last stage: the input register receives accumulator + f1*k
+       * (e.g. stage 5: update(vec_tmp1, vec_np, vec_n), :279-280) */
+      ox_rk_update(N, bcoef[s] * dt, 0.0, 1, tmp1, src, v2);
+    }
+  }
+  if (src != vec_np) memcpy(vec_np, src, sizeof(double) * N);
+}
+
+void ox_time_step(const ox_mesh *m, int degree, int integrator, double dt, double *vec_n,
+                  double *vec_np, double *tmp1, double *tmp2, int ader_recon) {
+  long N = ox_n_dofs(m, degree);
+  if (integrator == OX_EXPL_EULER) {
+    /* :24-34 */
+    ox_apply(m, degree, vec_n, vec_np, 1.0);
+    for (long i = 0; i < N; ++i) vec_np[i] = dt * vec_np[i] + vec_n[i];
+  } else if (integrator == OX_CLRK4) {
+    /* :66-102 */
+    const double w[4] = {dt / 6., dt / 3., dt / 3., dt / 6.};
+    const double cfac[3] = {0.5 * dt, 0.5 * dt, dt};
+    ox_apply(m, degree, vec_n, tmp1, 1.0);
+    for (int st = 0; st < 4; ++st) {
+      if (st > 0) ox_apply(m, degree, tmp2, tmp1, 1.0);
+      if (st == 0)
+        for (long i = 0; i < N; ++i) vec_np[i] = vec_n[i] + w[0] * tmp1[i];
+      else
+        for (long i = 0; i < N; ++i) vec_np[i] += w[st] * tmp1[i];
+      if (st < 3)
+        for (long i = 0; i < N; ++i) tmp2[i] = vec_n[i] + cfac[st] * tmp1[i];
+    }
+  } else if (integrator == OX_LSRK45R2) {
+    /* :235-283 */
+    const double a[5] = {970286171893. / 4311952581923., 6584761158862. / 12103376702013.,
+                         2251764453980. / 15575788980749., 26877169314380. / 34165994151039.,
+                         0.0};
+    const double b[5] = {1153189308089. / 22510343858157., 1772645290293. / 4653164025191.,
+                         -1672844663538. / 4480602732383., 2114624349019. / 3568978502595.,
+                         5198255086312. / 14908931495163.};
+    lsrk_2reg(m, degree, 5, a, b, dt, vec_n, vec_np, tmp1);
+  } else if (integrator == OX_LSRK33R2) {
+    /* :200-231 */
+    const double a[3] = {0.755726351946097, 0.386954477304099, 0.0};
+    const double b[3] = {0.245170287303492, 0.184896052186740, 0.569933660509768};
+    lsrk_2reg(m, degree, 3, a, b, dt, vec_n, vec_np, tmp1);
+  } else if (integrator == OX_LSRK59R2) {
+    /* :286-363 */
+    const double a[9] = {1107026461565. / 5417078080134.,   38141181049399. / 41724347789894.,
+                         493273079041. / 11940823631197.,   1851571280403. / 6147804934346.,
+                         11782306865191. / 62590030070788., 9452544825720. / 13648368537481.,
+                         4435885630781. / 26285702406235.,  2357909744247. / 11371140753790.,
+                         0.0};
+    const double b[9] = {2274579626619. / 23610510767302., 693987741272. / 12394497460941.,
+                         -347131529483. / 15096185902911., 1144057200723. / 32081666971178.,
+                         1562491064753. / 11797114684756., 13113619727965. / 44346030145118.,
+                         393957816125. / 7825732611452.,   720647959663. / 6565743875477.,
+                         3559252274877. / 14424734981077.};
+    lsrk_2reg(m, degree, 9, a, b, dt, vec_n, vec_np, tmp1);
+  } else if (integrator == OX_ADER) {
+    /* ArbitraryHighOrderDG (:37-50): vec_np = vec_n - apply_ader(vec_n);
+     * tmp1 and tmp2 must be adjacent (2N scratch) -> use both explicitly */
+    double *scr = (double *)malloc(sizeof(double) * 2 * N);
+    ox_apply_ader(m, degree, vec_n, vec_np, dt, ader_recon, scr);
+    free(scr);
+    for (long i = 0; i < N; ++i) vec_np[i] = -vec_np[i] + vec_n[i];
+    (void)tmp1;
+    (void)tmp2;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* exact solution, projection, norms                                          */
+/* ------------------------------------------------------------------------ */
+
+/* Superposition of membrane modes; for one mode with m_d = M and rho=c=1 it is
+ * input_exact_solution (include/input_parameters.h:94-128). */
+double ox_exact_value(const ox_exact *e, const double *x, double t, int component) {
+  const int dim = e->dim;
+  double val = 0.0;
+  for (int k = 0; k < e->n_modes; ++k) {
+    double mm = 0.0;
+    for (int d = 0; d < dim; ++d) mm += (double)e->m[k][d] * e->m[k][d];
+    mm = sqrt(mm);
+    double omega = e->c * M_PI * mm;
+    double v;
+    if (component == dim)
+      v = e->amp[k] * cos(omega * t);
+    else
+      v = -e->amp[k] * e->m[k][component] / (e->rho * e->c * mm) * sin(omega * t);
+    for (int d = 0; d < dim; ++d)
+      v *= (d != component) ? sin(e->m[k][d] * M_PI * x[d]) : cos(e->m[k][d] * M_PI * x[d]);
+    val += v;
+  }
+  return val;
+}
+
+/* project_initial_field (:902-935): with the Gauss-collocated basis the
+ * QGauss(k+1) L2 projection is the interpolation at the mapped Gauss points. */
+void ox_project(const ox_mesh *m, int degree, const ox_exact *e, double t, double *dst) {
+  ox_basis b;
+  ox_basis_init(&b, degree);
+  const int dim = m->dim, n = b.n, nf = dim + 1, nd = ipow(n, dim), nv = 1 << dim;
+#pragma omp parallel for schedule(static)
+  for (int c = 0; c < m->n_cells; ++c) {
+    const double *V = m->vertices + (long)c * nv * dim;
+    for (int q = 0; q < nd; ++q) {
+      double xi[3], x[3];
+      int t2 = q;
+      for (int d = 0; d < dim; ++d) {
+        xi[d] = b.x[t2 % n];
+        t2 /= n;
+      }
+      multilinear_point(dim, V, xi, x);
+      for (int k = 0; k < nf; ++k) dst[((long)c * nf + k) * nd + q] = ox_exact_value(e, x, t, k);
+    }
+  }
+}
+
+/* VectorTools::integrate_difference with QGauss(k+2) (source/explicit_wave.cc:452-496) */
+void ox_l2_errors(const ox_mesh *m, int degree, const ox_exact *e, double t, const double *u,
+                  double *out) {
+  ox_basis b;
+  ox_basis_init(&b, degree);
+  const int dim = m->dim, n = b.n, ne = n + 1, nf = dim + 1, nd = ipow(n, dim),
+            nde = ipow(ne, dim), nv = 1 << dim;
+  double xe[OX_MAX_N], we[OX_MAX_N], I[OX_MAX_N * OX_MAX_N];
+  ox_gauss(ne, xe, we);
+  for (int q = 0; q < ne; ++q) ox_lagrange(n, b.x, xe[q], I + q * n, NULL);
+  double ep = 0.0, ev = 0.0, mp = 0.0;
+#pragma omp parallel reduction(+ : ep, ev, mp)
+  {
+    double *a = (double *)malloc(sizeof(double) * nde);
+    double *bb = (double *)malloc(sizeof(double) * nde);
+    double *ue = (double *)malloc(sizeof(double) * nf * nde);
+#pragma omp for schedule(static)
+    for (int c = 0; c < m->n_cells; ++c) {
+      const double *V = m->vertices + (long)c * nv * dim;
+      for (int k = 0; k < nf; ++k) {
+        int szc[3] = {n, n, n};
+        const double *in = u + ((long)c * nf + k) * nd;
+        double *bufs[2] = {a, bb};
+        for (int d = 0; d < dim; ++d) {
+          double *o = (d == dim - 1) ? ue + k * nde : bufs[d & 1];
+          tensor_apply(dim, szc, d, I, ne, in, o, 0, 0);
+          szc[d] = ne;
+          in = o;
+        }
+      }
+      for (int q = 0; q < nde; ++q) {
+        double xi[3], x[3], J[9], Ji[9], w = 1.0;
+        int t2 = q;
+        for (int d = 0; d < dim; ++d) {
+          xi[d] = xe[t2 % ne];
+          w *= we[t2 % ne];
+          t2 /= ne;
+        }
+        multilinear_point(dim, V, xi, x);
+        double jxw = fabs(multilinear_jacobian(dim, V, xi, J, Ji)) * w;
+        double pex = ox_exact_value(e, x, t, dim);
+        double dp = ue[dim * nde + q] - pex;
+        ep += dp * dp * jxw;
+        mp += pex * pex * jxw;
+        for (int d = 0; d < dim; ++d) {
+          double dv = ue[d * nde + q] - ox_exact_value(e, x, t, d);
+          ev += dv * dv * jxw;
+        }
+      }
+    }
+    free(a);
+    free(bb);
+    free(ue);
+  }
+  out[0] = sqrt(ep);
+  out[1] = sqrt(ev);
+  out[2] = sqrt(mp);
+}
+
+double ox_energy(const ox_mesh *m, int degree, const double *u) {
+  ox_basis b;
+  ox_basis_init(&b, degree);
+  const int dim = m->dim, n = b.n, nf = dim + 1, nd = ipow(n, dim), nv = 1 << dim;
+  double E = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : E)
+  for (int c = 0; c < m->n_cells; ++c) {
+    const double *V = m->vertices + (long)c * nv * dim;
+    const double rho = m->rho[c], c2 = m->speed[c] * m->speed[c];
+    for (int q = 0; q < nd; ++q) {
+      double xi[3], J[9], Ji[9], w = 1.0;
+      int t2 = q;
+      for (int d = 0; d < dim; ++d) {
+        xi[d] = b.x[t2 % n];
+        w *= b.w[t2 % n];
+        t2 /= n;
+      }
+      double jxw = fabs(multilinear_jacobian(dim, V, xi, J, Ji)) * w;
+      const double *uc = u + (long)c * nf * nd;
+      double v2 = 0.0;
+      for (int d = 0; d < dim; ++d) v2 += uc[d * nd + q] * uc[d * nd + q];
+      double p = uc[dim * nd + q];
+      E += 0.5 * jxw * (rho * v2 + p * p / (rho * c2));
+    }
+  }
+  return E;
+}
+
+/* ------------------------------------------------------------------------ */
+/* driver: WaveEquationProblem::run with TimeControl                          */
+/* ------------------------------------------------------------------------ */
+
+int ox_run(const ox_problem *P, char *log, int log_len, double *final_energy) {
+  ox_mesh *m = ox_mesh_hypercube(P->dim, P->n_intervals, P->n_refinements, P->transform,
+                                 P->transform_factor, P->boundary_id);
+  const int k = P->degree;
+  long N = ox_n_dofs(m, k);
+  /* source/parameters.cc:195 and source/explicit_wave.cc:529-545 */
+  const double cfl = P->cfl_number / pow((double)k, 1.5);
+  const double dt_cfl = cfl * m->min_vertex_distance_over_c;
+  /* TimeControl::setup (source/utilities.cc:31-46) */
+  const double T = P->final_time, tick = P->output_every_time;
+  double time = 0.0, dt = dt_cfl;
+  unsigned step = 0, max_step;
+  dt = (T - time) / fmax(round((T - time) / dt), 1.0);
+  max_step = P->max_time_steps == 0 ? (unsigned)round((T - time) / dt) : (unsigned)P->max_time_steps;
+  /* make_dofs overwrites with the unrounded CFL step (source/explicit_wave.cc:222) */
+  dt = dt_cfl;
+
+  double *sol = (double *)malloc(sizeof(double) * N);
+  double *tmp = (double *)malloc(sizeof(double) * N);
+  double *t1 = (double *)malloc(sizeof(double) * N);
+  double *t2 = (double *)malloc(sizeof(double) * N);
+  ox_project(m, k, &P->exact, 0.0, sol);
+  int pos = 0;
+  if (log && log_len > 0) log[0] = 0;
+#define OX_LOG()                                                                           \
+  do {                                                                                     \
+    double err[3];                                                                         \
+    ox_l2_errors(m, k, &P->exact, time, sol, err);                                         \
+    if (log && pos < log_len)                                                              \
+      pos += snprintf(log + pos, log_len - pos, "%u %.17g %.10e %.10e %.10e\n", step, time, \
+                      err[0], err[1], err[2]);                                             \
+  } while (0)
+  OX_LOG();
+  /* while (!done()) : time >= final_time || step > max_step (utilities.cc:84-87) */
+  while (!(time >= T || step > max_step)) {
+    ++step;
+    time += dt;
+    double *sw = tmp;
+    tmp = sol;
+    sol = sw;
+    ox_time_step(m, k, P->integrator, dt, tmp, sol, t1, t2, P->ader_recon);
+    /* TimeControl::at_tick (utilities.cc:89-95) */
+    if ((int)(time / tick) != (int)((time - dt) / tick)) OX_LOG();
+  }
+#undef OX_LOG
+  if (final_energy) *final_energy = ox_energy(m, k, sol);
+  free(sol);
+  free(tmp);
+  free(t1);
+  free(t2);
+  ox_mesh_free(m);
+  return (int)step;
+}

@@ -1,0 +1,147 @@
+/*
+ * exwave_oracle.h -- CPU FP64 restatement of ExWave's hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP
+ * implementation in exwave_amd/: only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it.  The product path never calls it.
+ *
+ * What it restates (reference = kronbichler/exwave, paths relative to the
+ * reference root):
+ *   - DG operator M^-1 A: source/wave_equation_operations.cc:404-685
+ *     (evaluate_cell, evaluate_inner_face, evaluate_boundary_face, inverse mass)
+ *   - RKVectorUpdater / low-storage RK: source/time_integrators.cc:107-363
+ *   - ExplicitEuler / ClassRK4: source/time_integrators.cc:24-102
+ *   - ADER Cauchy-Kowalevski: source/wave_equation_operations.cc:939-1381
+ *   - initial projection: source/wave_equation_operations.cc:902-935
+ *   - L2 errors: source/explicit_wave.cc:439-496
+ *   - mesh + transform: include/input_parameters.h:41-88
+ *   - TimeControl: source/utilities.cc:31-95, dt: source/explicit_wave.cc:529-545
+ *
+ * Discretisation: FE_DGQ(k)^(dim+1) with QGauss(k+1) integration.  The
+ * oracle stores the polynomial in the Lagrange basis collocated at the k+1
+ * Gauss points (the reference's own FE_DGQArbitraryNodes(QGauss(k+1))
+ * variant, source/explicit_wave.cc:163); the QGauss(k+1) mass matrix is then
+ * diagonal and the discrete solution is the same polynomial as the
+ * reference's up to round-off.
+ *
+ * Vector layout (shared with the HIP library): cell-major; inside a cell
+ * field-major (v_0 .. v_{dim-1}, p); inside a field the (k+1)^dim nodes in
+ * lexicographic order with xi_0 fastest.
+ */
+#ifndef EXWAVE_ORACLE_H
+#define EXWAVE_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OX_MAX_N 14 /* points per direction: degree <= 12 plus the k+2 error rule */
+
+typedef struct ox_basis {
+  int n;                           /* k+1 Gauss points per direction */
+  double x[OX_MAX_N], w[OX_MAX_N]; /* QGauss(k+1) on [0,1] */
+  double D[OX_MAX_N * OX_MAX_N];   /* D[q*n+i] = l_i'(x_q) (collocation derivative) */
+  double face[2][OX_MAX_N];        /* l_i(0), l_i(1) */
+} ox_basis;
+
+/* A mesh of 2^dim-vertex multilinear cells with conforming faces in the
+ * standard (lexicographic) orientation. Face f = 2*d + s is the face with
+ * xi_d = s. neighbor[c*2*dim+f] >= 0 is the neighbouring cell (which sees the
+ * face as its face f^1), < 0 is -(boundary id). */
+typedef struct ox_mesh {
+  int dim;
+  int n_cells;
+  double *vertices; /* n_cells * 2^dim * dim */
+  int *neighbor;    /* n_cells * 2*dim */
+  double *rho;      /* n_cells */
+  double *speed;    /* n_cells */
+  double min_vertex_distance_over_c; /* min_K min_vertex_distance(K)/c_K */
+} ox_mesh;
+
+/* Exact solution: superposition of membrane modes in a homogeneous medium
+ * (include/input_parameters.h:94-128 is the single mode m_d = m, rho=c=1). */
+typedef struct ox_exact {
+  int dim;
+  int n_modes;
+  double amp[8];
+  int m[8][3];
+  double rho, c;
+} ox_exact;
+
+/* transform types for ox_mesh_hypercube */
+enum {
+  OX_TRANSFORM_NONE = 0,
+  OX_TRANSFORM_REFERENCE_SINE = 1, /* x0 += f*prod_d sin(pi x_d), include/input_parameters.h:41-50 */
+  OX_TRANSFORM_AXIS_2PI = 2,       /* x_i += f*prod_{e!=i} sin(2 pi x_e) (moves the boundary) */
+  OX_TRANSFORM_PRODUCT_2PI = 3     /* x_i += f*prod_e sin(2 pi x_e) (keeps the unit cube) */
+};
+
+void ox_basis_init(ox_basis *b, int degree);
+void ox_gauss(int n, double *x, double *w);
+/* values l_i(x) and derivatives of the Lagrange basis on nodes[0..n-1] */
+void ox_lagrange(int n, const double *nodes, double x, double *val, double *der);
+
+ox_mesh *ox_mesh_hypercube(int dim, int n_intervals, int n_refinements, int transform,
+                           double factor, int boundary_id);
+void ox_mesh_set_material(ox_mesh *m, const double *rho, const double *speed);
+void ox_mesh_free(ox_mesh *m);
+
+/* dst = M^-1 A(src) * (multiplied by boundary_fac: +1 RK, -1 second ADER sweep) */
+void ox_apply(const ox_mesh *m, int degree, const double *src, double *dst, double boundary_fac);
+
+/* RKVectorUpdater semantics, source/time_integrators.cc:128-155 */
+void ox_rk_update(long n, double f1, double f2, int is_last, const double *v1, double *v2,
+                  double *v3);
+
+/* integrators: same register semantics as the reference (vec_n is clobbered,
+ * result in vec_np); tmp1/tmp2 are scratch vectors of the same size. */
+enum {
+  OX_EXPL_EULER = 0,
+  OX_CLRK4 = 1,
+  OX_LSRK45R2 = 2,
+  OX_LSRK33R2 = 3,
+  OX_LSRK59R2 = 4,
+  OX_ADER = 5
+};
+void ox_time_step(const ox_mesh *m, int degree, int integrator, double dt, double *vec_n,
+                  double *vec_np, double *tmp1, double *tmp2, int ader_recon);
+
+/* ADER: tempsrc = Cauchy-Kowalevski time integral of src over [t1,t2] about te,
+ * recon = (optional) M^-1 A src used for the first time derivative. */
+void ox_ader_ck(const ox_mesh *m, int degree, const double *src, const double *recon, double t2,
+                double t1, double te, double *tempsrc);
+/* dst = result of WaveEquationOperationADER::apply_ader */
+void ox_apply_ader(const ox_mesh *m, int degree, const double *src, double *dst, double dt,
+                   int recon, double *scratch);
+
+double ox_exact_value(const ox_exact *e, const double *x, double t, int component);
+void ox_project(const ox_mesh *m, int degree, const ox_exact *e, double t, double *dst);
+/* out[0] = ||p_h - p||, out[1] = ||v_h - v||, out[2] = ||p|| with QGauss(k+2) */
+void ox_l2_errors(const ox_mesh *m, int degree, const ox_exact *e, double t, const double *u,
+                  double *out);
+/* energy 1/2 int rho |v|^2 + p^2/(rho c^2) with QGauss(k+1) */
+double ox_energy(const ox_mesh *m, int degree, const double *u);
+
+/* Whole solve following WaveEquationProblem::run (source/explicit_wave.cc:549-813).
+ * Writes one line per output tick into `log` (reference 'error' line format,
+ * with 'error p post' reported as -1 because post-processing is not restated).
+ * Returns the number of steps performed. */
+typedef struct ox_problem {
+  int dim, degree, n_intervals, n_refinements, transform;
+  double transform_factor;
+  int boundary_id;
+  int integrator;
+  double cfl_number; /* as in the prm file; divided by k^1.5 like source/parameters.cc:195 */
+  int max_time_steps;
+  double final_time, output_every_time;
+  int ader_recon;
+  ox_exact exact;
+} ox_problem;
+int ox_run(const ox_problem *p, char *log, int log_len, double *final_state_energy);
+
+long ox_n_dofs(const ox_mesh *m, int degree);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
