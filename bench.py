@@ -1,0 +1,183 @@
+#!/usr/bin/env python3
+"""Headline benchmark: GDoF/s per RK stage of the fused FP64 DG operator + LSRK45
+stage on one MI355X.
+
+Workload (the reference's 3D membrane at the north-star size): unit cube,
+64^3 Cartesian hexes, degree k=4 (125 nodes/cell, fields v_x, v_y, v_z, p:
+131,072,000 DoFs), rho = c = 1, boundary id 2 (p = 0), initial pressure
+sin(pi x)sin(pi y)sin(pi z) + sin(2 pi x)sin(3 pi y)sin(pi z), v = 0,
+LowStorageRK45Reg2 at CFL 0.2 (dt = 0.2/4^1.5 * 1/64).  A bench "step" is one
+LSRK45 time step = 5 fused operator+update sweeps.
+
+    python bench.py [--gpus 1] [--steps 100] [--warmup 5]
+
+Prints one JSON line (see the keys below).  `roofline` is computed for the
+fused stage kernel from algorithmic bytes (28.8 B/DoF per stage on average:
+18 FP64 vector passes per LSRK45 step, plus 64 B of per-cell data) over its
+average launch time measured with HIP events on the launch stream.
+`cpu_baseline` times the CPU oracle (a C/OpenMP restatement, not deal.II) on
+a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+PEAK_HBM_GBS = 8000.0
+N_CELLS_1D, DEGREE, DIM = 64, 4, 3
+STAGES = 5
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
+    ap.add_argument("--n", type=int, default=N_CELLS_1D, help="cells per direction")
+    ap.add_argument("--degree", type=int, default=DEGREE)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="target CPU time of the oracle sample")
+    return ap.parse_args()
+
+
+def cpu_baseline(n1d, k, target_s):
+    """Time the CPU oracle's apply + RKVectorUpdater on the same mesh and data
+    (stage by stage, as the reference executes it) until ~target_s elapsed."""
+    import ctypes as C
+
+    import numpy as np
+
+    import oracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    m = O.Mesh(DIM, n1d, 0, O.TRANSFORM_NONE, 0.0, 2)
+    N = m.n_dofs(k)
+    e = O.Exact.make(DIM, [[1, 1, 1], [2, 3, 1]])
+    vec_n = m.project(k, e, 0.0)
+    vec_np = np.zeros(N)
+    tmp = np.zeros(N)
+    L = O.lib()
+    a = [970286171893. / 4311952581923., 6584761158862. / 12103376702013.,
+         2251764453980. / 15575788980749., 26877169314380. / 34165994151039.]
+    b = [1153189308089. / 22510343858157., 1772645290293. / 4653164025191.,
+         -1672844663538. / 4480602732383., 2114624349019. / 3568978502595.,
+         5198255086312. / 14908931495163.]
+    dt = 0.2 / k ** 1.5 / n1d
+    regs = [(vec_n, vec_n, vec_np), (vec_n, vec_np, vec_n), (vec_np, vec_n, vec_np),
+            (vec_n, vec_np, vec_n), (vec_np, vec_np, vec_n)]
+    stages = 0
+    t0 = time.perf_counter()
+    while True:
+        s = stages % STAGES
+        src, v2, v3 = regs[s]
+        L.ox_apply(m._m, k, O._p(src), O._p(tmp), 1.0)
+        if s < STAGES - 1:
+            L.ox_rk_update(N, a[s] * dt, (b[s] - a[s]) * dt, 0, O._p(tmp), O._p(v2), O._p(v3))
+        else:
+            L.ox_rk_update(N, b[s] * dt, 0.0, 1, O._p(tmp), O._p(v2), O._p(v3))
+        stages += 1
+        el = time.perf_counter() - t0
+        if el >= target_s and stages >= 1:
+            break
+    return {"value": round(N * stages / el / 1e9, 4), "unit": "GDoF/s per RK stage",
+            "cores": threads, "kind": "port",
+            "sample": f"{stages} LSRK45 stages (oracle apply + RKVectorUpdater, FP64) of the "
+                      f"{n1d}^3 k={k} workload, {el:.1f} s, OpenMP {threads} threads; CPU "
+                      "restatement, not deal.II"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); the "
+              "multi-GPU sharding is a design note in DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    import exwave_amd as X
+    ctx = X.Context(0)
+    stream = torch.cuda.current_stream()
+    prec = X.FP64 if args.precision == "fp64" else X.FP32
+    n1d, k = args.n, args.degree
+    op = X.WaveOperator.hypercube(ctx, DIM, k, n1d, 0, X.TRANSFORM_NONE, 0.0, 2)
+    N = op.n_dofs
+    assert op.is_cartesian
+    modes = X.Modes.make(DIM, [[1, 1, 1], [2, 3, 1]])
+    vec_n, vec_np, work = op.new_vector(), op.new_vector(), op.new_vector()
+    op.project(modes, 0.0, vec_np)
+    dt = 0.2 / k ** 1.5 * op.min_vertex_distance_over_c
+    E0 = op.energy(vec_np)
+
+    def step():
+        nonlocal vec_n, vec_np
+        vec_n, vec_np = vec_np, vec_n  # tmp_solutions.swap(solutions), explicit_wave.cc:784
+        op.time_step(X.LSRK45R2, dt, vec_n, vec_np, work, precision=prec)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        step()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_ms = ev0.elapsed_time(ev1)
+    t_sec = gpu_ms / 1e3
+    launches = args.steps * STAGES
+    stage_ms = gpu_ms / launches
+    value = N * launches / t_sec / 1e9
+    # algorithmic bytes per launch (average over the 5 stages: 18 vectors/step)
+    n_cells = op.n_cells
+    bytes_per_launch = 18 * 8 * N / STAGES + 64 * n_cells
+    achieved = bytes_per_launch / (stage_ms / 1e3) / 1e9
+    # accuracy of the result (outside the timed region)
+    t_end = (args.warmup + args.steps) * dt
+    err = op.l2_errors(modes, t_end, vec_np)
+    E1 = op.energy(vec_np)
+    out = {
+        "metric": "GDoF/s per RK stage (3D k=4); batched solves/s + p99 latency at equal L2 err",
+        "value": round(value, 3),
+        "unit": "GDoF/s per RK stage",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(gpu_ms / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == X.FP64 else "f32 operator, f64 vectors",
+        "data": "synthetic (analytic membrane modes (1,1,1)+(2,3,1), projected)",
+        "config": {"workload": f"3D membrane, unit cube, {n1d}^3 Cartesian hexes, k={k}, "
+                               f"{N} DoFs, LSRK45R2 at CFL 0.2, rho=c=1, p=0 boundary",
+                   "n_dofs": N, "n_cells": n_cells, "degree": k, "integrator": "LSRK45R2",
+                   "dt": dt, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
+                     "traffic": None,
+                     "kernel": f"dg_operator_kernel<3,{k},{'double' if prec == X.FP64 else 'float'},cartesian>",
+                     "algorithmic_bytes_per_launch": int(bytes_per_launch),
+                     "avg_launch_ms": round(stage_ms, 5)},
+        "wall_s": round(wall, 4),
+        "l2_error_after": {"t": t_end, "p": err[0], "v": err[1], "mag_p": err[2]},
+        "energy_ratio": E1 / E0,
+    }
+    if not args.no_cpu_baseline:
+        del vec_n, vec_np, work
+        torch.cuda.empty_cache()
+        out["cpu_baseline"] = cpu_baseline(n1d, k, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,148 @@
+"""exwave_amd: MI355X-native matrix-free DG acoustic operator and explicit time
+stepping (the hot path of kronbichler/exwave) behind a C-ABI.
+
+Python API (thin, over the C-ABI of include/exwave_amd.h):
+
+    ctx = Context()                          # device 0, torch's current stream
+    op  = WaveOperator(ctx, dim, degree, vertices, neighbor, rho, speed)
+    op.apply(src, dst)                       # dst = M^-1 A src  (torch float64 tensors)
+    op.time_step(LSRK45R2, dt, vec_n, vec_np, work)   # ExplicitIntegrator::perform_time_step
+
+The C++ host mirror of the reference interface (WaveEquationOperationBase,
+ExplicitIntegrator, LowStorageRK45Reg2, ...) lives in include/exwave_amd.hpp.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+from .capi import (ADER, CLRK4, EXPL_EULER, FP32, FP64, INTEGRATORS, LSRK33R2, LSRK45R2,  # noqa
+                   LSRK59R2, TRANSFORM_AXIS_2PI, TRANSFORM_NONE, TRANSFORM_PRODUCT_2PI,
+                   TRANSFORM_REFERENCE_SINE, Epilogue, ExwError, Modes, Problem, check, load,
+                   mesh_hypercube)
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    import torch
+    assert isinstance(t, torch.Tensor) and t.dtype == torch.float64 and t.is_cuda, \
+        "device vectors are float64 torch tensors on the GPU"
+    assert t.is_contiguous()
+    return C.c_void_p(t.data_ptr())
+
+
+class Context:
+    """exw_ctx bound to a device and (by default) torch's current stream."""
+
+    def __init__(self, device: int = 0, stream=None):
+        import torch
+        L = load()
+        if stream is None:
+            stream = torch.cuda.current_stream(device).cuda_stream
+        h = C.c_void_p()
+        check(L.exw_ctx_create(device, C.c_void_p(stream), C.byref(h)))
+        self.h = h
+        self.device = device
+
+    def sync(self):
+        check(load().exw_sync(self.h))
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None:
+            load().exw_ctx_destroy(self.h)
+            self.h = None
+
+
+class WaveOperator:
+    """WaveEquationOperation<dim,k> on the device (setup + apply + integrators)."""
+
+    def __init__(self, ctx: Context, dim: int, degree: int, vertices, neighbor, rho, speed):
+        import numpy as np
+        L = load()
+        V = np.ascontiguousarray(vertices, dtype=np.float64)
+        nb = np.ascontiguousarray(neighbor, dtype=np.int32)
+        r = np.ascontiguousarray(rho, dtype=np.float64)
+        c = np.ascontiguousarray(speed, dtype=np.float64)
+        n_cells = V.shape[0]
+        assert V.shape == (n_cells, 1 << dim, dim) and nb.shape == (n_cells, 2 * dim)
+        assert r.shape == (n_cells,) and c.shape == (n_cells,)
+        h = C.c_void_p()
+        dp, ip = C.POINTER(C.c_double), C.POINTER(C.c_int)
+        check(L.exw_op_create(ctx.h, dim, degree, n_cells, V.ctypes.data_as(dp),
+                              nb.ctypes.data_as(ip), r.ctypes.data_as(dp), c.ctypes.data_as(dp),
+                              C.byref(h)))
+        self.h, self.ctx, self.dim, self.degree, self.n_cells = h, ctx, dim, degree, n_cells
+        self.n_dofs = int(L.exw_op_n_dofs(h))
+        self.is_cartesian = bool(L.exw_op_is_cartesian(h))
+        self.min_vertex_distance_over_c = float(L.exw_op_min_vertex_distance_over_c(h))
+
+    @classmethod
+    def hypercube(cls, ctx, dim, degree, n_intervals, n_refinements=0,
+                  transform=TRANSFORM_NONE, factor=0.0, boundary_id=2, rho=1.0, speed=1.0):
+        import numpy as np
+        V, nb = mesh_hypercube(dim, n_intervals, n_refinements, transform, factor, boundary_id)
+        n = V.shape[0]
+        r = np.broadcast_to(np.asarray(rho, dtype=np.float64), (n,)).copy()
+        c = np.broadcast_to(np.asarray(speed, dtype=np.float64), (n,)).copy()
+        op = cls(ctx, dim, degree, V, nb, r, c)
+        op.vertices, op.neighbor, op.rho, op.speed = V, nb, r, c
+        return op
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None:
+            load().exw_op_destroy(self.h)
+            self.h = None
+
+    def new_vector(self, n=1):
+        import torch
+        return torch.zeros(n * self.n_dofs, dtype=torch.float64, device=f"cuda:{self.ctx.device}")
+
+    def apply(self, src, dst, precision=FP64):
+        check(load().exw_op_apply(self.h, _ptr(src), _ptr(dst), precision))
+
+    def apply_fused(self, src, b=None, o1=None, a1=1.0, c=None, o2=None, a2=0.0,
+                    o2_from_o1=False, precision=FP64):
+        e = Epilogue(_ptr(b), _ptr(o1), a1, _ptr(c), _ptr(o2), a2, int(o2_from_o1))
+        check(load().exw_op_apply_fused(self.h, _ptr(src), C.byref(e), precision))
+
+    def rk_stage(self, src, b, y_out, b_out, f1, f2, is_last, precision=FP64):
+        check(load().exw_rk_stage(self.h, _ptr(src), _ptr(b), _ptr(y_out), _ptr(b_out), f1, f2,
+                                  int(is_last), precision))
+
+    def ader_ck(self, src, recon, t2, t1, te, out, precision=FP64):
+        check(load().exw_ader_ck(self.h, _ptr(src), _ptr(recon), t2, t1, te, _ptr(out),
+                                 precision))
+
+    def time_step(self, integrator, dt, vec_n, vec_np, work=None, ader_recon=False,
+                  precision=FP64):
+        check(load().exw_time_step(self.h, integrator, dt, _ptr(vec_n), _ptr(vec_np), _ptr(work),
+                                   int(ader_recon), precision))
+
+    @staticmethod
+    def work_vectors(integrator):
+        return int(load().exw_time_step_work_vectors(integrator))
+
+    def project(self, modes: Modes, t, dst):
+        check(load().exw_project(self.h, C.byref(modes), t, _ptr(dst)))
+
+    def l2_errors(self, modes: Modes, t, u):
+        out = (C.c_double * 3)()
+        check(load().exw_l2_errors(self.h, C.byref(modes), t, _ptr(u), out))
+        return list(out)
+
+    def energy(self, u):
+        out = C.c_double(0.0)
+        check(load().exw_energy(self.h, _ptr(u), C.byref(out)))
+        return out.value
+
+
+def run(ctx: Context, problem: Problem):
+    """exw_run: returns (n_steps, [(step, time, err_p, err_v, mag_p)], wall_seconds)."""
+    buf = C.create_string_buffer(1 << 16)
+    wall = C.c_double(0.0)
+    steps = check(load().exw_run(ctx.h, C.byref(problem), buf, len(buf), C.byref(wall)))
+    rows = []
+    for line in buf.value.decode().strip().splitlines():
+        s, t, ep, ev, mp = line.split()
+        rows.append((int(s), float(t), float(ep), float(ev), float(mp)))
+    return steps, rows, wall.value

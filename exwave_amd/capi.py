@@ -1,0 +1,137 @@
+"""ctypes binding of the C-ABI in include/exwave_amd.h.
+
+The library is built in-tree by ``python -m exwave_amd.build`` into
+``exwave_amd/_lib/libexwave_amd.so``.  There is no fallback: if the library
+is missing every entry point raises.  Device vectors are passed as raw
+pointers; torch CUDA/HIP tensors are the usual owners (``tensor.data_ptr()``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_lib", "libexwave_amd.so")
+
+EXW_OK, EXW_EINVAL, EXW_EDEVICE, EXW_ENOMEM, EXW_EUNSUP = 0, -1, -2, -3, -4
+FP64, FP32 = 0, 1
+EXPL_EULER, CLRK4, LSRK45R2, LSRK33R2, LSRK59R2, ADER = range(6)
+INTEGRATORS = {"ExplEuler": EXPL_EULER, "clRK4": CLRK4, "LSRK45R2": LSRK45R2,
+               "LSRK33R2": LSRK33R2, "LSRK59R2": LSRK59R2, "ADER": ADER}
+TRANSFORM_NONE, TRANSFORM_REFERENCE_SINE, TRANSFORM_AXIS_2PI, TRANSFORM_PRODUCT_2PI = range(4)
+
+
+class ExwError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"exwave_amd error {code}: {msg}")
+        self.code = code
+
+
+class Modes(C.Structure):
+    _fields_ = [("n_modes", C.c_int), ("amp", C.c_double * 8), ("m", (C.c_int * 3) * 8),
+                ("rho", C.c_double), ("c", C.c_double)]
+
+    @staticmethod
+    def make(dim, modes, amps=None, rho=1.0, c=1.0):
+        e = Modes()
+        e.n_modes = len(modes)
+        for k, mm in enumerate(modes):
+            e.amp[k] = 1.0 if amps is None else float(amps[k])
+            for d in range(dim):
+                e.m[k][d] = int(mm[d])
+        e.rho, e.c = float(rho), float(c)
+        return e
+
+    @staticmethod
+    def reference(dim, membrane_modes):
+        """ExactSolution of include/input_parameters.h:94-128."""
+        return Modes.make(dim, [[membrane_modes] * dim])
+
+
+class Epilogue(C.Structure):
+    _fields_ = [("b", C.c_void_p), ("o1", C.c_void_p), ("a1", C.c_double), ("c", C.c_void_p),
+                ("o2", C.c_void_p), ("a2", C.c_double), ("o2_from_o1", C.c_int)]
+
+
+class Problem(C.Structure):
+    _fields_ = [("dim", C.c_int), ("degree", C.c_int), ("n_intervals", C.c_int),
+                ("n_refinements", C.c_int), ("transform", C.c_int),
+                ("transform_factor", C.c_double), ("boundary_id", C.c_int),
+                ("integrator", C.c_int), ("cfl_number", C.c_double),
+                ("max_time_steps", C.c_int), ("final_time", C.c_double),
+                ("output_every_time", C.c_double), ("ader_recon", C.c_int),
+                ("precision", C.c_int), ("modes", Modes)]
+
+
+# (name, restype, argtypes) of every exported entry point
+_VP, _D, _I, _DP, _IP = C.c_void_p, C.c_double, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int)
+SIGNATURES = [
+    ("exw_last_error", C.c_char_p, []),
+    ("exw_version", C.c_char_p, []),
+    ("exw_ctx_create", _I, [_I, _VP, C.POINTER(_VP)]),
+    ("exw_ctx_set_stream", _I, [_VP, _VP]),
+    ("exw_ctx_destroy", _I, [_VP]),
+    ("exw_sync", _I, [_VP]),
+    ("exw_alloc", _I, [_VP, C.c_size_t, C.POINTER(_VP)]),
+    ("exw_free", _I, [_VP, _VP]),
+    ("exw_memcpy_h2d", _I, [_VP, _VP, _VP, C.c_size_t]),
+    ("exw_memcpy_d2h", _I, [_VP, _VP, _VP, C.c_size_t]),
+    ("exw_op_create", _I, [_VP, _I, _I, _I, _DP, _IP, _DP, _DP, C.POINTER(_VP)]),
+    ("exw_op_destroy", _I, [_VP]),
+    ("exw_op_n_dofs", C.c_int64, [_VP]),
+    ("exw_op_n_cells", _I, [_VP]),
+    ("exw_op_is_cartesian", _I, [_VP]),
+    ("exw_op_min_vertex_distance_over_c", _D, [_VP]),
+    ("exw_op_apply", _I, [_VP, _VP, _VP, _I]),
+    ("exw_op_apply_fused", _I, [_VP, _VP, C.POINTER(Epilogue), _I]),
+    ("exw_rk_stage", _I, [_VP, _VP, _VP, _VP, _VP, _D, _D, _I, _I]),
+    ("exw_ader_ck", _I, [_VP, _VP, _VP, _D, _D, _D, _VP, _I]),
+    ("exw_ader_step", _I, [_VP, _VP, _VP, _VP, _D, _I, _I]),
+    ("exw_time_step", _I, [_VP, _I, _D, _VP, _VP, _VP, _I, _I]),
+    ("exw_time_step_work_vectors", _I, [_I]),
+    ("exw_project", _I, [_VP, C.POINTER(Modes), _D, _VP]),
+    ("exw_l2_errors", _I, [_VP, C.POINTER(Modes), _D, _VP, _DP]),
+    ("exw_energy", _I, [_VP, _VP, _DP]),
+    ("exw_mesh_hypercube", _I, [_I, _I, _I, _I, _D, _I, _IP, _DP, _IP]),
+    ("exw_run", _I, [_VP, C.POINTER(Problem), C.c_char_p, _I, _DP]),
+]
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Load the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is missing: build it with `python -m exwave_amd.build` "
+                          "(there is no CPU fallback)")
+    lib = C.CDLL(path)
+    for name, res, args in SIGNATURES:
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc):
+    if rc < 0:
+        raise ExwError(rc, load().exw_last_error().decode())
+    return rc
+
+
+def mesh_hypercube(dim, n_intervals, n_refinements=0, transform=TRANSFORM_NONE, factor=0.0,
+                   boundary_id=2):
+    """input_geometry_description: returns (vertices[n,2^dim,dim], neighbor[n,2*dim])."""
+    import numpy as np
+    L = load()
+    n = C.c_int(0)
+    check(L.exw_mesh_hypercube(dim, n_intervals, n_refinements, transform, factor, boundary_id,
+                               C.byref(n), None, None))
+    V = np.zeros((n.value, 1 << dim, dim))
+    nb = np.zeros((n.value, 2 * dim), dtype=np.int32)
+    check(L.exw_mesh_hypercube(dim, n_intervals, n_refinements, transform, factor, boundary_id,
+                               C.byref(n), V.ctypes.data_as(_DP), nb.ctypes.data_as(_IP)))
+    return V, nb

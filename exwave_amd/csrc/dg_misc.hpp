@@ -1,0 +1,175 @@
+// dg_misc.hpp -- setup and norm kernels: initial projection, L2 errors, energy.
+//
+// project_initial_field (source/wave_equation_operations.cc:902-935): with the
+// Gauss-collocated basis the QGauss(k+1) L2 projection is the value of the
+// function at the mapped Gauss points.
+// L2 errors (VectorTools::integrate_difference with QGauss(k+2),
+// source/explicit_wave.cc:452-496): per cell, interpolate to the k+2 Gauss
+// points, compare with the exact modes, reduce per workgroup; a second
+// single-group pass sums the partials in a fixed order (deterministic).
+#pragma once
+#include "dg_common.hpp"
+
+namespace exw {
+
+struct DevModes {
+  int n_modes;
+  double amp[8];
+  int m[8][3];
+  double rho, c;
+};
+
+template <int DIM>
+__device__ __forceinline__ double modes_value(const DevModes &e, const double *x, double t,
+                                              int comp) {
+  constexpr double PI = 3.14159265358979323846;
+  double val = 0.0;
+  for (int k = 0; k < e.n_modes; ++k) {
+    double mm = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) mm += double(e.m[k][d]) * e.m[k][d];
+    mm = sqrt(mm);
+    const double omega = e.c * PI * mm;
+    double v = comp == DIM ? e.amp[k] * cos(omega * t)
+                           : -e.amp[k] * e.m[k][comp] / (e.rho * e.c * mm) * sin(omega * t);
+#pragma unroll
+    for (int d = 0; d < DIM; ++d)
+      v *= d != comp ? sin(e.m[k][d] * PI * x[d]) : cos(e.m[k][d] * PI * x[d]);
+    val += v;
+  }
+  return val;
+}
+
+template <int DIM, int K>
+__global__ void project_kernel(const DevMesh mesh, const double *__restrict__ tab,
+                               const DevModes modes, double t, double *__restrict__ dst) {
+  using S = Shape<DIM, K>;
+  constexpr int N = S::N, ND = S::ND, NF = S::NF;
+  const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (size_t)mesh.n_cells * ND) return;
+  const int cell = int(gid / ND), q = int(gid % ND);
+  const int qi[3] = {q % N, (q / N) % N, q / (N * N)};
+  double xi[3], x[3];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) xi[d] = tab[TabOff<N>::X + qi[d]];
+  multilinear_point<DIM>(mesh.vertices + (size_t)cell * (1 << DIM) * DIM, xi, x);
+#pragma unroll
+  for (int f = 0; f < NF; ++f) dst[((size_t)cell * NF + f) * ND + q] = modes_value<DIM>(modes, x, t, f);
+}
+
+// errtab: xe[N+1], we[N+1], I[(N+1)*N] (I[q*N+i] = l_i(xe_q))
+template <int DIM, int K>
+__global__ void __launch_bounds__(256)
+    l2_error_kernel(const DevMesh mesh, const double *__restrict__ errtab, const DevModes modes,
+                    double t, const double *__restrict__ u, double *__restrict__ partial) {
+  using S = Shape<DIM, K>;
+  constexpr int N = S::N, ND = S::ND, NF = S::NF;
+  constexpr int NE = N + 1;
+  constexpr int NDE = DIM == 2 ? NE * NE : NE * NE * NE;
+  __shared__ double red[3][256];
+  double ep = 0.0, ev = 0.0, mp = 0.0;
+  const double *xe = errtab, *we = errtab + NE, *I = errtab + 2 * NE;
+  const size_t total = (size_t)mesh.n_cells * NDE;
+  for (size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x; gid < total;
+       gid += (size_t)gridDim.x * blockDim.x) {
+    const int cell = int(gid / NDE), q = int(gid % NDE);
+    const int qi[3] = {q % NE, (q / NE) % NE, q / (NE * NE)};
+    double xi[3], x[3], J[9], Ji[9], w = 1.0;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      xi[d] = xe[qi[d]];
+      w *= we[qi[d]];
+    }
+    const double *V = mesh.vertices + (size_t)cell * (1 << DIM) * DIM;
+    multilinear_point<DIM>(V, xi, x);
+    const double jxw = fabs(multilinear_jacobian<DIM>(V, xi, J, Ji)) * w;
+    const double *uc = u + (size_t)cell * NF * ND;
+    double val[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) val[f] = 0.0;
+    for (int n = 0; n < ND; ++n) {
+      const int ni[3] = {n % N, (n / N) % N, n / (N * N)};
+      double s = I[qi[0] * N + ni[0]] * I[qi[1] * N + ni[1]];
+      if (DIM == 3) s *= I[qi[2] * N + ni[2]];
+#pragma unroll
+      for (int f = 0; f < NF; ++f) val[f] += s * uc[f * ND + n];
+    }
+    const double pex = modes_value<DIM>(modes, x, t, DIM);
+    ep += (val[DIM] - pex) * (val[DIM] - pex) * jxw;
+    mp += pex * pex * jxw;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      const double dv = val[d] - modes_value<DIM>(modes, x, t, d);
+      ev += dv * dv * jxw;
+    }
+  }
+  red[0][threadIdx.x] = ep;
+  red[1][threadIdx.x] = ev;
+  red[2][threadIdx.x] = mp;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+      for (int k = 0; k < 3; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 3; ++k) partial[blockIdx.x * 3 + k] = red[k][0];
+}
+
+template <int DIM, int K>
+__global__ void __launch_bounds__(256)
+    energy_kernel(const DevMesh mesh, const double *__restrict__ tab,
+                  const double *__restrict__ u, double *__restrict__ partial) {
+  using S = Shape<DIM, K>;
+  constexpr int N = S::N, ND = S::ND, NF = S::NF;
+  __shared__ double red[256];
+  double E = 0.0;
+  const size_t total = (size_t)mesh.n_cells * ND;
+  for (size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x; gid < total;
+       gid += (size_t)gridDim.x * blockDim.x) {
+    const int cell = int(gid / ND), q = int(gid % ND);
+    const int qi[3] = {q % N, (q / N) % N, q / (N * N)};
+    double xi[3], J[9], Ji[9], w = 1.0;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      xi[d] = tab[TabOff<N>::X + qi[d]];
+      w *= tab[TabOff<N>::W + qi[d]];
+    }
+    const double *V = mesh.vertices + (size_t)cell * (1 << DIM) * DIM;
+    const double jxw = fabs(multilinear_jacobian<DIM>(V, xi, J, Ji)) * w;
+    const double *uc = u + (size_t)cell * NF * ND;
+    double v2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) v2 += uc[d * ND + q] * uc[d * ND + q];
+    const double p = uc[DIM * ND + q];
+    const double rho = mesh.rho[cell], c = mesh.speed[cell];
+    E += 0.5 * jxw * (rho * v2 + p * p / (rho * c * c));
+  }
+  red[threadIdx.x] = E;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// fixed-order final sum of `n` partial vectors of width `w`
+static __global__ void __launch_bounds__(256)
+    sum_partials_kernel(const double *__restrict__ partial, int n, int w, double *__restrict__ out) {
+  __shared__ double red[256];
+  for (int k = 0; k < w; ++k) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) s += partial[i * w + k];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+      if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[k] = red[0];
+    __syncthreads();
+  }
+}
+
+}  // namespace exw

@@ -1,0 +1,566 @@
+// exwave_capi.hip -- C-ABI of the MI355X ExWave hot path (include/exwave_amd.h).
+//
+// Host side of the library: operator setup (replaces WaveEquationOperation::
+// setup / reset_data_vectors), the fused operator + RK stage launches, the
+// ADER step and the explicit integrators (ExplicitIntegrator::perform_time_step
+// of source/time_integrators.cc), projection and norms.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/exwave_amd.h"
+#include "basis.hpp"
+#include "launchers.hpp"
+
+using namespace exw;
+
+namespace {
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define EXW_HIP(call)                                                                     \
+  do {                                                                                    \
+    hipError_t e_ = (call);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return fail(EXW_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_));        \
+  } while (0)
+
+const DimKernels &kernels(int dim) { return dim == 2 ? kernels_2d() : kernels_3d(); }
+}  // namespace
+
+struct exw_ctx_s {
+  int device;
+  hipStream_t stream;
+};
+
+struct exw_op_s {
+  exw_ctx ctx;
+  int dim, degree, n_cells;
+  bool cart;
+  double minvd;
+  int *d_neighbor = nullptr;
+  double *d_rho = nullptr, *d_speed = nullptr, *d_h = nullptr, *d_vertices = nullptr;
+  double *d_tab = nullptr, *d_errtab = nullptr, *d_cktab = nullptr;
+  double *d_partial = nullptr, *d_sum = nullptr;
+  int n_partial = 0;
+  CkTab ck{};
+  DevMesh view() const {
+    DevMesh m;
+    m.n_cells = n_cells;
+    m.neighbor = d_neighbor;
+    m.rho = d_rho;
+    m.speed = d_speed;
+    m.h = d_h;
+    m.vertices = d_vertices;
+    return m;
+  }
+  int64_t n_dofs() const {
+    int64_t nd = 1;
+    for (int d = 0; d < dim; ++d) nd *= degree + 1;
+    return (int64_t)n_cells * (dim + 1) * nd;
+  }
+};
+
+extern "C" {
+
+const char *exw_last_error(void) { return g_err.c_str(); }
+const char *exw_version(void) { return "exwave_amd 0.1 (gfx950)"; }
+
+int exw_ctx_create(int device, void *stream, exw_ctx *out) {
+  if (!out) return fail(EXW_EINVAL, "exw_ctx_create: out is NULL");
+  int n = 0;
+  EXW_HIP(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return fail(EXW_EINVAL, "exw_ctx_create: bad device id");
+  EXW_HIP(hipSetDevice(device));
+  exw_ctx c = new exw_ctx_s;
+  c->device = device;
+  c->stream = (hipStream_t)stream;
+  *out = c;
+  return EXW_OK;
+}
+
+int exw_ctx_set_stream(exw_ctx ctx, void *stream) {
+  if (!ctx) return fail(EXW_EINVAL, "null context");
+  ctx->stream = (hipStream_t)stream;
+  return EXW_OK;
+}
+
+int exw_ctx_destroy(exw_ctx ctx) {
+  delete ctx;
+  return EXW_OK;
+}
+
+int exw_sync(exw_ctx ctx) {
+  if (!ctx) return fail(EXW_EINVAL, "null context");
+  EXW_HIP(hipStreamSynchronize(ctx->stream));
+  return EXW_OK;
+}
+
+int exw_alloc(exw_ctx ctx, size_t bytes, void **dptr) {
+  if (!ctx || !dptr) return fail(EXW_EINVAL, "exw_alloc: null argument");
+  EXW_HIP(hipSetDevice(ctx->device));
+  if (hipMalloc(dptr, bytes) != hipSuccess) return fail(EXW_ENOMEM, "exw_alloc: hipMalloc failed");
+  return EXW_OK;
+}
+
+int exw_free(exw_ctx ctx, void *dptr) {
+  if (!ctx) return fail(EXW_EINVAL, "exw_free: null context");
+  EXW_HIP(hipFree(dptr));
+  return EXW_OK;
+}
+
+int exw_memcpy_h2d(exw_ctx ctx, void *dst, const void *src, size_t bytes) {
+  EXW_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  EXW_HIP(hipStreamSynchronize(ctx->stream));
+  return EXW_OK;
+}
+
+int exw_memcpy_d2h(exw_ctx ctx, void *dst, const void *src, size_t bytes) {
+  EXW_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  EXW_HIP(hipStreamSynchronize(ctx->stream));
+  return EXW_OK;
+}
+
+static void free_op(exw_op op) {
+  if (!op) return;
+  hipFree(op->d_neighbor);
+  hipFree(op->d_rho);
+  hipFree(op->d_speed);
+  hipFree(op->d_h);
+  hipFree(op->d_vertices);
+  hipFree(op->d_tab);
+  hipFree(op->d_errtab);
+  hipFree(op->d_cktab);
+  hipFree(op->d_partial);
+  hipFree(op->d_sum);
+  delete op;
+}
+
+extern "C++" {
+template <typename T>
+static int upload(exw_op op, T **dptr, const T *host, size_t n) {
+  if (hipMalloc((void **)dptr, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess)
+    return fail(EXW_ENOMEM, "exw_op_create: hipMalloc failed");
+  if (n) EXW_HIP(hipMemcpy(*dptr, host, n * sizeof(T), hipMemcpyHostToDevice));
+  return EXW_OK;
+}
+}  // extern "C++"
+
+int exw_op_create(exw_ctx ctx, int dim, int degree, int n_cells, const double *vertices,
+                  const int *neighbor, const double *rho, const double *speed, exw_op *out) {
+  if (!ctx || !out || !vertices || !neighbor || !rho || !speed)
+    return fail(EXW_EINVAL, "exw_op_create: null argument");
+  if (dim != 2 && dim != 3) return fail(EXW_EINVAL, "exw_op_create: dim must be 2 or 3");
+  if (degree < 1 || degree > MAX_DEGREE)
+    return fail(EXW_EUNSUP, "exw_op_create: degree " + std::to_string(degree) +
+                                " not compiled (1.." + std::to_string(MAX_DEGREE) + ")");
+  if (n_cells < 1) return fail(EXW_EINVAL, "exw_op_create: n_cells < 1");
+  const int nv = 1 << dim, nface = 2 * dim, N = degree + 1;
+  // validate connectivity and materials (source/wave_equation_operations.cc:615-616)
+  for (int64_t i = 0; i < (int64_t)n_cells * nface; ++i) {
+    const int nb = neighbor[i];
+    if (nb >= n_cells || (nb < 0 && (nb < -3 || nb > -1)))
+      return fail(EXW_EINVAL, "set your boundary ids correctly: 1 - soft wall, 2 - hard wall, "
+                              "3 - first order ABC (or neighbour index out of range)");
+  }
+  for (int c = 0; c < n_cells; ++c)
+    if (!(rho[c] > 0.0) || !(speed[c] > 0.0))
+      return fail(EXW_EINVAL, "exw_op_create: density and speed must be positive");
+  EXW_HIP(hipSetDevice(ctx->device));
+  exw_op op = new exw_op_s;
+  op->ctx = ctx;
+  op->dim = dim;
+  op->degree = degree;
+  op->n_cells = n_cells;
+  // geometry classification: axis-aligned boxes -> Cartesian kernels
+  bool cart = true;
+  std::vector<double> h((size_t)n_cells * dim);
+  double minvd = 1e300;
+  for (int c = 0; c < n_cells; ++c) {
+    const double *V = vertices + (size_t)c * nv * dim;
+    for (int d = 0; d < dim; ++d) h[(size_t)c * dim + d] = V[(1 << d) * dim + d] - V[d];
+    for (int v = 0; v < nv && cart; ++v)
+      for (int i = 0; i < dim; ++i) {
+        const double want = V[i] + (((v >> i) & 1) ? h[(size_t)c * dim + i] : 0.0);
+        if (std::fabs(V[v * dim + i] - want) > 1e-13 * std::fabs(h[(size_t)c * dim + i]))
+          cart = false;
+      }
+    // minimum vertex distance over all vertex pairs / c (source/explicit_wave.cc:529-545)
+    for (int a = 0; a < nv; ++a)
+      for (int b = a + 1; b < nv; ++b) {
+        double s = 0.0;
+        for (int d = 0; d < dim; ++d) {
+          const double df = V[a * dim + d] - V[b * dim + d];
+          s += df * df;
+        }
+        minvd = std::min(minvd, std::sqrt(s) / speed[c]);
+      }
+  }
+  op->cart = cart;
+  op->minvd = minvd;
+  int rc;
+  if ((rc = upload(op, &op->d_neighbor, neighbor, (size_t)n_cells * nface)) ||
+      (rc = upload(op, &op->d_rho, rho, (size_t)n_cells)) ||
+      (rc = upload(op, &op->d_speed, speed, (size_t)n_cells)) ||
+      (rc = upload(op, &op->d_h, h.data(), h.size())) ||
+      (rc = upload(op, &op->d_vertices, vertices, (size_t)n_cells * nv * dim))) {
+    free_op(op);
+    return rc;
+  }
+  // operator basis table (layout: TabOff in dg_common.hpp)
+  std::vector<double> x, w;
+  gauss_legendre01(N, x, w);
+  std::vector<double> tab(2 * N * N + 6 * N);
+  double *D = tab.data(), *DT = D + N * N, *L = DT + N * N, *LW = L + 2 * N, *X = LW + 2 * N,
+         *W = X + N;
+  for (int q = 0; q < N; ++q) lagrange_barycentric(x, x[q], nullptr, D + q * N);
+  for (int i = 0; i < N; ++i)
+    for (int m = 0; m < N; ++m) DT[i * N + m] = D[m * N + i] * w[m] / w[i];
+  lagrange_barycentric(x, 0.0, L, nullptr);
+  lagrange_barycentric(x, 1.0, L + N, nullptr);
+  for (int s = 0; s < 2; ++s)
+    for (int i = 0; i < N; ++i) LW[s * N + i] = L[s * N + i] / w[i];
+  for (int i = 0; i < N; ++i) {
+    X[i] = x[i];
+    W[i] = w[i];
+  }
+  // error rule QGauss(k+2)
+  std::vector<double> xe, we;
+  gauss_legendre01(N + 1, xe, we);
+  std::vector<double> etab(2 * (N + 1) + (N + 1) * N);
+  for (int i = 0; i <= N; ++i) {
+    etab[i] = xe[i];
+    etab[N + 1 + i] = we[i];
+    lagrange_barycentric(x, xe[i], etab.data() + 2 * (N + 1) + i * N, nullptr);
+  }
+  // CK level tables
+  std::vector<double> ckt;
+  CkTab ck{};
+  const int nsteps = degree >= 2 ? degree - 1 : 1;
+  ck.n_levels = (nsteps - 1) / 2 + 1;
+  std::vector<double> xprev, wprev;
+  for (int l = 0; l < ck.n_levels; ++l) {
+    const int n = N - 2 * l;
+    std::vector<double> xl, wl;
+    gauss_legendre01(n, xl, wl);
+    ck.off_D[l] = (int)ckt.size();
+    ckt.resize(ckt.size() + n * n);
+    for (int q = 0; q < n; ++q) lagrange_barycentric(xl, xl[q], nullptr, &ckt[ck.off_D[l] + q * n]);
+    ck.off_X[l] = (int)ckt.size();
+    for (int i = 0; i < n; ++i) ckt.push_back(xl[i]);
+    if (l > 0) {
+      const int nh = n + 2;
+      std::vector<double> vl(n);
+      ck.off_P[l] = (int)ckt.size();
+      ckt.resize(ckt.size() + n * nh);
+      ck.off_V[l] = (int)ckt.size();
+      ckt.resize(ckt.size() + nh * n);
+      for (int q = 0; q < nh; ++q) {
+        lagrange_barycentric(xl, xprev[q], vl.data(), nullptr);
+        for (int i = 0; i < n; ++i) {
+          ckt[ck.off_P[l] + i * nh + q] = vl[i] * wprev[q] / wl[i];
+          ckt[ck.off_V[l] + q * n + i] = vl[i];
+        }
+      }
+    }
+    xprev = xl;
+    wprev = wl;
+  }
+  ck.size = (int)ckt.size();
+  op->ck = ck;
+  if ((rc = upload(op, &op->d_tab, tab.data(), tab.size())) ||
+      (rc = upload(op, &op->d_errtab, etab.data(), etab.size())) ||
+      (rc = upload(op, &op->d_cktab, ckt.data(), ckt.size()))) {
+    free_op(op);
+    return rc;
+  }
+  op->n_partial = 1024;
+  if (hipMalloc((void **)&op->d_partial, sizeof(double) * 3 * op->n_partial) != hipSuccess ||
+      hipMalloc((void **)&op->d_sum, sizeof(double) * 4) != hipSuccess) {
+    free_op(op);
+    return fail(EXW_ENOMEM, "exw_op_create: hipMalloc failed");
+  }
+  *out = op;
+  return EXW_OK;
+}
+
+int exw_op_destroy(exw_op op) {
+  free_op(op);
+  return EXW_OK;
+}
+
+int64_t exw_op_n_dofs(exw_op op) { return op ? op->n_dofs() : -1; }
+int exw_op_n_cells(exw_op op) { return op ? op->n_cells : -1; }
+int exw_op_is_cartesian(exw_op op) { return op ? (int)op->cart : -1; }
+double exw_op_min_vertex_distance_over_c(exw_op op) { return op ? op->minvd : -1.0; }
+
+static int check_prec(int precision) {
+  if (precision != EXW_FP64 && precision != EXW_FP32)
+    return fail(EXW_EINVAL, "precision must be EXW_FP64 or EXW_FP32");
+  return EXW_OK;
+}
+
+int exw_op_apply_fused(exw_op op, const double *src, const exw_epilogue *e, int precision) {
+  if (!op || !src || !e || !e->o1) return fail(EXW_EINVAL, "exw_op_apply_fused: null argument");
+  if (int rc = check_prec(precision)) return rc;
+  if ((const double *)e->o1 == src || (const double *)e->o2 == src)
+    return fail(EXW_EINVAL, "exw_op_apply_fused: an output aliases src (neighbour reads race)");
+  if (e->o2 && !e->o2_from_o1 && !e->c)
+    return fail(EXW_EINVAL, "exw_op_apply_fused: o2 needs c or o2_from_o1");
+  Epilogue ep{e->b, e->o1, e->a1, e->c, e->o2, e->a2, e->o2_from_o1};
+  OpLaunch l = kernels(op->dim).op[op->degree][precision][op->cart ? 1 : 0];
+  if (!l) return fail(EXW_EUNSUP, "operator kernel not compiled");
+  EXW_HIP(l(op->view(), src, op->d_tab, ep, op->ctx->stream));
+  return EXW_OK;
+}
+
+int exw_op_apply(exw_op op, const double *src, double *dst, int precision) {
+  exw_epilogue e{nullptr, dst, 1.0, nullptr, nullptr, 0.0, 0};
+  return exw_op_apply_fused(op, src, &e, precision);
+}
+
+int exw_rk_stage(exw_op op, const double *src, const double *b, double *y_out, double *b_out,
+                 double f1, double f2, int is_last, int precision) {
+  if (!b) return fail(EXW_EINVAL, "exw_rk_stage: b is NULL");
+  if (!is_last && !b_out) return fail(EXW_EINVAL, "exw_rk_stage: b_out is NULL");
+  exw_epilogue e{b, y_out, f1, nullptr, is_last ? nullptr : b_out, f2, 1};
+  return exw_op_apply_fused(op, src, &e, precision);
+}
+
+int exw_ader_ck(exw_op op, const double *src, const double *recon, double t2, double t1,
+                double te, double *tempsrc, int precision) {
+  if (!op || !src || !tempsrc) return fail(EXW_EINVAL, "exw_ader_ck: null argument");
+  if (int rc = check_prec(precision)) return rc;
+  if (tempsrc == src || tempsrc == recon)
+    return fail(EXW_EINVAL, "exw_ader_ck: tempsrc aliases an input");
+  CkLaunch l = kernels(op->dim).ck[op->degree][precision][op->cart ? 1 : 0];
+  if (!l) return fail(EXW_EUNSUP, "CK kernel not compiled");
+  EXW_HIP(l(op->view(), src, recon, op->d_cktab, op->ck, t2, t1, te, tempsrc, op->ctx->stream));
+  return EXW_OK;
+}
+
+int exw_ader_step(exw_op op, const double *vec_n, double *vec_np, double *work, double dt,
+                  int recon, int precision) {
+  if (!op || !vec_n || !vec_np || !work) return fail(EXW_EINVAL, "exw_ader_step: null argument");
+  const int64_t N = op->n_dofs();
+  double *tempsrc = work, *rec = work + N;
+  int rc;
+  if (recon && (rc = exw_op_apply(op, vec_n, rec, precision))) return rc;
+  if ((rc = exw_ader_ck(op, vec_n, recon ? rec : nullptr, dt, 0.0, 0.0, tempsrc, precision)))
+    return rc;
+  // vec_np = vec_n - (-M^-1 A tempsrc)  (source/time_integrators.cc:44-47 with the sign
+  // flipped second sweep of source/wave_equation_operations.cc:1301-1342)
+  exw_epilogue e{vec_n, vec_np, 1.0, nullptr, nullptr, 0.0, 0};
+  return exw_op_apply_fused(op, tempsrc, &e, precision);
+}
+
+int exw_time_step_work_vectors(int integrator) {
+  switch (integrator) {
+    case EXW_EXPL_EULER: return 0;
+    case EXW_CLRK4: return 2;
+    case EXW_LSRK45R2:
+    case EXW_LSRK33R2:
+    case EXW_LSRK59R2: return 1;
+    case EXW_ADER: return 2;
+    default: return -1;
+  }
+}
+
+// Low-storage RK with the RKVectorUpdater fused into each operator sweep.
+// Registers: R_n (vec_n), R_np (vec_np), R_t (work).  Stage 1 reads the
+// state from R_n and writes the next stage input to X1 and the accumulator to
+// X2 ({X1,X2} = {R_t,R_np}); every later non-last stage reads (input, acc)
+// and writes the new input over acc (pointwise, in place) and the new acc into
+// the free register.  The last stage writes acc + f1*k into whichever of
+// {acc, free} is R_np.  X1/X2 are chosen so that the last stage's input is
+// not R_np.  Same arithmetic as source/time_integrators.cc:128-155.
+static int lsrk_fused(exw_op op, int S, const double *a, const double *b, double dt,
+                      double *vec_n, double *vec_np, double *work, int precision) {
+  // symbolic rotation to find the register read by the last stage
+  enum { RN = 0, X1 = 1, X2 = 2 };
+  int src = RN, acc = RN, fre = -1;
+  for (int s = 0; s < S - 1; ++s) {
+    if (s == 0) {
+      src = X1;
+      acc = X2;
+      fre = RN;
+    } else {
+      const int ns = acc, na = fre, nf = src;
+      src = ns;
+      acc = na;
+      fre = nf;
+    }
+  }
+  double *reg[3];
+  reg[RN] = vec_n;
+  if (S == 1 || src == X1) {
+    reg[X1] = work;
+    reg[X2] = vec_np;
+  } else if (src == X2) {
+    reg[X1] = vec_np;
+    reg[X2] = work;
+  } else {  // last input is R_n: any assignment works
+    reg[X1] = work;
+    reg[X2] = vec_np;
+  }
+  int cs = RN, ca = RN, cf = -1;
+  for (int s = 0; s < S; ++s) {
+    const bool last = s == S - 1;
+    int rc;
+    if (!last) {
+      const int y_out = s == 0 ? X1 : ca, b_out = s == 0 ? X2 : cf;
+      rc = exw_rk_stage(op, reg[cs], reg[ca], reg[y_out], reg[b_out], a[s] * dt,
+                        (b[s] - a[s]) * dt, 0, precision);
+      if (rc) return rc;
+      if (s == 0) {
+        cs = X1;
+        ca = X2;
+        cf = RN;
+      } else {
+        const int ns = ca, na = cf, nf = cs;
+        cs = ns;
+        ca = na;
+        cf = nf;
+      }
+    } else {
+      double *out = (reg[ca] == vec_np) ? reg[ca] : reg[cf];
+      if (S == 1) out = vec_np;
+      if (out != vec_np) return fail(EXW_EINVAL, "internal: LSRK register plan failed");
+      rc = exw_rk_stage(op, reg[cs], reg[ca], out, nullptr, b[s] * dt, 0.0, 1, precision);
+      if (rc) return rc;
+    }
+  }
+  return EXW_OK;
+}
+
+int exw_time_step(exw_op op, int integrator, double dt, double *vec_n, double *vec_np,
+                  double *work, int ader_recon, int precision) {
+  if (!op || !vec_n || !vec_np) return fail(EXW_EINVAL, "exw_time_step: null argument");
+  if (vec_n == vec_np) return fail(EXW_EINVAL, "exw_time_step: vec_n aliases vec_np");
+  const int64_t N = op->n_dofs();
+  switch (integrator) {
+    case EXW_EXPL_EULER: {
+      // source/time_integrators.cc:24-34: vec_np = vec_n + dt*M^-1 A vec_n
+      exw_epilogue e{vec_n, vec_np, dt, nullptr, nullptr, 0.0, 0};
+      return exw_op_apply_fused(op, vec_n, &e, precision);
+    }
+    case EXW_CLRK4: {
+      // source/time_integrators.cc:66-102 with the vector updates fused
+      if (!work) return fail(EXW_EINVAL, "exw_time_step: work is NULL");
+      double *ta = work, *tb = work + N;
+      const double wgt[4] = {dt / 6., dt / 3., dt / 3., dt / 6.};
+      const double cf[3] = {0.5 * dt, 0.5 * dt, dt};
+      const double *src = vec_n;
+      double *nxt = ta;
+      for (int s = 0; s < 4; ++s) {
+        exw_epilogue e{s == 0 ? vec_n : vec_np, vec_np, wgt[s], vec_n, s < 3 ? nxt : nullptr,
+                       s < 3 ? cf[s] : 0.0, 0};
+        if (int rc = exw_op_apply_fused(op, src, &e, precision)) return rc;
+        src = nxt;
+        nxt = nxt == ta ? tb : ta;
+      }
+      return EXW_OK;
+    }
+    case EXW_LSRK45R2: {
+      // source/time_integrators.cc:235-283
+      const double a[5] = {970286171893. / 4311952581923., 6584761158862. / 12103376702013.,
+                           2251764453980. / 15575788980749., 26877169314380. / 34165994151039.,
+                           0.0};
+      const double b[5] = {1153189308089. / 22510343858157., 1772645290293. / 4653164025191.,
+                           -1672844663538. / 4480602732383., 2114624349019. / 3568978502595.,
+                           5198255086312. / 14908931495163.};
+      if (!work) return fail(EXW_EINVAL, "exw_time_step: work is NULL");
+      return lsrk_fused(op, 5, a, b, dt, vec_n, vec_np, work, precision);
+    }
+    case EXW_LSRK33R2: {
+      // source/time_integrators.cc:200-231
+      const double a[3] = {0.755726351946097, 0.386954477304099, 0.0};
+      const double b[3] = {0.245170287303492, 0.184896052186740, 0.569933660509768};
+      if (!work) return fail(EXW_EINVAL, "exw_time_step: work is NULL");
+      return lsrk_fused(op, 3, a, b, dt, vec_n, vec_np, work, precision);
+    }
+    case EXW_LSRK59R2: {
+      // source/time_integrators.cc:286-363
+      const double a[9] = {1107026461565. / 5417078080134.,   38141181049399. / 41724347789894.,
+                           493273079041. / 11940823631197.,   1851571280403. / 6147804934346.,
+                           11782306865191. / 62590030070788., 9452544825720. / 13648368537481.,
+                           4435885630781. / 26285702406235.,  2357909744247. / 11371140753790.,
+                           0.0};
+      const double b[9] = {2274579626619. / 23610510767302., 693987741272. / 12394497460941.,
+                           -347131529483. / 15096185902911., 1144057200723. / 32081666971178.,
+                           1562491064753. / 11797114684756., 13113619727965. / 44346030145118.,
+                           393957816125. / 7825732611452.,   720647959663. / 6565743875477.,
+                           3559252274877. / 14424734981077.};
+      if (!work) return fail(EXW_EINVAL, "exw_time_step: work is NULL");
+      return lsrk_fused(op, 9, a, b, dt, vec_n, vec_np, work, precision);
+    }
+    case EXW_ADER:
+      if (!work) return fail(EXW_EINVAL, "exw_time_step: work is NULL");
+      return exw_ader_step(op, vec_n, vec_np, work, dt, ader_recon, precision);
+    default:
+      return fail(EXW_EINVAL, "unknown time integrator " + std::to_string(integrator) +
+                                  " requested");
+  }
+}
+
+static DevModes to_dev(const exw_modes *m) {
+  DevModes d{};
+  d.n_modes = m->n_modes;
+  for (int k = 0; k < 8; ++k) {
+    d.amp[k] = m->amp[k];
+    for (int j = 0; j < 3; ++j) d.m[k][j] = m->m[k][j];
+  }
+  d.rho = m->rho;
+  d.c = m->c;
+  return d;
+}
+
+int exw_project(exw_op op, const exw_modes *modes, double t, double *dst) {
+  if (!op || !modes || !dst) return fail(EXW_EINVAL, "exw_project: null argument");
+  if (modes->n_modes < 0 || modes->n_modes > 8) return fail(EXW_EINVAL, "n_modes must be 0..8");
+  EXW_HIP(kernels(op->dim).project[op->degree](op->view(), op->d_tab, to_dev(modes), t, dst,
+                                                op->ctx->stream));
+  return EXW_OK;
+}
+
+int exw_l2_errors(exw_op op, const exw_modes *modes, double t, const double *u,
+                  double *out_host) {
+  if (!op || !modes || !u || !out_host) return fail(EXW_EINVAL, "exw_l2_errors: null argument");
+  if (modes->n_modes < 0 || modes->n_modes > 8) return fail(EXW_EINVAL, "n_modes must be 0..8");
+  const int nb = op->n_partial;
+  EXW_HIP(kernels(op->dim).l2err[op->degree](op->view(), op->d_errtab, to_dev(modes), t, u,
+                                              op->d_partial, nb, op->ctx->stream));
+  hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(256), 0, op->ctx->stream, op->d_partial,
+                     nb, 3, op->d_sum);
+  EXW_HIP(hipGetLastError());
+  double s[3];
+  EXW_HIP(hipMemcpyAsync(s, op->d_sum, sizeof(s), hipMemcpyDeviceToHost, op->ctx->stream));
+  EXW_HIP(hipStreamSynchronize(op->ctx->stream));
+  for (int k = 0; k < 3; ++k) out_host[k] = std::sqrt(s[k]);
+  return EXW_OK;
+}
+
+int exw_energy(exw_op op, const double *u, double *out_host) {
+  if (!op || !u || !out_host) return fail(EXW_EINVAL, "exw_energy: null argument");
+  const int nb = op->n_partial;
+  EXW_HIP(kernels(op->dim).energy[op->degree](op->view(), op->d_tab, u, op->d_partial, nb,
+                                               op->ctx->stream));
+  hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(256), 0, op->ctx->stream, op->d_partial,
+                     nb, 1, op->d_sum);
+  EXW_HIP(hipGetLastError());
+  EXW_HIP(hipMemcpyAsync(out_host, op->d_sum, sizeof(double), hipMemcpyDeviceToHost,
+                         op->ctx->stream));
+  EXW_HIP(hipStreamSynchronize(op->ctx->stream));
+  return EXW_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,164 @@
+// exwave_driver.hip -- problem setup and the whole solve behind the C-ABI:
+// the hypercube mesh (input_geometry_description, include/input_parameters.h:
+// 59-88), TimeControl (source/utilities.cc:31-95) and the time loop of
+// WaveEquationProblem::run (source/explicit_wave.cc:549-813).
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/exwave_amd.h"
+
+namespace {
+
+void transform_vertex(int dim, int type, double f, double *x) {
+  const double in[3] = {x[0], x[1], dim > 2 ? x[2] : 0.0};
+  if (type == EXW_TRANSFORM_REFERENCE_SINE) {
+    double s = f;
+    for (int d = 0; d < dim; ++d) s *= std::sin(M_PI * in[d]);
+    x[0] = in[0] + s;
+  } else if (type == EXW_TRANSFORM_AXIS_2PI) {
+    for (int i = 0; i < dim; ++i) {
+      double s = f;
+      for (int e = 0; e < dim; ++e)
+        if (e != i) s *= std::sin(2.0 * M_PI * in[e]);
+      x[i] = in[i] + s;
+    }
+  } else if (type == EXW_TRANSFORM_PRODUCT_2PI) {
+    double s = f;
+    for (int e = 0; e < dim; ++e) s *= std::sin(2.0 * M_PI * in[e]);
+    for (int i = 0; i < dim; ++i) x[i] = in[i] + s;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int exw_mesh_hypercube(int dim, int n_intervals, int n_refinements, int transform, double factor,
+                       int boundary_id, int *n_cells, double *vertices, int *neighbor) {
+  if ((dim != 2 && dim != 3) || n_intervals < 1 || n_refinements < 0 || n_refinements > 12 ||
+      transform < 0 || transform > 3 || boundary_id < 1 || boundary_id > 3)
+    return EXW_EINVAL;
+  const int R = 1 << n_refinements;
+  const long Nf = (long)n_intervals * R;
+  long nc = 1;
+  for (int d = 0; d < dim; ++d) nc *= Nf;
+  if (nc > 2000000000L) return EXW_EINVAL;
+  if (n_cells) *n_cells = (int)nc;
+  if (!vertices || !neighbor) return EXW_OK;
+  const int nv = 1 << dim;
+  for (long c = 0; c < nc; ++c) {
+    long idx[3] = {0, 0, 0}, t = c;
+    for (int d = 0; d < dim; ++d) {
+      idx[d] = t % Nf;
+      t /= Nf;
+    }
+    // coarse parent corners after the transform
+    double PV[8][3];
+    for (int v = 0; v < nv; ++v) {
+      double x[3] = {0, 0, 0};
+      for (int d = 0; d < dim; ++d) x[d] = double(idx[d] / R + ((v >> d) & 1)) / n_intervals;
+      transform_vertex(dim, transform, factor, x);
+      for (int d = 0; d < dim; ++d) PV[v][d] = x[d];
+    }
+    // children are the multilinear map of the parent at dyadic points
+    for (int v = 0; v < nv; ++v) {
+      double xi[3];
+      for (int d = 0; d < dim; ++d) xi[d] = double(idx[d] % R + ((v >> d) & 1)) / R;
+      double *out = vertices + (c * nv + v) * dim;
+      for (int i = 0; i < dim; ++i) out[i] = 0.0;
+      for (int u = 0; u < nv; ++u) {
+        double s = 1.0;
+        for (int d = 0; d < dim; ++d) s *= ((u >> d) & 1) ? xi[d] : 1.0 - xi[d];
+        for (int i = 0; i < dim; ++i) out[i] += s * PV[u][i];
+      }
+    }
+    for (int d = 0; d < dim; ++d) {
+      long stride = 1;
+      for (int e = 0; e < d; ++e) stride *= Nf;
+      neighbor[c * 2 * dim + 2 * d] = idx[d] > 0 ? int(c - stride) : -boundary_id;
+      neighbor[c * 2 * dim + 2 * d + 1] = idx[d] + 1 < Nf ? int(c + stride) : -boundary_id;
+    }
+  }
+  return EXW_OK;
+}
+
+int exw_run(exw_ctx ctx, const exw_problem *p, char *log, int log_len, double *wall_seconds) {
+  if (!ctx || !p) return EXW_EINVAL;
+  int n_cells = 0, rc;
+  if ((rc = exw_mesh_hypercube(p->dim, p->n_intervals, p->n_refinements, p->transform,
+                               p->transform_factor, p->boundary_id, &n_cells, nullptr, nullptr)))
+    return rc;
+  const int nv = 1 << p->dim;
+  std::vector<double> V((size_t)n_cells * nv * p->dim);
+  std::vector<int> nb((size_t)n_cells * 2 * p->dim);
+  if ((rc = exw_mesh_hypercube(p->dim, p->n_intervals, p->n_refinements, p->transform,
+                               p->transform_factor, p->boundary_id, &n_cells, V.data(), nb.data())))
+    return rc;
+  std::vector<double> rho(n_cells, p->modes.rho), speed(n_cells, p->modes.c);
+  exw_op op = nullptr;
+  if ((rc = exw_op_create(ctx, p->dim, p->degree, n_cells, V.data(), nb.data(), rho.data(),
+                          speed.data(), &op)))
+    return rc;
+  const int64_t N = exw_op_n_dofs(op);
+  const int nwork = exw_time_step_work_vectors(p->integrator);
+  if (nwork < 0) {
+    exw_op_destroy(op);
+    return EXW_EINVAL;
+  }
+  double *buf = nullptr;
+  if ((rc = exw_alloc(ctx, sizeof(double) * N * (2 + nwork), (void **)&buf))) {
+    exw_op_destroy(op);
+    return rc;
+  }
+  double *sol = buf, *tmp = buf + N, *work = buf + 2 * N;
+  // parameters.cc:195 and explicit_wave.cc:529-545
+  const double cfl = p->cfl_number / std::pow((double)p->degree, 1.5);
+  const double dt_cfl = cfl * exw_op_min_vertex_distance_over_c(op);
+  // TimeControl::setup (utilities.cc:31-46), then make_dofs resets the step to
+  // the unrounded CFL value (explicit_wave.cc:222)
+  const double T = p->final_time, tick = p->output_every_time;
+  double time = 0.0;
+  double dt = (T - time) / std::max(std::round((T - time) / dt_cfl), 1.0);
+  unsigned step = 0;
+  const unsigned max_step =
+      p->max_time_steps == 0 ? (unsigned)std::round((T - time) / dt) : (unsigned)p->max_time_steps;
+  dt = dt_cfl;
+  int pos = 0;
+  if (log && log_len > 0) log[0] = 0;
+  auto output = [&]() -> int {
+    double err[3];
+    int r = exw_l2_errors(op, &p->modes, time, sol, err);
+    if (r) return r;
+    if (log && pos < log_len)
+      pos += std::snprintf(log + pos, log_len - pos, "%u %.17g %.10e %.10e %.10e\n", step, time,
+                           err[0], err[1], err[2]);
+    return 0;
+  };
+  double wall = 0.0;
+  if (!(rc = exw_project(op, &p->modes, 0.0, sol)) && !(rc = output())) {
+    while (!(time >= T || step > max_step)) {
+      ++step;
+      time += dt;
+      std::swap(sol, tmp);
+      exw_sync(ctx);
+      auto t0 = std::chrono::steady_clock::now();
+      if ((rc = exw_time_step(op, p->integrator, dt, tmp, sol, work, p->ader_recon, p->precision)))
+        break;
+      if ((rc = exw_sync(ctx))) break;
+      wall += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      // TimeControl::at_tick (utilities.cc:89-95)
+      if ((int)(time / tick) != (int)((time - dt) / tick) && (rc = output())) break;
+    }
+  }
+  exw_free(ctx, buf);
+  exw_op_destroy(op);
+  if (wall_seconds) *wall_seconds = wall;
+  return rc ? rc : (int)step;
+}
+
+}  // extern "C"

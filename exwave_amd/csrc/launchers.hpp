@@ -1,0 +1,105 @@
+// launchers.hpp -- typed launchers and per-(dim,degree,precision,geometry)
+// dispatch tables of the MI355X kernels.
+#pragma once
+#include "dg_ader.hpp"
+#include "dg_misc.hpp"
+#include "dg_operator.hpp"
+
+namespace exw {
+
+constexpr int MAX_DEGREE = 8;
+
+using OpLaunch = hipError_t (*)(const DevMesh &, const double *, const double *, const Epilogue &,
+                                hipStream_t);
+using CkLaunch = hipError_t (*)(const DevMesh &, const double *, const double *, const double *,
+                                const CkTab &, double, double, double, double *, hipStream_t);
+using ProjLaunch = hipError_t (*)(const DevMesh &, const double *, const DevModes &, double,
+                                  double *, hipStream_t);
+using ErrLaunch = hipError_t (*)(const DevMesh &, const double *, const DevModes &, double,
+                                 const double *, double *, int, hipStream_t);
+using EnergyLaunch = hipError_t (*)(const DevMesh &, const double *, const double *, double *,
+                                    int, hipStream_t);
+
+struct DimKernels {
+  OpLaunch op[MAX_DEGREE + 1][2][2];  // [k][fp32][cart]
+  CkLaunch ck[MAX_DEGREE + 1][2][2];
+  ProjLaunch project[MAX_DEGREE + 1];
+  ErrLaunch l2err[MAX_DEGREE + 1];
+  EnergyLaunch energy[MAX_DEGREE + 1];
+  int block[MAX_DEGREE + 1];
+  int cells_per_block[MAX_DEGREE + 1];
+};
+
+template <int DIM, int K, typename Number, bool CART>
+hipError_t launch_op(const DevMesh &m, const double *src, const double *tab, const Epilogue &e,
+                     hipStream_t s) {
+  using S = Shape<DIM, K>;
+  const int grid = (m.n_cells + S::CPB - 1) / S::CPB;
+  if (grid == 0) return hipSuccess;
+  hipLaunchKernelGGL((dg_operator_kernel<DIM, K, Number, CART>), dim3(grid), dim3(S::BLOCK), 0,
+                     s, m, src, tab, e);
+  return hipGetLastError();
+}
+
+template <int DIM, int K, typename Number, bool CART>
+hipError_t launch_ck(const DevMesh &m, const double *src, const double *recon,
+                     const double *cktab, const CkTab &ck, double t2, double t1, double te,
+                     double *out, hipStream_t s) {
+  using S = Shape<DIM, K>;
+  const int grid = (m.n_cells + S::CPB - 1) / S::CPB;
+  if (grid == 0) return hipSuccess;
+  hipLaunchKernelGGL((ader_ck_kernel<DIM, K, Number, CART>), dim3(grid), dim3(S::BLOCK), 0, s,
+                     m, src, recon, cktab, ck, t2, t1, te, out);
+  return hipGetLastError();
+}
+
+template <int DIM, int K>
+hipError_t launch_project(const DevMesh &m, const double *tab, const DevModes &md, double t,
+                          double *dst, hipStream_t s) {
+  using S = Shape<DIM, K>;
+  const size_t total = (size_t)m.n_cells * S::ND;
+  const int grid = int((total + 255) / 256);
+  if (grid == 0) return hipSuccess;
+  hipLaunchKernelGGL((project_kernel<DIM, K>), dim3(grid), dim3(256), 0, s, m, tab, md, t, dst);
+  return hipGetLastError();
+}
+
+template <int DIM, int K>
+hipError_t launch_l2err(const DevMesh &m, const double *errtab, const DevModes &md, double t,
+                        const double *u, double *partial, int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL((l2_error_kernel<DIM, K>), dim3(nblocks), dim3(256), 0, s, m, errtab, md, t,
+                     u, partial);
+  return hipGetLastError();
+}
+
+template <int DIM, int K>
+hipError_t launch_energy(const DevMesh &m, const double *tab, const double *u, double *partial,
+                         int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL((energy_kernel<DIM, K>), dim3(nblocks), dim3(256), 0, s, m, tab, u, partial);
+  return hipGetLastError();
+}
+
+template <int DIM, int K>
+void fill_dim_kernels(DimKernels &t) {
+  if constexpr (K >= 1) {
+    t.op[K][0][0] = launch_op<DIM, K, double, false>;
+    t.op[K][0][1] = launch_op<DIM, K, double, true>;
+    t.op[K][1][0] = launch_op<DIM, K, float, false>;
+    t.op[K][1][1] = launch_op<DIM, K, float, true>;
+    t.ck[K][0][0] = launch_ck<DIM, K, double, false>;
+    t.ck[K][0][1] = launch_ck<DIM, K, double, true>;
+    t.ck[K][1][0] = launch_ck<DIM, K, float, false>;
+    t.ck[K][1][1] = launch_ck<DIM, K, float, true>;
+    t.project[K] = launch_project<DIM, K>;
+    t.l2err[K] = launch_l2err<DIM, K>;
+    t.energy[K] = launch_energy<DIM, K>;
+    t.block[K] = Shape<DIM, K>::BLOCK;
+    t.cells_per_block[K] = Shape<DIM, K>::CPB;
+    fill_dim_kernels<DIM, K - 1>(t);
+  }
+}
+
+const DimKernels &kernels_2d();
+const DimKernels &kernels_3d();
+
+}  // namespace exw

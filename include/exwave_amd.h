@@ -1,0 +1,206 @@
+/*
+ * exwave_amd.h -- C-ABI of the MI355X-native ExWave hot path.
+ *
+ * Plain pointers and sizes only.  Vector arguments are DEVICE pointers to
+ * FP64 arrays of exw_op_n_dofs() entries in the layout
+ *   [cell][field][node],  field = v_0..v_{dim-1}, p,  node = lexicographic
+ *   index of the (k+1)^dim Gauss points of the cell, xi_0 fastest.
+ * (The polynomial space and quadrature are the reference's FE_DGQ(k)^(dim+1)
+ * with QGauss(k+1); the basis is the Lagrange basis collocated at the Gauss
+ * points, the reference's FE_DGQArbitraryNodes(QGauss(k+1)) variant,
+ * source/explicit_wave.cc:163.)
+ *
+ * Every call is asynchronous on the context's stream and returns 0 on
+ * success or a negative EXW_E* code; exw_last_error() gives the message.
+ * A context is not thread-safe: use one host thread per context.
+ *
+ * Reference citations are paths in kronbichler/exwave.
+ */
+#ifndef EXWAVE_AMD_H
+#define EXWAVE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define EXW_OK 0
+#define EXW_EINVAL (-1)   /* invalid argument (shape, degree, boundary id, aliasing) */
+#define EXW_EDEVICE (-2)  /* HIP runtime error */
+#define EXW_ENOMEM (-3)   /* device allocation failed */
+#define EXW_EUNSUP (-4)   /* combination not compiled (dim/degree/precision) */
+
+/* operator arithmetic; vectors, RK accumulation and norms are always FP64 */
+#define EXW_FP64 0
+#define EXW_FP32 1
+
+/* time integrators: the reference's IntegratorType names
+ * (include/parameters.h, source/parameters.cc:150-193) */
+#define EXW_EXPL_EULER 0
+#define EXW_CLRK4 1
+#define EXW_LSRK45R2 2
+#define EXW_LSRK33R2 3
+#define EXW_LSRK59R2 4
+#define EXW_ADER 5
+
+typedef struct exw_ctx_s *exw_ctx;
+typedef struct exw_op_s *exw_op;
+
+/* Superposition of membrane modes in a homogeneous medium:
+ *   p   = sum_j amp_j cos(w_j t) prod_d sin(m_jd pi x_d),  w_j = c pi |m_j|
+ *   v_d = -sum_j amp_j m_jd/(rho c |m_j|) sin(w_j t) cos(m_jd pi x_d) prod_{e!=d} sin(m_je pi x_e)
+ * One mode with m_jd = M and rho = c = 1 is the reference's ExactSolution
+ * (include/input_parameters.h:94-128). */
+typedef struct exw_modes {
+  int n_modes; /* <= 8 */
+  double amp[8];
+  int m[8][3];
+  double rho, c;
+} exw_modes;
+
+/* Fused epilogue applied to k = M^-1 A(src) at every DoF i:
+ *   o1[i] = (b ? b[i] : 0) + a1*k[i]
+ *   o2[i] = (o2_from_o1 ? o1[i] : c[i]) + a2*k[i]        (if o2 != NULL)
+ * This covers apply (b=NULL, a1=1), RKVectorUpdater (b=v2, o1=v2, o2=v3,
+ * o2_from_o1=1), its last stage, explicit Euler, ClassRK4 stages and the
+ * ADER update.  o1/o2 may alias b/c but never src. */
+typedef struct exw_epilogue {
+  const double *b;
+  double *o1;
+  double a1;
+  const double *c;
+  double *o2;
+  double a2;
+  int o2_from_o1;
+} exw_epilogue;
+
+const char *exw_last_error(void);
+const char *exw_version(void);
+
+/* ---- context ------------------------------------------------------------
+ * ctx_create binds a HIP device and a stream (hipStream_t passed as void*;
+ * NULL = the device's null stream). */
+int exw_ctx_create(int device, void *stream, exw_ctx *out);
+int exw_ctx_set_stream(exw_ctx ctx, void *stream);
+int exw_ctx_destroy(exw_ctx ctx);
+int exw_sync(exw_ctx ctx);
+/* device memory helpers (callers may also bring their own allocations) */
+int exw_alloc(exw_ctx ctx, size_t bytes, void **dptr);
+int exw_free(exw_ctx ctx, void *dptr);
+int exw_memcpy_h2d(exw_ctx ctx, void *dst, const void *src, size_t bytes);
+int exw_memcpy_d2h(exw_ctx ctx, void *dst, const void *src, size_t bytes);
+
+/* ---- operator (replaces WaveEquationOperation<dim,k>::setup,
+ * source/wave_equation_operations.cc:200-250, and reset_data_vectors, :383-400)
+ *
+ * Host arrays, copied to the device:
+ *   vertices [n_cells][2^dim][dim]  multilinear cell corners (lexicographic)
+ *   neighbor [n_cells][2*dim]       face f = 2*d+s is the face xi_d = s; >= 0:
+ *                                   neighbouring cell (seeing it as face f^1,
+ *                                   same tangential orientation); < 0: -(boundary id)
+ *                                   with ids 1 soft wall, 2 p=0, 3 absorbing
+ *                                   (source/wave_equation_operations.cc:592-617)
+ *   rho, speed [n_cells]            material per cell (Material, include/utilities.h:28-34)
+ * Axis-aligned box meshes select the Cartesian kernels, anything else the
+ * general multilinear kernels (Jacobians recomputed from the vertices). */
+int exw_op_create(exw_ctx ctx, int dim, int degree, int n_cells, const double *vertices,
+                  const int *neighbor, const double *rho, const double *speed, exw_op *out);
+int exw_op_destroy(exw_op op);
+int64_t exw_op_n_dofs(exw_op op);
+int exw_op_n_cells(exw_op op);
+int exw_op_is_cartesian(exw_op op);
+/* dt = cfl/k^1.5 * min_K min_vertex_distance(K)/c_K without the cfl factor
+ * (source/explicit_wave.cc:529-545, source/parameters.cc:195) */
+double exw_op_min_vertex_distance_over_c(exw_op op);
+
+/* ---- hot path ---------------------------------------------------------- */
+/* dst = M^-1 A(src): WaveEquationOperationBase<dim>::apply
+ * (include/wave_equation_operations.h:75-76; source/wave_equation_operations.cc:665-685) */
+int exw_op_apply(exw_op op, const double *src, double *dst, int precision);
+/* k = M^-1 A(src) with a fused epilogue (the operator + RKVectorUpdater fusion) */
+int exw_op_apply_fused(exw_op op, const double *src, const exw_epilogue *epi, int precision);
+/* One low-storage RK stage: RKVectorUpdater semantics fused into the operator
+ * (source/time_integrators.cc:107-163):
+ *   not last: y_out = b + f1*k ; b_out = y_out + f2*k
+ *   last    : y_out = b + f1*k
+ * with k = M^-1 A(src).  y_out and b_out must not alias src. */
+int exw_rk_stage(exw_op op, const double *src, const double *b, double *y_out, double *b_out,
+                 double f1, double f2, int is_last, int precision);
+/* ADER-CK time integral (integrate_taylor_cauchykovalewski,
+ * source/wave_equation_operations.cc:961-1262): tempsrc = int_{t1}^{t2} u dt
+ * by the Taylor series about te; recon (optional, may be NULL) replaces the
+ * first time derivative (use_ader_post). */
+int exw_ader_ck(exw_op op, const double *src, const double *recon, double t2, double t1,
+                double te, double *tempsrc, int precision);
+/* ArbitraryHighOrderDG::perform_time_step + WaveEquationOperationADER::apply_ader
+ * (source/time_integrators.cc:37-50; source/wave_equation_operations.cc:1346-1381):
+ * vec_np = vec_n - apply_ader(vec_n).  work: 2*n_dofs doubles (tempsrc, recon). */
+int exw_ader_step(exw_op op, const double *vec_n, double *vec_np, double *work, double dt,
+                  int recon, int precision);
+/* ExplicitIntegrator<..>::perform_time_step (include/time_integrators.h:25-39)
+ * for the integrator ids above.  Same contract as the reference: vec_n is
+ * clobbered, the result is left in vec_np.  work: 3*n_dofs doubles (the
+ * reference's vec_tmp1/vec_tmp2 plus one register for the fused stages). */
+int exw_time_step(exw_op op, int integrator, double dt, double *vec_n, double *vec_np,
+                  double *work, int ader_recon, int precision);
+/* number of work doubles exw_time_step needs per DoF */
+int exw_time_step_work_vectors(int integrator);
+
+/* ---- setup and norms --------------------------------------------------- */
+/* project_initial_field (source/wave_equation_operations.cc:902-935) */
+int exw_project(exw_op op, const exw_modes *modes, double t, double *dst);
+/* integrate_difference with QGauss(k+2) (source/explicit_wave.cc:452-496):
+ * out_host[0] = ||p_h - p||, [1] = ||v_h - v||, [2] = ||p||; synchronises. */
+int exw_l2_errors(exw_op op, const exw_modes *modes, double t, const double *u,
+                  double *out_host);
+/* E = 1/2 int rho|v|^2 + p^2/(rho c^2) with QGauss(k+1); synchronises. */
+int exw_energy(exw_op op, const double *u, double *out_host);
+
+/* ---- problem setup and the whole solve ---------------------------------- */
+/* transforms of the unit hypercube's vertices */
+#define EXW_TRANSFORM_NONE 0
+#define EXW_TRANSFORM_REFERENCE_SINE 1 /* x0 += f prod_d sin(pi x_d), include/input_parameters.h:41-50 */
+#define EXW_TRANSFORM_AXIS_2PI 2       /* x_i += f prod_{e!=i} sin(2 pi x_e) */
+#define EXW_TRANSFORM_PRODUCT_2PI 3    /* x_i += f prod_e sin(2 pi x_e) (keeps the unit cube) */
+
+/* GridGenerator::subdivided_hyper_cube(n, 0, 1) + transform of the coarse
+ * vertices + refine_global(r) with flat (multilinear) refinement
+ * (input_geometry_description, include/input_parameters.h:59-88).  Cells are
+ * numbered lexicographically on the fine grid.  Call with vertices = neighbor
+ * = NULL to get n_cells; then pass arrays of n_cells*2^dim*dim doubles and
+ * n_cells*2*dim ints. */
+int exw_mesh_hypercube(int dim, int n_intervals, int n_refinements, int transform, double factor,
+                       int boundary_id, int *n_cells, double *vertices, int *neighbor);
+
+/* Parameters of one solve (the keys of the reference's prm files that the
+ * hot path uses, source/parameters.cc:36-100). */
+typedef struct exw_problem {
+  int dim, degree, n_intervals, n_refinements, transform;
+  double transform_factor;
+  int boundary_id;
+  int integrator;      /* EXW_EXPL_EULER .. EXW_ADER */
+  double cfl_number;   /* as written in the prm file (divided by k^1.5 internally) */
+  int max_time_steps;  /* 0: derived from final_time */
+  double final_time, output_every_time;
+  int ader_recon;      /* use_ader_post */
+  int precision;       /* EXW_FP64 / EXW_FP32 operator */
+  exw_modes modes;     /* initial condition and exact solution */
+} exw_problem;
+
+/* WaveEquationProblem<dim>::run (source/explicit_wave.cc:549-813): mesh,
+ * operator, projection, TimeControl with the reference's dt rules
+ * (source/utilities.cc:31-95, source/explicit_wave.cc:222), the time loop and
+ * the L2 errors at every output tick.  Writes one line per tick
+ *   "<step> <time> <error p> <error v> <solution mag p>\n"
+ * into log (if not NULL).  Returns the number of steps (>= 0) or an error
+ * code.  wall_seconds (optional) receives the time-loop wall time excluding
+ * the error evaluations. */
+int exw_run(exw_ctx ctx, const exw_problem *p, char *log, int log_len, double *wall_seconds);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

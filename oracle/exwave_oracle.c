@@ -642,7 +642,8 @@ void ox_ader_ck(const ox_mesh *m, int degree, const double *src, const double *r
           s[dim * nd + q] = -np;
         }
       }
-      if (degree >= 2) ck_step(&X, 0, 1, s, contrib);
+      /* step 0 runs for every degree (the recursion guard is inside) */
+      ck_step(&X, 0, 1, s, contrib);
       memcpy(tempsrc + (long)c * nf * nd, contrib, sizeof(double) * nf * nd);
     }
     free(s);

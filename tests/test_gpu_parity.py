@@ -1,0 +1,222 @@
+"""GPU parity: the HIP path (through the C-ABI) against the CPU FP64 oracle on the
+same seeded inputs, plus the reference fixtures run end to end on the GPU.
+
+Tolerances: FP64 operator <= 1e-12 relative in max-norm (the plain FP64 path's
+bar); FP32 operator <= 2e-5 relative; full solves print the reference's
+4-digit error lines exactly.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+FP32_TOL = 2e-5
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def X():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    import exwave_amd
+    exwave_amd.load()
+    return exwave_amd
+
+
+@pytest.fixture(scope="module")
+def ctx(X):
+    return X.Context(0)
+
+
+def _pair(X, ctx, dim, nint, k, transform=0, factor=0.0, bid=2, nref=0, seed=None):
+    om = O.Mesh(dim, nint, nref, transform, factor, bid)
+    if seed is not None:
+        rng = np.random.default_rng(seed)
+        om.set_material(rng.uniform(0.5, 2.0, om.n_cells), rng.uniform(0.5, 2.0, om.n_cells))
+    op = X.WaveOperator(ctx, dim, k, om.vertices, om.neighbor, om.rho, om.speed)
+    assert op.n_dofs == om.n_dofs(k)
+    return om, op
+
+
+def _dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def _host(t):
+    import torch
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def _rel(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+APPLY_CASES = [
+    # dim, nint, k, transform, factor, bid, material seed
+    (2, 4, 1, 0, 0.0, 2, None), (2, 4, 3, 1, 0.1, 2, 1), (2, 3, 5, 1, 0.1, 1, 2),
+    (2, 3, 8, 0, 0.0, 3, 3), (2, 5, 2, 3, 0.04, 2, None),
+    (3, 3, 1, 1, 0.1, 2, 4), (3, 3, 2, 0, 0.0, 1, None), (3, 3, 4, 0, 0.0, 2, None),
+    (3, 3, 4, 1, 0.1, 2, 5), (3, 3, 6, 2, 0.05, 3, 6), (3, 2, 7, 1, 0.05, 2, None),
+    (3, 2, 8, 0, 0.0, 2, 7),
+]
+
+
+@pytest.mark.parametrize("case", APPLY_CASES, ids=lambda c: "d%d_n%d_k%d_t%d_b%d" % (
+    c[0], c[1], c[2], c[3], c[5]))
+def test_apply_parity(X, ctx, case):
+    dim, nint, k, tr, f, bid, seed = case
+    _say("apply parity", case)
+    om, op = _pair(X, ctx, dim, nint, k, tr, f, bid, seed=seed)
+    assert op.is_cartesian == (tr == 0)
+    rng = np.random.default_rng(100 + k)
+    u = rng.standard_normal(om.n_dofs(k))
+    want = om.apply(k, u)
+    src = _dev(u)
+    dst = op.new_vector()
+    op.apply(src, dst)
+    assert _rel(_host(dst), want) <= FP64_TOL
+    op.apply(src, dst, precision=X.FP32)
+    assert _rel(_host(dst), want) <= FP32_TOL
+
+
+def test_rk_stage_matches_updater(X, ctx):
+    """exw_rk_stage == apply + RKVectorUpdater (source/time_integrators.cc:128-155)."""
+    dim, k = 3, 4
+    om, op = _pair(X, ctx, dim, 3, k, 1, 0.1, 2, seed=9)
+    rng = np.random.default_rng(3)
+    N = om.n_dofs(k)
+    y, b = rng.standard_normal(N), rng.standard_normal(N)
+    f1, f2 = 0.013, -0.007
+    kk = om.apply(k, y)
+    v2 = b.copy()
+    v3 = np.zeros(N)
+    O.lib().ox_rk_update(N, f1, f2, 0, O._p(kk), O._p(v2), O._p(v3))
+    dy, db, dbo = _dev(y), _dev(b), op.new_vector()
+    op.rk_stage(dy, db, db, dbo, f1, f2, False)  # y_out over b in place, b_out separate
+    assert _rel(_host(db), v2) <= FP64_TOL
+    assert _rel(_host(dbo), v3) <= FP64_TOL
+    # last stage
+    v2 = y.copy()
+    v3 = b.copy()
+    O.lib().ox_rk_update(N, f1, 0.0, 1, O._p(kk), O._p(v2), O._p(v3))
+    db2, out = _dev(b), op.new_vector()
+    op.rk_stage(dy, db2, out, None, f1, 0.0, True)
+    assert _rel(_host(out), v2) <= FP64_TOL
+
+
+@pytest.mark.parametrize("integ", [O.EXPL_EULER, O.CLRK4, O.LSRK45R2, O.LSRK33R2, O.LSRK59R2,
+                                   O.ADER])
+@pytest.mark.parametrize("geom", ["cart3d", "deformed2d"])
+def test_time_step_parity(X, ctx, integ, geom):
+    """exw_time_step (fused stages, 3-register rotation) == the oracle's
+    reference-order integrator on the same state."""
+    _say("time step parity", integ, geom)
+    if geom == "cart3d":
+        dim, k, om_op = 3, 4, _pair(X, ctx, 3, 3, 4)
+    else:
+        dim, k, om_op = 2, 3, _pair(X, ctx, 2, 4, 3, 1, 0.1, 2, seed=11)
+    om, op = om_op
+    N = om.n_dofs(k)
+    u0 = om.project(k, O.Exact.make(dim, [[1] * dim, [2, 3, 1][:dim]]), 0.0)
+    u0 += 1e-3 * np.random.default_rng(5).standard_normal(N)
+    dt = 0.2 / k ** 1.5 * om.min_vertex_distance_over_c
+    for recon in ([0, 1] if integ == O.ADER else [0]):
+        vn, vnp = u0.copy(), np.zeros(N)
+        om.time_step(k, integ, dt, vn, vnp, ader_recon=recon)
+        dn, dnp = _dev(u0), op.new_vector()
+        nw = op.work_vectors(integ)
+        work = op.new_vector(max(nw, 1))
+        op.time_step(integ, dt, dn, dnp, work, ader_recon=bool(recon))
+        assert _rel(_host(dnp), vnp) <= FP64_TOL, (integ, recon)
+
+
+@pytest.mark.parametrize("dim,k,tr", [(2, 3, 1), (2, 6, 0), (3, 4, 0), (3, 5, 1), (3, 7, 1),
+                                      (2, 1, 0), (3, 2, 1)])
+def test_ader_ck_parity(X, ctx, dim, k, tr):
+    _say("ader ck parity", dim, k, tr)
+    om, op = _pair(X, ctx, dim, 3, k, tr, 0.08, 2, seed=21)
+    rng = np.random.default_rng(8)
+    N = om.n_dofs(k)
+    u = rng.standard_normal(N)
+    rec = rng.standard_normal(N)
+    dt = 0.01
+    for r in (None, rec):
+        want = om.ader_ck(k, u, r, dt, 0.0, 0.0)
+        out = op.new_vector()
+        op.ader_ck(_dev(u), None if r is None else _dev(r), dt, 0.0, 0.0, out)
+        assert _rel(_host(out), want) <= FP64_TOL
+    # general interval (LTS form): t1 != 0, te != 0
+    want = om.ader_ck(k, u, None, 0.013, 0.004, 0.002)
+    out = op.new_vector()
+    op.ader_ck(_dev(u), None, 0.013, 0.004, 0.002, out)
+    assert _rel(_host(out), want) <= FP64_TOL
+
+
+@pytest.mark.parametrize("dim,k,tr", [(2, 3, 1), (3, 4, 1), (3, 2, 0)])
+def test_project_and_norms(X, ctx, dim, k, tr):
+    om, op = _pair(X, ctx, dim, 3, k, tr, 0.1, 2)
+    e = O.Exact.make(dim, [[1] * dim, [2, 3, 1][:dim]], amps=[1.0, 0.5])
+    m = X.Modes.make(dim, [[1] * dim, [2, 3, 1][:dim]], amps=[1.0, 0.5])
+    want = om.project(k, e, 0.3)
+    d = op.new_vector()
+    op.project(m, 0.3, d)
+    assert _rel(_host(d), want) <= 1e-14
+    errs = np.array(op.l2_errors(m, 0.35, d))
+    werr = om.l2_errors(k, e, 0.35, want)
+    assert np.max(np.abs(errs - werr) / werr) <= 1e-11
+    assert abs(op.energy(d) - om.energy(k, want)) <= 1e-12 * om.energy(k, want)
+
+
+FIXTURES = ["lsrk45r2_2d_cartesian", "clrk4_3d", "ader_2d_norecon_ref2", "ader_2d_recon_ref2"]
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_reference_fixture_on_gpu(X, ctx, fixtures, name):
+    """The reference's regression tests run end to end on the GPU (exw_run):
+    every 'error' line at 4 printed digits, number of steps."""
+    _say("fixture on gpu", name)
+    fx = fixtures[name]
+    prm = fx["params"]
+    p = X.Problem()
+    p.dim, p.degree = prm["dimension"], prm["fe_degree"]
+    p.n_intervals, p.n_refinements = prm["n_initial_intervals"], prm["n_refinements"]
+    p.transform, p.transform_factor = X.TRANSFORM_REFERENCE_SINE, prm["grid_transform_factor"]
+    p.boundary_id = 2
+    p.integrator = X.INTEGRATORS[prm["time_integrator"]]
+    p.cfl_number, p.max_time_steps = prm["cfl_number"], prm["max_time_steps"]
+    p.final_time, p.output_every_time = prm["final_time"], prm["output_every_time"]
+    p.ader_recon = 1 if prm.get("use_ader_post") else 0
+    p.precision = X.FP64
+    p.modes = X.Modes.reference(p.dim, prm["membrane_modes"])
+    steps, rows, _ = X.run(ctx, p)
+    assert steps == fx["n_steps"]
+    for (st, t, ep, ev, mp), ref in zip(rows, fx["error_lines"]):
+        assert "%.2f" % t == ref[0]
+        assert ["%.4e" % ep, "%.4e" % ev, "%.4e" % mp] == [ref[1], ref[3], ref[4]], (t, ref)
+
+
+def test_error_paths(X, ctx):
+    om = O.Mesh(2, 2, 0, 0, 0.0, 2)
+    nb = om.neighbor.copy()
+    nb[nb < 0] = -7
+    with pytest.raises(X.ExwError):
+        X.WaveOperator(ctx, 2, 2, om.vertices, nb, om.rho, om.speed)
+    with pytest.raises(X.ExwError):
+        X.WaveOperator(ctx, 2, 13, om.vertices, om.neighbor, om.rho, om.speed)
+    op = X.WaveOperator(ctx, 2, 2, om.vertices, om.neighbor, om.rho, om.speed)
+    v = op.new_vector()
+    with pytest.raises(X.ExwError):
+        op.apply(v, v)  # output aliases the input (neighbour reads would race)
+    with pytest.raises(X.ExwError):
+        op.time_step(17, 0.1, v, op.new_vector(), op.new_vector())
