@@ -85,6 +85,7 @@ SIGNATURES = [
     ("exw_op_apply", _I, [_VP, _VP, _VP, _I]),
     ("exw_op_apply_fused", _I, [_VP, _VP, C.POINTER(Epilogue), _I]),
     ("exw_rk_stage", _I, [_VP, _VP, _VP, _VP, _VP, _D, _D, _I, _I]),
+    ("exw_rk_update", _I, [_VP, C.c_int64, _D, _D, _I, _VP, _VP, _VP]),
     ("exw_ader_ck", _I, [_VP, _VP, _VP, _D, _D, _D, _VP, _I]),
     ("exw_ader_step", _I, [_VP, _VP, _VP, _VP, _D, _I, _I]),
     ("exw_time_step", _I, [_VP, _I, _D, _VP, _VP, _VP, _I, _I]),

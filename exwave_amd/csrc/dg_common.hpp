@@ -58,6 +58,16 @@ struct Epilogue {
   int o2_from_o1;
 };
 
+// MI355X dispatches workgroup b to XCD b % 8.  Give every XCD one contiguous
+// range of logical blocks (a bijection for any grid size), so that cells that
+// are neighbours in the numbering share one L2.
+constexpr int N_XCD = 8;
+__device__ __forceinline__ int xcd_remap(int b, int nblk) {
+  const int q = nblk / N_XCD, r = nblk % N_XCD;
+  const int x = b % N_XCD, i = b / N_XCD;
+  return x * q + (x < r ? x : r) + i;
+}
+
 template <int N>
 __device__ __forceinline__ constexpr int pow_n(int e) {
   return e == 0 ? 1 : (e == 1 ? N : N * N);

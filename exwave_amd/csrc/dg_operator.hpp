@@ -2,17 +2,23 @@
 //
 // One workgroup processes CPB cells; one lane owns one Gauss node of one cell
 // (all dim+1 fields).  Per cell:
-//   1. the lanes load their node (coalesced, cell-contiguous layout) into LDS,
-//   2. cell integral (evaluate_cell, source/wave_equation_operations.cc:424-456):
+//   1. every independent global load is issued up front: the lane's own node,
+//      the epilogue operand(s) (RK accumulator), the cell's neighbour row;
+//   2. the neighbours' nodes are loaded node-aligned (coalesced exactly like
+//      the own node) into LDS while the cell integral runs from LDS
+//      (evaluate_cell, source/wave_equation_operations.cc:424-456):
 //        dv/dt += -rho^-1 grad p        (strong, collocation derivative),
-//        dp/dt += rho c^2 * weak divergence of v  (transposed derivative),
-//   3. face integrals of all 2*dim faces of the cell, evaluated from this cell's
-//      side (the inner-face flux is symmetric in +/-, so each cell computes its
-//      own half and nothing is scattered: no atomics), the neighbour's trace is
-//      interpolated from its nodes in global memory (L2/MALL resident),
-//   4. inverse mass = 1/(w_q det J_q) (diagonal in this basis),
+//        dp/dt += rho c^2 * weak divergence of v  (transposed derivative);
+//   3. face integrals of all 2*dim faces of the cell, evaluated from this
+//      cell's side (evaluate_inner_face / evaluate_boundary_face,
+//      :480-551, :569-629; the flux is symmetric in +/-, so each cell computes
+//      its own half and nothing is scattered: no atomics);
+//   4. inverse mass = 1/(w_q det J_q) (diagonal in this basis, :633-661);
 //   5. the epilogue writes the RK update (RKVectorUpdater,
-//      source/time_integrators.cc:128-155) straight from registers.
+//      source/time_integrators.cc:128-155) from registers with streaming stores.
+// Only two dependent global-memory round trips per cell (own + neighbour row,
+// then neighbour nodes).  Workgroups are remapped so each XCD walks a
+// contiguous range of cells (neighbours share an L2).
 // Number is the operator arithmetic (double, or float for the mixed path);
 // loads, the epilogue and all vectors are FP64.
 #pragma once
@@ -32,37 +38,48 @@ __global__ void __launch_bounds__((Shape<DIM, K>::BLOCK))
   using TO = TabOff<N>;
 
   __shared__ Number s_u[CPB][NF][ND];
-  __shared__ Number s_g[CART ? 1 : CPB][CART ? 1 : DIM][ND];
+  // Cartesian: the neighbours' v_n and p at every node (node-aligned loads)
+  __shared__ Number s_nb[CART ? CPB : 1][CART ? NFACE : 1][CART ? 2 : 1][CART ? ND : 1];
+  __shared__ Number s_g[CART ? 1 : CPB][CART ? 1 : DIM][CART ? 1 : ND];
   __shared__ Number s_F[CPB][NFACE][NFF][NT];
   __shared__ Number s_tab[TO::SIZE];
   __shared__ double s_V[CART ? 1 : CPB][CART ? 1 : (1 << DIM) * DIM];
+  __shared__ int s_nbr[CPB][NFACE];
+  __shared__ Number s_nmat[CPB][NFACE][2];
 
   const int tid = threadIdx.x;
   const int lc = tid / ND;
   const int q = tid - lc * ND;
-  const int cell = blockIdx.x * CPB + lc;
+  const int lblock = xcd_remap(blockIdx.x, gridDim.x);
+  const int cell = lblock * CPB + lc;
   const bool lane_ok = lc < CPB;
   const bool active = lane_ok && cell < mesh.n_cells;
+  const size_t cbase = (size_t)cell * NF * ND;
 
+  // ---------------- phase 1: independent loads ----------------
+  double uo[NF], bv[NF], cv[NF];
+#pragma unroll
+  for (int f = 0; f < NF; ++f) uo[f] = bv[f] = cv[f] = 0.0;
+  if (active) {
+#pragma unroll
+    for (int f = 0; f < NF; ++f) uo[f] = src[cbase + f * ND + q];
+    if (epi.b) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f) bv[f] = __builtin_nontemporal_load(epi.b + cbase + f * ND + q);
+    }
+    if (epi.o2 && !epi.o2_from_o1) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f) cv[f] = __builtin_nontemporal_load(epi.c + cbase + f * ND + q);
+    }
+    if (q < NFACE) s_nbr[lc][q] = mesh.neighbor[(size_t)cell * NFACE + q];
+  }
   for (int i = tid; i < TO::SIZE; i += blockDim.x) s_tab[i] = Number(tab[i]);
   if constexpr (!CART) {
     constexpr int NVD = (1 << DIM) * DIM;
     for (int i = tid; i < CPB * NVD; i += blockDim.x) {
-      const int c2 = blockIdx.x * CPB + i / NVD;
+      const int c2 = lblock * CPB + i / NVD;
       s_V[i / NVD][i % NVD] = c2 < mesh.n_cells ? mesh.vertices[(size_t)c2 * NVD + i % NVD] : 0.0;
     }
-  }
-
-  // node multi-index (xi_0 fastest)
-  int qi[3];
-  qi[0] = q % N;
-  qi[1] = (q / N) % N;
-  qi[2] = DIM == 3 ? q / (N * N) : 0;
-
-  const size_t cbase = (size_t)cell * NF * ND;
-  if (active) {
-#pragma unroll
-    for (int f = 0; f < NF; ++f) s_u[lc][f][q] = Number(src[cbase + f * ND + q]);
   }
   Number rho = Number(1), cc = Number(1), hh[DIM];
 #pragma unroll
@@ -77,7 +94,37 @@ __global__ void __launch_bounds__((Shape<DIM, K>::BLOCK))
   }
   const Number rho_inv = Number(1) / rho;
   const Number rc2 = rho * cc * cc;
+  if (lane_ok) {
+#pragma unroll
+    for (int f = 0; f < NF; ++f) s_u[lc][f][q] = Number(uo[f]);
+  }
+
+  // node multi-index (xi_0 fastest)
+  int qi[3];
+  qi[0] = q % N;
+  qi[1] = (q / N) % N;
+  qi[2] = DIM == 3 ? q / (N * N) : 0;
   __syncthreads();
+
+  // ---------------- phase 2: neighbour loads, overlapped with the cell term ----
+  double nbv[CART ? NFACE : 1][2];
+  if constexpr (CART) {
+#pragma unroll
+    for (int f = 0; f < NFACE; ++f) {
+      const int nb = active ? s_nbr[lc][f] : -1;
+      nbv[f][0] = nbv[f][1] = 0.0;
+      if (nb >= 0) {
+        const double *un = src + (size_t)nb * NF * ND;
+        nbv[f][0] = un[(f >> 1) * ND + q];
+        nbv[f][1] = un[DIM * ND + q];
+      }
+    }
+  }
+  if (active && q < NFACE) {
+    const int nb = s_nbr[lc][q];
+    s_nmat[lc][q][0] = nb >= 0 ? Number(mesh.rho[nb]) : rho;
+    s_nmat[lc][q][1] = nb >= 0 ? Number(mesh.speed[nb]) : cc;
+  }
 
   const Number *D = s_tab + TO::D;
   const Number *DT = s_tab + TO::DT;
@@ -89,7 +136,6 @@ __global__ void __launch_bounds__((Shape<DIM, K>::BLOCK))
   Number kv[DIM], kp = Number(0);
   Number jxw_node = Number(1);  // w_q det J_q (general geometry)
 
-  // ---------------- cell integral ----------------
   if (lane_ok) {
     if constexpr (CART) {
 #pragma unroll
@@ -107,6 +153,11 @@ __global__ void __launch_bounds__((Shape<DIM, K>::BLOCK))
         kp += dv * ih;
       }
       kp *= rc2;
+#pragma unroll
+      for (int f = 0; f < NFACE; ++f) {
+        s_nb[lc][f][0][q] = Number(nbv[f][0]);
+        s_nb[lc][f][1][q] = Number(nbv[f][1]);
+      }
     } else {
       Number xi[DIM], J[DIM * DIM], Ji[DIM * DIM];
       Number wq = Number(1);
@@ -144,6 +195,7 @@ __global__ void __launch_bounds__((Shape<DIM, K>::BLOCK))
       }
     }
   }
+  __syncthreads();
 
   // ---------------- faces ----------------
   if (lane_ok) {
@@ -161,13 +213,10 @@ __global__ void __launch_bounds__((Shape<DIM, K>::BLOCK))
                         : (d == 1 ? tdig[0] + tdig[1] * N * N : tdig[0] + tdig[1] * N);
       }
       const int sd = d == 0 ? 1 : (d == 1 ? N : N * N);
-      const int nb = active ? mesh.neighbor[(size_t)cell * NFACE + f] : -2;
+      const int nb = active ? s_nbr[lc][f] : -2;
       const Number *Ls = L + s * N, *Lo = L + (1 - s) * N;
-      Number rho_m = rho, c_m = cc;
-      if (nb >= 0) {
-        rho_m = Number(mesh.rho[nb]);
-        c_m = Number(mesh.speed[nb]);
-      }
+      const Number rho_m = active ? s_nmat[lc][f][0] : rho;
+      const Number c_m = active ? s_nmat[lc][f][1] : cc;
       const int bid = nb >= 0 ? 0 : -nb;
       if constexpr (CART) {
         Number vd = Number(0), pp = Number(0), vdm = Number(0), pm = Number(0);
@@ -175,22 +224,14 @@ __global__ void __launch_bounds__((Shape<DIM, K>::BLOCK))
         for (int i = 0; i < N; ++i) {
           vd += Ls[i] * s_u[lc][d][base + i * sd];
           pp += Ls[i] * s_u[lc][DIM][base + i * sd];
-        }
-        if (nb >= 0) {
-          const double *un = src + (size_t)nb * NF * ND;
-#pragma unroll
-          for (int i = 0; i < N; ++i) {
-            vdm += Lo[i] * Number(un[d * ND + base + i * sd]);
-            pm += Lo[i] * Number(un[DIM * ND + base + i * sd]);
-          }
+          vdm += Lo[i] * s_nb[lc][f][0][base + i * sd];
+          pm += Lo[i] * s_nb[lc][f][1][base + i * sd];
         }
         const Number sg = s ? Number(1) : Number(-1);
         Number val_pn, val_p;
         flux<Number>(sg * vd, pp, sg * vdm, pm, rho, cc, rho_m, c_m, bid, val_pn, val_p);
-        if (lc < CPB) {
-          s_F[lc][f][0][t] = sg * val_pn;
-          s_F[lc][f][1][t] = val_p;
-        }
+        s_F[lc][f][0][t] = sg * val_pn;
+        s_F[lc][f][1][t] = val_p;
       } else {
         Number up[NF], um[NF];
 #pragma unroll
@@ -244,11 +285,9 @@ __global__ void __launch_bounds__((Shape<DIM, K>::BLOCK))
         }
         Number val_pn, val_p;
         flux<Number>(nvp, up[DIM], nvm, um[DIM], rho, cc, rho_m, c_m, bid, val_pn, val_p);
-        if (lc < CPB) {
 #pragma unroll
-          for (int i = 0; i < DIM; ++i) s_F[lc][f][i][t] = val_pn * nrm[i] * dS;
-          s_F[lc][f][DIM][t] = val_p * dS;
-        }
+        for (int i = 0; i < DIM; ++i) s_F[lc][f][i][t] = val_pn * nrm[i] * dS;
+        s_F[lc][f][DIM][t] = val_p * dS;
       }
     }
   }
@@ -309,15 +348,16 @@ __global__ void __launch_bounds__((Shape<DIM, K>::BLOCK))
     }
   }
 
-  // ---------------- fused epilogue (FP64) ----------------
+  // ---------------- fused epilogue (FP64, streaming stores) ----------------
   if (active) {
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
       const size_t idx = cbase + f * ND + q;
       const double kk = double(f < DIM ? kv[f < DIM ? f : 0] : kp);
-      const double o1 = (epi.b ? epi.b[idx] : 0.0) + epi.a1 * kk;
-      epi.o1[idx] = o1;
-      if (epi.o2) epi.o2[idx] = (epi.o2_from_o1 ? o1 : epi.c[idx]) + epi.a2 * kk;
+      const double o1 = bv[f] + epi.a1 * kk;
+      __builtin_nontemporal_store(o1, epi.o1 + idx);
+      if (epi.o2)
+        __builtin_nontemporal_store((epi.o2_from_o1 ? o1 : cv[f]) + epi.a2 * kk, epi.o2 + idx);
     }
   }
 }

@@ -1,0 +1,222 @@
+// dg_operator_cart.hpp -- fused operator + RK epilogue for Cartesian (box)
+// cells: the hot path of the 3D k=4 benchmark.
+//
+// On a box cell J = diag(h) and the normal of face 2d+s is +-e_d, so the
+// operator splits by direction (evaluate_cell and the face fluxes,
+// source/wave_equation_operations.cc:424-551, 569-629):
+//   dv_d/dt = -rho^-1 d_d p / h_d                     + lifting of faces +-d
+//   dp/dt   = sum_d rho c^2 (weak d_d v_d) / h_d      + lifting of faces +-d
+// and every term of direction d only needs v_d and p on lines parallel to e_d.
+//
+// One workgroup per cell batch, three phases:
+//   A (lane = node): coalesced loads of the cell's nodes, of the epilogue
+//     operand and of the six neighbours' (v_n, p) nodes, staged in LDS;
+//   B (lane = line): each of the dim*N^(dim-1) lines of the cell is read
+//     from LDS into registers once; the collocation derivative, the weak
+//     derivative, both face traces, both fluxes and the face lifting are
+//     register FMAs with wave-uniform 1D matrices taken from the kernel
+//     arguments; the line's dv_d/dt and its share of dp/dt go back to LDS over
+//     the neighbour staging it has just consumed (only this lane reads it);
+//   C (lane = node): sum the pressure shares, fused RK epilogue (FP64,
+//     RKVectorUpdater semantics, source/time_integrators.cc:128-155) with
+//     streaming stores.
+// Global traffic is coalesced exactly like a streaming kernel; LDS traffic is
+// one read + one write per value and line instead of N reads per derivative.
+#pragma once
+#include "dg_common.hpp"
+
+namespace exw {
+
+template <int DIM, int K>
+struct CartShape {
+  static constexpr int N = K + 1;
+  static constexpr int ND = DIM == 2 ? N * N : N * N * N;
+  static constexpr int NT = DIM == 2 ? N : N * N;  // lines per direction
+  static constexpr int NF = DIM + 1;
+  static constexpr int NLINE = DIM * NT;           // lines per cell
+  static constexpr int LANES = ND > NLINE ? ND : NLINE;
+  // cells per workgroup: about 128 lanes per group (more groups in flight)
+  static constexpr int CPB = LANES >= 128 ? 1 : 128 / LANES;
+  static constexpr int BLOCK = ((CPB * LANES + 63) / 64) * 64;
+};
+
+// 1D matrices of degree K (meaning as TabOff), passed by value so that every
+// lane reads them from scalar registers.
+template <int N>
+struct CartTab {
+  double D[N * N];
+  double DT[N * N];
+  double L[2 * N];
+  double LW[2 * N];
+};
+
+template <int DIM, int K, typename Number>
+__global__ void __launch_bounds__((CartShape<DIM, K>::BLOCK))
+    dg_operator_cart_kernel(const DevMesh mesh, const double *__restrict__ src,
+                            const CartTab<K + 1> tab, const Epilogue epi) {
+  using S = CartShape<DIM, K>;
+  constexpr int N = S::N, ND = S::ND, NT = S::NT, NF = S::NF, LANES = S::LANES, CPB = S::CPB;
+  constexpr int NFACE = 2 * DIM;
+  __shared__ Number s_u[CPB][NF][ND];
+  // neighbour (v_n, p) nodes of face f; phase B overwrites [2d][0] with the
+  // lines' dv_d/dt and [2d][1] with their dp/dt share
+  __shared__ Number s_nb[CPB][NFACE][2][ND];
+  __shared__ int s_nbr[CPB][NFACE];
+  __shared__ Number s_nmat[CPB][NFACE][2];
+
+  const int tid = threadIdx.x;
+  const int lc = tid / LANES;
+  const int lane = tid - lc * LANES;
+  const int cell = xcd_remap(blockIdx.x, gridDim.x) * CPB + lc;
+  const bool lane_ok = lc < CPB;
+  const bool active = lane_ok && cell < mesh.n_cells;
+  const bool node_ok = active && lane < ND;
+  const size_t cbase = (size_t)cell * NF * ND;
+
+  // ---------------- A: own nodes, epilogue operand, neighbours ----------------
+  // the neighbour row is uniform over the cell's lanes (scalar loads for one
+  // cell per group)
+  int nbr[NFACE];
+#pragma unroll
+  for (int f = 0; f < NFACE; ++f) nbr[f] = active ? mesh.neighbor[(size_t)cell * NFACE + f] : -1;
+  double uo[NF], bv[NF], nv[NFACE][2];
+#pragma unroll
+  for (int f = 0; f < NF; ++f) uo[f] = bv[f] = 0.0;
+  if (node_ok) {
+#pragma unroll
+    for (int f = 0; f < NF; ++f) uo[f] = src[cbase + f * ND + lane];
+    if (epi.b) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f)
+        bv[f] = __builtin_nontemporal_load(epi.b + cbase + f * ND + lane);
+    }
+  }
+#pragma unroll
+  for (int f = 0; f < NFACE; ++f) {
+    nv[f][0] = nv[f][1] = 0.0;
+    if (node_ok && nbr[f] >= 0) {
+      const double *un = src + (size_t)nbr[f] * NF * ND;
+      nv[f][0] = un[(f >> 1) * ND + lane];
+      nv[f][1] = un[DIM * ND + lane];
+    }
+  }
+  Number rho = Number(1), cc = Number(1);
+  if (active) {
+    rho = Number(mesh.rho[cell]);
+    cc = Number(mesh.speed[cell]);
+  }
+  if (active && lane < NFACE) {
+    int nb = nbr[0];
+#pragma unroll
+    for (int f = 1; f < NFACE; ++f)
+      if (lane == f) nb = nbr[f];
+    s_nbr[lc][lane] = nb;
+    s_nmat[lc][lane][0] = nb >= 0 ? Number(mesh.rho[nb]) : rho;
+    s_nmat[lc][lane][1] = nb >= 0 ? Number(mesh.speed[nb]) : cc;
+  }
+  if (node_ok) {
+#pragma unroll
+    for (int f = 0; f < NF; ++f) s_u[lc][f][lane] = Number(uo[f]);
+#pragma unroll
+    for (int f = 0; f < NFACE; ++f) {
+      s_nb[lc][f][0][lane] = Number(nv[f][0]);
+      s_nb[lc][f][1][lane] = Number(nv[f][1]);
+    }
+  }
+  __syncthreads();
+
+  // ---------------- B: one lane per line ----------------
+  if (active && lane < S::NLINE) {
+    const int d = lane / NT, t = lane - d * NT;
+    int base, sd;
+    if constexpr (DIM == 2) {
+      base = d == 0 ? t * N : t;
+      sd = d == 0 ? 1 : N;
+    } else {
+      const int t0 = t % N, t1 = t / N;
+      base = d == 0 ? t0 * N + t1 * N * N : (d == 1 ? t0 + t1 * N * N : t0 + t1 * N);
+      sd = d == 0 ? 1 : (d == 1 ? N : N * N);
+    }
+    const Number hd = Number(mesh.h[(size_t)cell * DIM + d]);
+    const Number ih = Number(1) / hd, rho_inv = Number(1) / rho, rc2 = rho * cc * cc;
+    Number vl[N], pl[N], vm[N], pm[N], vp[N], pp[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const int n = base + i * sd;
+      vl[i] = s_u[lc][d][n];
+      pl[i] = s_u[lc][DIM][n];
+      vm[i] = s_nb[lc][2 * d][0][n];
+      pm[i] = s_nb[lc][2 * d][1][n];
+      vp[i] = s_nb[lc][2 * d + 1][0][n];
+      pp[i] = s_nb[lc][2 * d + 1][1][n];
+    }
+    Number kv[N], kp[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      Number gp = Number(0), dv = Number(0);
+#pragma unroll
+      for (int m = 0; m < N; ++m) {
+        gp += Number(tab.D[i * N + m]) * pl[m];
+        dv += Number(tab.DT[i * N + m]) * vl[m];
+      }
+      kv[i] = -rho_inv * gp * ih;
+      kp[i] = rc2 * dv * ih;
+    }
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      Number vo = Number(0), po = Number(0), vn = Number(0), pn = Number(0);
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        const Number ls = Number(tab.L[s * N + i]), lo = Number(tab.L[(1 - s) * N + i]);
+        vo += ls * vl[i];
+        po += ls * pl[i];
+        vn += lo * (s ? vp[i] : vm[i]);
+        pn += lo * (s ? pp[i] : pm[i]);
+      }
+      const int f = 2 * d + s;
+      const int nb = s_nbr[lc][f];
+      const Number sg = s ? Number(1) : Number(-1);
+      Number val_pn, val_p;
+      flux<Number>(sg * vo, po, sg * vn, pn, rho, cc, s_nmat[lc][f][0], s_nmat[lc][f][1],
+                   nb >= 0 ? 0 : -nb, val_pn, val_p);
+      val_pn *= sg * ih;
+      val_p *= ih;
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        const Number lw = Number(tab.LW[s * N + i]);
+        kv[i] += lw * val_pn;
+        kp[i] += lw * val_p;
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const int n = base + i * sd;
+      s_nb[lc][2 * d][0][n] = kv[i];
+      s_nb[lc][2 * d][1][n] = kp[i];
+    }
+  }
+  __syncthreads();
+
+  // ---------------- C: one lane per node, fused epilogue ----------------
+  if (node_ok) {
+    Number k[NF];
+    Number kp = s_nb[lc][0][1][lane];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      k[d] = s_nb[lc][2 * d][0][lane];
+      if (d > 0) kp += s_nb[lc][2 * d][1][lane];
+    }
+    k[DIM] = kp;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      const size_t idx = cbase + f * ND + lane;
+      const double kk = double(k[f]);
+      const double o1 = bv[f] + epi.a1 * kk;
+      __builtin_nontemporal_store(o1, epi.o1 + idx);
+      if (epi.o2)
+        __builtin_nontemporal_store((epi.o2_from_o1 ? o1 : epi.c[idx]) + epi.a2 * kk, epi.o2 + idx);
+    }
+  }
+}
+
+}  // namespace exw

@@ -53,6 +53,7 @@ struct exw_op_s {
   double *d_partial = nullptr, *d_sum = nullptr;
   int n_partial = 0;
   CkTab ck{};
+  std::vector<double> host_tab;
   DevMesh view() const {
     DevMesh m;
     m.n_cells = n_cells;
@@ -277,6 +278,7 @@ int exw_op_create(exw_ctx ctx, int dim, int degree, int n_cells, const double *v
   }
   ck.size = (int)ckt.size();
   op->ck = ck;
+  op->host_tab = tab;
   if ((rc = upload(op, &op->d_tab, tab.data(), tab.size())) ||
       (rc = upload(op, &op->d_errtab, etab.data(), etab.size())) ||
       (rc = upload(op, &op->d_cktab, ckt.data(), ckt.size()))) {
@@ -319,7 +321,7 @@ int exw_op_apply_fused(exw_op op, const double *src, const exw_epilogue *e, int 
   Epilogue ep{e->b, e->o1, e->a1, e->c, e->o2, e->a2, e->o2_from_o1};
   OpLaunch l = kernels(op->dim).op[op->degree][precision][op->cart ? 1 : 0];
   if (!l) return fail(EXW_EUNSUP, "operator kernel not compiled");
-  EXW_HIP(l(op->view(), src, op->d_tab, ep, op->ctx->stream));
+  EXW_HIP(l(op->view(), src, op->d_tab, op->host_tab.data(), ep, op->ctx->stream));
   return EXW_OK;
 }
 
@@ -334,6 +336,40 @@ int exw_rk_stage(exw_op op, const double *src, const double *b, double *y_out, d
   if (!is_last && !b_out) return fail(EXW_EINVAL, "exw_rk_stage: b_out is NULL");
   exw_epilogue e{b, y_out, f1, nullptr, is_last ? nullptr : b_out, f2, 1};
   return exw_op_apply_fused(op, src, &e, precision);
+}
+
+}  // extern "C"
+
+namespace {
+__global__ void __launch_bounds__(256)
+    rk_update_kernel(int64_t n, double f1, double f2, int is_last, const double *__restrict__ v1,
+                     double *__restrict__ v2, double *__restrict__ v3) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double up = v1[i];
+    if (is_last) {
+      v2[i] = v3[i] + f1 * up;
+    } else {
+      const double y = v2[i] + f1 * up;
+      v2[i] = y;
+      v3[i] = y + f2 * up;
+    }
+  }
+}
+}  // namespace
+
+extern "C" {
+
+int exw_rk_update(exw_ctx ctx, int64_t n, double factor1, double factor2, int is_last,
+                  const double *vector1, double *vector2, double *vector3) {
+  if (!ctx || !vector1 || !vector2 || !vector3 || n < 0)
+    return fail(EXW_EINVAL, "exw_rk_update: null argument");
+  if (n == 0) return EXW_OK;
+  const int64_t blocks = std::min<int64_t>((n + 255) / 256, 65536);
+  hipLaunchKernelGGL(rk_update_kernel, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, n,
+                     factor1, factor2, is_last, vector1, vector2, vector3);
+  EXW_HIP(hipGetLastError());
+  return EXW_OK;
 }
 
 int exw_ader_ck(exw_op op, const double *src, const double *recon, double t2, double t1,

@@ -4,13 +4,15 @@
 #include "dg_ader.hpp"
 #include "dg_misc.hpp"
 #include "dg_operator.hpp"
+#include "dg_operator_cart.hpp"
 
 namespace exw {
 
 constexpr int MAX_DEGREE = 8;
 
-using OpLaunch = hipError_t (*)(const DevMesh &, const double *, const double *, const Epilogue &,
-                                hipStream_t);
+// (mesh, src, device basis table, host basis table, epilogue, stream)
+using OpLaunch = hipError_t (*)(const DevMesh &, const double *, const double *, const double *,
+                                const Epilogue &, hipStream_t);
 using CkLaunch = hipError_t (*)(const DevMesh &, const double *, const double *, const double *,
                                 const CkTab &, double, double, double, double *, hipStream_t);
 using ProjLaunch = hipError_t (*)(const DevMesh &, const double *, const DevModes &, double,
@@ -31,13 +33,33 @@ struct DimKernels {
 };
 
 template <int DIM, int K, typename Number, bool CART>
-hipError_t launch_op(const DevMesh &m, const double *src, const double *tab, const Epilogue &e,
-                     hipStream_t s) {
-  using S = Shape<DIM, K>;
-  const int grid = (m.n_cells + S::CPB - 1) / S::CPB;
-  if (grid == 0) return hipSuccess;
-  hipLaunchKernelGGL((dg_operator_kernel<DIM, K, Number, CART>), dim3(grid), dim3(S::BLOCK), 0,
-                     s, m, src, tab, e);
+hipError_t launch_op(const DevMesh &m, const double *src, const double *tab,
+                     const double *host_tab, const Epilogue &e, hipStream_t s) {
+  if constexpr (CART) {
+    // box cells: node/line/node kernel with the 1D matrices in the kernel arguments
+    using CS = CartShape<DIM, K>;
+    constexpr int N = K + 1;
+    using TO = TabOff<N>;
+    CartTab<N> ct;
+    for (int i = 0; i < N * N; ++i) {
+      ct.D[i] = host_tab[TO::D + i];
+      ct.DT[i] = host_tab[TO::DT + i];
+    }
+    for (int i = 0; i < 2 * N; ++i) {
+      ct.L[i] = host_tab[TO::L + i];
+      ct.LW[i] = host_tab[TO::LW + i];
+    }
+    const int grid = (m.n_cells + CS::CPB - 1) / CS::CPB;
+    if (grid == 0) return hipSuccess;
+    hipLaunchKernelGGL((dg_operator_cart_kernel<DIM, K, Number>), dim3(grid), dim3(CS::BLOCK), 0,
+                       s, m, src, ct, e);
+  } else {
+    using S = Shape<DIM, K>;
+    const int grid = (m.n_cells + S::CPB - 1) / S::CPB;
+    if (grid == 0) return hipSuccess;
+    hipLaunchKernelGGL((dg_operator_kernel<DIM, K, Number, CART>), dim3(grid), dim3(S::BLOCK), 0,
+                       s, m, src, tab, e);
+  }
   return hipGetLastError();
 }
 

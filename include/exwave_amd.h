@@ -129,6 +129,13 @@ int exw_op_apply_fused(exw_op op, const double *src, const exw_epilogue *epi, in
  * with k = M^-1 A(src).  y_out and b_out must not alias src. */
 int exw_rk_stage(exw_op op, const double *src, const double *b, double *y_out, double *b_out,
                  double f1, double f2, int is_last, int precision);
+/* The unfused vector update RKVectorUpdater::apply_to_subrange
+ * (source/time_integrators.cc:128-155) on n DoFs, for callers that compute
+ * vector1 = M^-1 A(src) themselves:
+ *   not last: vector2 += factor1*vector1; vector3 = vector2 + factor2*vector1
+ *   last    : vector2 = vector3 + factor1*vector1 */
+int exw_rk_update(exw_ctx ctx, int64_t n, double factor1, double factor2, int is_last,
+                  const double *vector1, double *vector2, double *vector3);
 /* ADER-CK time integral (integrate_taylor_cauchykovalewski,
  * source/wave_equation_operations.cc:961-1262): tempsrc = int_{t1}^{t2} u dt
  * by the Taylor series about te; recon (optional, may be NULL) replaces the
