@@ -101,8 +101,9 @@ class WaveOperator:
         check(load().exw_op_apply(self.h, _ptr(src), _ptr(dst), precision))
 
     def apply_fused(self, src, b=None, o1=None, a1=1.0, c=None, o2=None, a2=0.0,
-                    o2_from_o1=False, precision=FP64):
-        e = Epilogue(_ptr(b), _ptr(o1), a1, _ptr(c), _ptr(o2), a2, int(o2_from_o1))
+                    o2_from_o1=False, cell_scale=None, precision=FP64):
+        e = Epilogue(_ptr(b), _ptr(o1), a1, _ptr(c), _ptr(o2), a2, int(o2_from_o1),
+                     _ptr(cell_scale))
         check(load().exw_op_apply_fused(self.h, _ptr(src), C.byref(e), precision))
 
     def rk_stage(self, src, b, y_out, b_out, f1, f2, is_last, precision=FP64):

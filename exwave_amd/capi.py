@@ -50,7 +50,19 @@ class Modes(C.Structure):
 
 class Epilogue(C.Structure):
     _fields_ = [("b", C.c_void_p), ("o1", C.c_void_p), ("a1", C.c_double), ("c", C.c_void_p),
-                ("o2", C.c_void_p), ("a2", C.c_double), ("o2_from_o1", C.c_int)]
+                ("o2", C.c_void_p), ("a2", C.c_double), ("o2_from_o1", C.c_int),
+                ("cell_scale", C.c_void_p)]
+
+
+class Request(C.Structure):
+    _fields_ = [("dim", C.c_int), ("degree", C.c_int), ("n_refinements", C.c_int),
+                ("rho", C.c_double), ("c", C.c_double), ("modes", Modes),
+                ("final_time", C.c_double), ("cfl_number", C.c_double), ("precision", C.c_int)]
+
+
+class Result(C.Structure):
+    _fields_ = [("id", C.c_int64), ("n_steps", C.c_int), ("time", C.c_double),
+                ("err_p", C.c_double), ("err_v", C.c_double), ("mag_p", C.c_double)]
 
 
 class Problem(C.Structure):
@@ -95,6 +107,13 @@ SIGNATURES = [
     ("exw_energy", _I, [_VP, _VP, _DP]),
     ("exw_mesh_hypercube", _I, [_I, _I, _I, _I, _D, _I, _IP, _DP, _IP]),
     ("exw_run", _I, [_VP, C.POINTER(Problem), C.c_char_p, _I, _DP]),
+    ("exw_server_create", _I, [_VP, C.c_size_t, C.POINTER(_VP)]),
+    ("exw_server_destroy", _I, [_VP]),
+    ("exw_server_submit", _I, [_VP, C.POINTER(Request), C.POINTER(C.c_int64)]),
+    ("exw_server_progress", _I, [_VP, _I]),
+    ("exw_server_poll", _I, [_VP, C.POINTER(Result), _I]),
+    ("exw_server_status", _I, [_VP, _IP, _IP, _IP, C.POINTER(C.c_size_t)]),
+    ("exw_solve", _I, [_VP, C.POINTER(Request), C.POINTER(Result)]),
 ]
 
 _lib = None

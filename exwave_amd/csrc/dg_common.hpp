@@ -48,6 +48,9 @@ struct DevMesh {
   const double *__restrict__ vertices;   // general: [n_cells][2^dim][dim]
 };
 
+// o1 = b + a1*s*k, o2 = (o1 or c) + a2*s*k with s = cell_scale[cell] (1 if
+// cell_scale is NULL); cells with s == 0 are inactive and skipped entirely
+// (the multi-instance pools of the serving loop).
 struct Epilogue {
   const double *b;
   double *o1;
@@ -56,6 +59,7 @@ struct Epilogue {
   double *o2;
   double a2;
   int o2_from_o1;
+  const double *cell_scale;
 };
 
 // MI355X dispatches workgroup b to XCD b % 8.  Give every XCD one contiguous

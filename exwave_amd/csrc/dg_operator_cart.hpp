@@ -69,8 +69,14 @@ __global__ void __launch_bounds__((CartShape<DIM, K>::BLOCK))
   const int lane = tid - lc * LANES;
   const int cell = xcd_remap(blockIdx.x, gridDim.x) * CPB + lc;
   const bool lane_ok = lc < CPB;
-  const bool active = lane_ok && cell < mesh.n_cells;
+  bool active = lane_ok && cell < mesh.n_cells;
+  double cscale = 1.0;
+  if (active && epi.cell_scale) {
+    cscale = epi.cell_scale[cell];
+    active = cscale != 0.0;
+  }
   const bool node_ok = active && lane < ND;
+  const double ea1 = epi.a1 * cscale, ea2 = epi.a2 * cscale;
   const size_t cbase = (size_t)cell * NF * ND;
 
   // ---------------- A: own nodes, epilogue operand, neighbours ----------------
@@ -211,10 +217,10 @@ __global__ void __launch_bounds__((CartShape<DIM, K>::BLOCK))
     for (int f = 0; f < NF; ++f) {
       const size_t idx = cbase + f * ND + lane;
       const double kk = double(k[f]);
-      const double o1 = bv[f] + epi.a1 * kk;
+      const double o1 = bv[f] + ea1 * kk;
       __builtin_nontemporal_store(o1, epi.o1 + idx);
       if (epi.o2)
-        __builtin_nontemporal_store((epi.o2_from_o1 ? o1 : epi.c[idx]) + epi.a2 * kk, epi.o2 + idx);
+        __builtin_nontemporal_store((epi.o2_from_o1 ? o1 : epi.c[idx]) + ea2 * kk, epi.o2 + idx);
     }
   }
 }

@@ -16,6 +16,7 @@
 #include "../../include/exwave_amd.h"
 #include "basis.hpp"
 #include "launchers.hpp"
+#include "lsrk.hpp"
 
 using namespace exw;
 
@@ -36,6 +37,11 @@ int fail(int code, const std::string &msg) {
 
 const DimKernels &kernels(int dim) { return dim == 2 ? kernels_2d() : kernels_3d(); }
 }  // namespace
+
+namespace exw {
+// error message of the calling thread, shared by every translation unit
+void set_last_error(const std::string &msg) { g_err = msg; }
+}  // namespace exw
 
 struct exw_ctx_s {
   int device;
@@ -217,66 +223,9 @@ int exw_op_create(exw_ctx ctx, int dim, int degree, int n_cells, const double *v
     free_op(op);
     return rc;
   }
-  // operator basis table (layout: TabOff in dg_common.hpp)
-  std::vector<double> x, w;
-  gauss_legendre01(N, x, w);
-  std::vector<double> tab(2 * N * N + 6 * N);
-  double *D = tab.data(), *DT = D + N * N, *L = DT + N * N, *LW = L + 2 * N, *X = LW + 2 * N,
-         *W = X + N;
-  for (int q = 0; q < N; ++q) lagrange_barycentric(x, x[q], nullptr, D + q * N);
-  for (int i = 0; i < N; ++i)
-    for (int m = 0; m < N; ++m) DT[i * N + m] = D[m * N + i] * w[m] / w[i];
-  lagrange_barycentric(x, 0.0, L, nullptr);
-  lagrange_barycentric(x, 1.0, L + N, nullptr);
-  for (int s = 0; s < 2; ++s)
-    for (int i = 0; i < N; ++i) LW[s * N + i] = L[s * N + i] / w[i];
-  for (int i = 0; i < N; ++i) {
-    X[i] = x[i];
-    W[i] = w[i];
-  }
-  // error rule QGauss(k+2)
-  std::vector<double> xe, we;
-  gauss_legendre01(N + 1, xe, we);
-  std::vector<double> etab(2 * (N + 1) + (N + 1) * N);
-  for (int i = 0; i <= N; ++i) {
-    etab[i] = xe[i];
-    etab[N + 1 + i] = we[i];
-    lagrange_barycentric(x, xe[i], etab.data() + 2 * (N + 1) + i * N, nullptr);
-  }
-  // CK level tables
-  std::vector<double> ckt;
+  std::vector<double> tab, etab, ckt;
   CkTab ck{};
-  const int nsteps = degree >= 2 ? degree - 1 : 1;
-  ck.n_levels = (nsteps - 1) / 2 + 1;
-  std::vector<double> xprev, wprev;
-  for (int l = 0; l < ck.n_levels; ++l) {
-    const int n = N - 2 * l;
-    std::vector<double> xl, wl;
-    gauss_legendre01(n, xl, wl);
-    ck.off_D[l] = (int)ckt.size();
-    ckt.resize(ckt.size() + n * n);
-    for (int q = 0; q < n; ++q) lagrange_barycentric(xl, xl[q], nullptr, &ckt[ck.off_D[l] + q * n]);
-    ck.off_X[l] = (int)ckt.size();
-    for (int i = 0; i < n; ++i) ckt.push_back(xl[i]);
-    if (l > 0) {
-      const int nh = n + 2;
-      std::vector<double> vl(n);
-      ck.off_P[l] = (int)ckt.size();
-      ckt.resize(ckt.size() + n * nh);
-      ck.off_V[l] = (int)ckt.size();
-      ckt.resize(ckt.size() + nh * n);
-      for (int q = 0; q < nh; ++q) {
-        lagrange_barycentric(xl, xprev[q], vl.data(), nullptr);
-        for (int i = 0; i < n; ++i) {
-          ckt[ck.off_P[l] + i * nh + q] = vl[i] * wprev[q] / wl[i];
-          ckt[ck.off_V[l] + q * n + i] = vl[i];
-        }
-      }
-    }
-    xprev = xl;
-    wprev = wl;
-  }
-  ck.size = (int)ckt.size();
+  build_tables(degree, tab, etab, ckt, ck);
   op->ck = ck;
   op->host_tab = tab;
   if ((rc = upload(op, &op->d_tab, tab.data(), tab.size())) ||
@@ -318,7 +267,7 @@ int exw_op_apply_fused(exw_op op, const double *src, const exw_epilogue *e, int 
     return fail(EXW_EINVAL, "exw_op_apply_fused: an output aliases src (neighbour reads race)");
   if (e->o2 && !e->o2_from_o1 && !e->c)
     return fail(EXW_EINVAL, "exw_op_apply_fused: o2 needs c or o2_from_o1");
-  Epilogue ep{e->b, e->o1, e->a1, e->c, e->o2, e->a2, e->o2_from_o1};
+  Epilogue ep{e->b, e->o1, e->a1, e->c, e->o2, e->a2, e->o2_from_o1, e->cell_scale};
   OpLaunch l = kernels(op->dim).op[op->degree][precision][op->cart ? 1 : 0];
   if (!l) return fail(EXW_EUNSUP, "operator kernel not compiled");
   EXW_HIP(l(op->view(), src, op->d_tab, op->host_tab.data(), ep, op->ctx->stream));
@@ -326,7 +275,7 @@ int exw_op_apply_fused(exw_op op, const double *src, const exw_epilogue *e, int 
 }
 
 int exw_op_apply(exw_op op, const double *src, double *dst, int precision) {
-  exw_epilogue e{nullptr, dst, 1.0, nullptr, nullptr, 0.0, 0};
+  exw_epilogue e{nullptr, dst, 1.0, nullptr, nullptr, 0.0, 0, nullptr};
   return exw_op_apply_fused(op, src, &e, precision);
 }
 
@@ -334,7 +283,7 @@ int exw_rk_stage(exw_op op, const double *src, const double *b, double *y_out, d
                  double f1, double f2, int is_last, int precision) {
   if (!b) return fail(EXW_EINVAL, "exw_rk_stage: b is NULL");
   if (!is_last && !b_out) return fail(EXW_EINVAL, "exw_rk_stage: b_out is NULL");
-  exw_epilogue e{b, y_out, f1, nullptr, is_last ? nullptr : b_out, f2, 1};
+  exw_epilogue e{b, y_out, f1, nullptr, is_last ? nullptr : b_out, f2, 1, nullptr};
   return exw_op_apply_fused(op, src, &e, precision);
 }
 
@@ -395,7 +344,7 @@ int exw_ader_step(exw_op op, const double *vec_n, double *vec_np, double *work, 
     return rc;
   // vec_np = vec_n - (-M^-1 A tempsrc)  (source/time_integrators.cc:44-47 with the sign
   // flipped second sweep of source/wave_equation_operations.cc:1301-1342)
-  exw_epilogue e{vec_n, vec_np, 1.0, nullptr, nullptr, 0.0, 0};
+  exw_epilogue e{vec_n, vec_np, 1.0, nullptr, nullptr, 0.0, 0, nullptr};
   return exw_op_apply_fused(op, tempsrc, &e, precision);
 }
 
@@ -411,71 +360,18 @@ int exw_time_step_work_vectors(int integrator) {
   }
 }
 
-// Low-storage RK with the RKVectorUpdater fused into each operator sweep.
-// Registers: R_n (vec_n), R_np (vec_np), R_t (work).  Stage 1 reads the
-// state from R_n and writes the next stage input to X1 and the accumulator to
-// X2 ({X1,X2} = {R_t,R_np}); every later non-last stage reads (input, acc)
-// and writes the new input over acc (pointwise, in place) and the new acc into
-// the free register.  The last stage writes acc + f1*k into whichever of
-// {acc, free} is R_np.  X1/X2 are chosen so that the last stage's input is
-// not R_np.  Same arithmetic as source/time_integrators.cc:128-155.
+// Low-storage RK with the RKVectorUpdater fused into each operator sweep
+// (register schedule in lsrk.hpp).
 static int lsrk_fused(exw_op op, int S, const double *a, const double *b, double dt,
                       double *vec_n, double *vec_np, double *work, int precision) {
-  // symbolic rotation to find the register read by the last stage
-  enum { RN = 0, X1 = 1, X2 = 2 };
-  int src = RN, acc = RN, fre = -1;
-  for (int s = 0; s < S - 1; ++s) {
-    if (s == 0) {
-      src = X1;
-      acc = X2;
-      fre = RN;
-    } else {
-      const int ns = acc, na = fre, nf = src;
-      src = ns;
-      acc = na;
-      fre = nf;
-    }
-  }
-  double *reg[3];
-  reg[RN] = vec_n;
-  if (S == 1 || src == X1) {
-    reg[X1] = work;
-    reg[X2] = vec_np;
-  } else if (src == X2) {
-    reg[X1] = vec_np;
-    reg[X2] = work;
-  } else {  // last input is R_n: any assignment works
-    reg[X1] = work;
-    reg[X2] = vec_np;
-  }
-  int cs = RN, ca = RN, cf = -1;
-  for (int s = 0; s < S; ++s) {
-    const bool last = s == S - 1;
-    int rc;
-    if (!last) {
-      const int y_out = s == 0 ? X1 : ca, b_out = s == 0 ? X2 : cf;
-      rc = exw_rk_stage(op, reg[cs], reg[ca], reg[y_out], reg[b_out], a[s] * dt,
-                        (b[s] - a[s]) * dt, 0, precision);
-      if (rc) return rc;
-      if (s == 0) {
-        cs = X1;
-        ca = X2;
-        cf = RN;
-      } else {
-        const int ns = ca, na = cf, nf = cs;
-        cs = ns;
-        ca = na;
-        cf = nf;
-      }
-    } else {
-      double *out = (reg[ca] == vec_np) ? reg[ca] : reg[cf];
-      if (S == 1) out = vec_np;
-      if (out != vec_np) return fail(EXW_EINVAL, "internal: LSRK register plan failed");
-      rc = exw_rk_stage(op, reg[cs], reg[ca], out, nullptr, b[s] * dt, 0.0, 1, precision);
-      if (rc) return rc;
-    }
-  }
-  return EXW_OK;
+  const int rc = lsrk_fused_schedule(
+      S, a, b, dt, vec_n, vec_np, work,
+      [&](const double *src, const double *acc, double *y_out, double *b_out, double f1,
+          double f2, bool last) {
+        return exw_rk_stage(op, src, acc, y_out, b_out, f1, f2, last ? 1 : 0, precision);
+      });
+  if (rc == -1) return fail(EXW_EINVAL, "internal: LSRK register plan failed");
+  return rc;
 }
 
 int exw_time_step(exw_op op, int integrator, double dt, double *vec_n, double *vec_np,
@@ -486,7 +382,7 @@ int exw_time_step(exw_op op, int integrator, double dt, double *vec_n, double *v
   switch (integrator) {
     case EXW_EXPL_EULER: {
       // source/time_integrators.cc:24-34: vec_np = vec_n + dt*M^-1 A vec_n
-      exw_epilogue e{vec_n, vec_np, dt, nullptr, nullptr, 0.0, 0};
+      exw_epilogue e{vec_n, vec_np, dt, nullptr, nullptr, 0.0, 0, nullptr};
       return exw_op_apply_fused(op, vec_n, &e, precision);
     }
     case EXW_CLRK4: {
@@ -499,7 +395,7 @@ int exw_time_step(exw_op op, int integrator, double dt, double *vec_n, double *v
       double *nxt = ta;
       for (int s = 0; s < 4; ++s) {
         exw_epilogue e{s == 0 ? vec_n : vec_np, vec_np, wgt[s], vec_n, s < 3 ? nxt : nullptr,
-                       s < 3 ? cf[s] : 0.0, 0};
+                       s < 3 ? cf[s] : 0.0, 0, nullptr};
         if (int rc = exw_op_apply_fused(op, src, &e, precision)) return rc;
         src = nxt;
         nxt = nxt == ta ? tb : ta;

@@ -61,9 +61,11 @@ typedef struct exw_modes {
   double rho, c;
 } exw_modes;
 
-/* Fused epilogue applied to k = M^-1 A(src) at every DoF i:
- *   o1[i] = (b ? b[i] : 0) + a1*k[i]
- *   o2[i] = (o2_from_o1 ? o1[i] : c[i]) + a2*k[i]        (if o2 != NULL)
+/* Fused epilogue applied to k = M^-1 A(src) at every DoF i of cell K:
+ *   o1[i] = (b ? b[i] : 0) + a1*s_K*k[i]
+ *   o2[i] = (o2_from_o1 ? o1[i] : c[i]) + a2*s_K*k[i]        (if o2 != NULL)
+ * with s_K = cell_scale[K] (1 if cell_scale is NULL); cells with s_K == 0
+ * are skipped (no reads, no writes).
  * This covers apply (b=NULL, a1=1), RKVectorUpdater (b=v2, o1=v2, o2=v3,
  * o2_from_o1=1), its last stage, explicit Euler, ClassRK4 stages and the
  * ADER update.  o1/o2 may alias b/c but never src. */
@@ -75,6 +77,7 @@ typedef struct exw_epilogue {
   double *o2;
   double a2;
   int o2_from_o1;
+  const double *cell_scale; /* optional, device, one double per cell */
 } exw_epilogue;
 
 const char *exw_last_error(void);
@@ -206,6 +209,54 @@ typedef struct exw_problem {
  * code.  wall_seconds (optional) receives the time-loop wall time excluding
  * the error evaluations. */
 int exw_run(exw_ctx ctx, const exw_problem *p, char *log, int log_len, double *wall_seconds);
+
+/* ---- serving: many independent solves stepped together -------------------
+ * No counterpart in the reference (a build-only subsystem).  A request is one
+ * membrane solve on the unit hypercube with 2^n_refinements Cartesian cells
+ * per direction, boundary id 2 (p = 0), homogeneous rho and c, initial data and
+ * exact solution `modes` (its rho and c must equal the request's), LSRK45 at
+ * dt = cfl_number/k^1.5 * h/c stepped while t < final_time (the reference's
+ * time loop, source/explicit_wave.cc:779-799). */
+typedef struct exw_request {
+  int dim, degree, n_refinements;
+  double rho, c;
+  exw_modes modes;
+  double final_time, cfl_number;
+  int precision; /* EXW_FP64 / EXW_FP32 operator */
+} exw_request;
+
+typedef struct exw_result {
+  int64_t id;
+  int n_steps;
+  double time;                 /* final time reached (>= final_time) */
+  double err_p, err_v, mag_p;  /* L2 errors against the exact modes, QGauss(k+2) */
+} exw_result;
+
+typedef struct exw_server_s *exw_server;
+
+/* The serving loop keeps every admitted solve resident in HBM.  Solves of the
+ * same (dim, degree, precision) share a device pool of cells and are stepped
+ * together: one operator launch per RK stage covers every cell of every solve
+ * in the pool (per-cell time steps in the fused epilogue), the 5 launches of a
+ * step are captured once per pool in a hipGraph and replayed, each pool runs on
+ * its own stream.  Requests are admitted in arrival order while the pools'
+ * footprint fits in hbm_budget_bytes; finished solves leave their cells to the
+ * next admissions.  The loop itself runs on the calling host thread. */
+int exw_server_create(exw_ctx ctx, size_t hbm_budget_bytes, exw_server *out);
+int exw_server_destroy(exw_server s);
+/* queue a request; *id receives its id */
+int exw_server_submit(exw_server s, const exw_request *r, int64_t *id);
+/* admit what fits and enqueue up to max_steps batch steps (one time step of
+ * every active solve each); returns the number of active solves */
+int exw_server_progress(exw_server s, int max_steps);
+/* collect finished solves (up to max_results); returns how many were written */
+int exw_server_poll(exw_server s, exw_result *out, int max_results);
+/* queued (not admitted), active and finishing solves; bytes of HBM in use */
+int exw_server_status(exw_server s, int *n_queued, int *n_active, int *n_finishing,
+                      size_t *hbm_bytes);
+/* The same solve alone, one launch sequence per step (the one-at-a-time path
+ * the serving loop is compared with).  Synchronises. */
+int exw_solve(exw_ctx ctx, const exw_request *r, exw_result *out);
 
 #ifdef __cplusplus
 }

@@ -251,7 +251,7 @@ class WaveEquationOperationADER : public WaveEquationOperation<dim, fe_degree> {
     if (recon_) check(exw_op_apply(this->op_, src.data(), rec, this->precision_));
     check(exw_ader_ck(this->op_, src.data(), recon_ ? rec : nullptr,
                       this->time_control.get_time_step(), 0.0, 0.0, tempsrc, this->precision_));
-    exw_epilogue e{nullptr, dst.data(), -1.0, nullptr, nullptr, 0.0, 0};
+    exw_epilogue e{nullptr, dst.data(), -1.0, nullptr, nullptr, 0.0, 0, nullptr};
     check(exw_op_apply_fused(this->op_, tempsrc, &e, this->precision_));
   }
   bool use_ader_post() const { return recon_; }

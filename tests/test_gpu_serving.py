@@ -1,0 +1,117 @@
+"""GPU tests of the serving loop (exw_server_*) and the one-at-a-time path
+(exw_solve):
+  * exw_solve (FP64) against the oracle running the same request,
+  * every request gives the same result alone and in the mix (bit-identical
+    for the same precision), and the same as exw_solve (graph replays with
+    admissions in between against eager launches),
+  * admission under a budget that holds one solve at a time,
+  * the FP32-operator result stays within 1.1x of the FP64 error.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+REQS = [
+    dict(id=0, arrival=0.0, dim=2, degree=3, n_refinements=3, rho=1.3, c=0.7, modes=[2, 1],
+         final_time=0.25, cfl=0.2),
+    dict(id=1, arrival=0.0, dim=2, degree=2, n_refinements=4, rho=0.6, c=1.9, modes=[1, 3],
+         final_time=0.3, cfl=0.2),
+    dict(id=2, arrival=0.0, dim=3, degree=4, n_refinements=2, rho=1.0, c=1.0, modes=[1, 1, 2],
+         final_time=0.2, cfl=0.2),
+    dict(id=3, arrival=0.0, dim=2, degree=3, n_refinements=3, rho=2.0, c=0.5, modes=[4, 4],
+         final_time=0.15, cfl=0.2),
+    dict(id=4, arrival=0.0, dim=3, degree=2, n_refinements=3, rho=0.8, c=1.4, modes=[2, 1, 1],
+         final_time=0.1, cfl=0.2),
+    dict(id=5, arrival=0.0, dim=2, degree=6, n_refinements=3, rho=1.1, c=0.9, modes=[3, 2],
+         final_time=0.2, cfl=0.2),
+]
+
+
+@pytest.fixture(scope="module")
+def S():
+    import torch
+    assert torch.cuda.is_available()
+    import exwave_amd
+    from exwave_amd import serving
+    exwave_amd.load()
+    return exwave_amd, serving
+
+
+@pytest.fixture(scope="module")
+def ctx(S):
+    return S[0].Context(0)
+
+
+def _oracle_solve(q):
+    dim, k = q["dim"], q["degree"]
+    m = O.Mesh(dim, 1 << q["n_refinements"], 0, O.TRANSFORM_NONE, 0.0, 2)
+    m.set_material(np.full(m.n_cells, q["rho"]), np.full(m.n_cells, q["c"]))
+    e = O.Exact.make(dim, [q["modes"]], rho=q["rho"], c=q["c"])
+    u = m.project(k, e, 0.0)
+    h = 1.0 / (1 << q["n_refinements"])
+    dt = q["cfl"] / k ** 1.5 * h / q["c"]
+    t, n = 0.0, 0
+    while t < q["final_time"]:
+        t += dt
+        n += 1
+    for _ in range(n):
+        vn, vnp = u, np.zeros_like(u)
+        m.time_step(k, O.LSRK45R2, dt, vn, vnp)
+        u = vnp
+    err = m.l2_errors(k, e, t, u)
+    return n, t, err
+
+
+@pytest.mark.parametrize("i", [0, 2, 4])
+def test_solve_matches_oracle(S, ctx, i):
+    X, srv = S
+    q = REQS[i]
+    res, _ = srv.run_one_at_a_time(ctx, [q], X.FP64)
+    n, t, err = _oracle_solve(q)
+    r = res[q["id"]]
+    assert r["n_steps"] == n and r["time"] == t
+    got = np.array([r["err_p"], r["err_v"], r["mag_p"]])
+    assert np.max(np.abs(got - err) / err) < 1e-9, (got, err)
+
+
+def _key(r):
+    return (r["n_steps"], r["time"], r["err_p"], r["err_v"], r["mag_p"])
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_alone_equals_mix(S, ctx, prec):
+    X, srv = S
+    p = X.FP64 if prec == "fp64" else X.FP32
+    solo, _ = srv.run_one_at_a_time(ctx, REQS, p)
+    alone = {}
+    for q in REQS:
+        r, _ = srv.run_server(ctx, [q], lambda _q: p, realtime=False)
+        alone.update(r)
+    mix, _ = srv.run_server(ctx, REQS, lambda _q: p, realtime=False)
+    for q in REQS:
+        i = q["id"]
+        assert _key(alone[i]) == _key(mix[i]), (i, alone[i], mix[i])
+        assert _key(solo[i]) == _key(mix[i]), (i, solo[i], mix[i])
+
+
+def test_budget_admission(S, ctx):
+    """A budget that holds about one solve at a time still serves every request,
+    with the same results."""
+    X, srv = S
+    free, _ = srv.run_server(ctx, REQS, lambda _q: X.FP64, realtime=False)
+    tight, _ = srv.run_server(ctx, REQS, lambda _q: X.FP64, hbm_budget_bytes=3 << 20,
+                              realtime=False)
+    for q in REQS:
+        assert _key(free[q["id"]]) == _key(tight[q["id"]])
+
+
+def test_fp32_error_bar(S, ctx):
+    X, srv = S
+    f64, _ = srv.run_server(ctx, REQS, lambda _q: X.FP64, realtime=False)
+    f32, _ = srv.run_server(ctx, REQS, lambda _q: X.FP32, realtime=False)
+    for q in REQS:
+        a, b = f64[q["id"]], f32[q["id"]]
+        assert b["err_p"] <= 1.1 * a["err_p"] and b["err_v"] <= 1.1 * a["err_v"], (q, a, b)
