@@ -15,7 +15,7 @@ from __future__ import annotations
 
 import ctypes as C
 
-from .capi import (ADER, CLRK4, EXPL_EULER, FP32, FP64, INTEGRATORS, LSRK33R2, LSRK45R2,  # noqa
+from .capi import (ADER, ADER_LTS, CLRK4, EXPL_EULER, FP32, FP64, INTEGRATORS, LSRK33R2, LSRK45R2,  # noqa
                    LSRK59R2, TRANSFORM_AXIS_2PI, TRANSFORM_NONE, TRANSFORM_PRODUCT_2PI,
                    TRANSFORM_REFERENCE_SINE, Epilogue, ExwError, Modes, Problem, check, load,
                    mesh_hypercube)
@@ -135,6 +135,55 @@ class WaveOperator:
         out = C.c_double(0.0)
         check(load().exw_energy(self.h, _ptr(u), C.byref(out)))
         return out.value
+
+
+def lts_categorize(dim, vertices, neighbor, cfl, max_clusters, max_diff):
+    """exw_lts_categorize (host only): (cluster ids, flags, n_clusters,
+    cluster_diff, iterations); flags 1 = slower neighbour, 2 = faster."""
+    import numpy as np
+    V = np.ascontiguousarray(vertices, dtype=np.float64)
+    nb = np.ascontiguousarray(neighbor, dtype=np.int32)
+    n = V.shape[0]
+    cl = np.zeros(n, np.int32)
+    fl = np.zeros(n, np.int32)
+    nc, diff, it = C.c_int(), C.c_int(), C.c_int()
+    ip, dp = C.POINTER(C.c_int), C.POINTER(C.c_double)
+    check(load().exw_lts_categorize(dim, n, V.ctypes.data_as(dp), nb.ctypes.data_as(ip), cfl,
+                                    max_clusters, max_diff, cl.ctypes.data_as(ip),
+                                    fl.ctypes.data_as(ip), C.byref(nc), C.byref(diff),
+                                    C.byref(it)))
+    return cl, fl, nc.value, diff.value, it.value
+
+
+class LocalTimeStepping:
+    """ADER with local time stepping on a WaveOperator (exw_lts_*)."""
+
+    def __init__(self, op: WaveOperator, cfl, fastest_dt, max_clusters, max_diff, time0=0.0,
+                 recon=True):
+        h = C.c_void_p()
+        check(load().exw_lts_create(op.h, cfl, fastest_dt, max_clusters, max_diff, time0,
+                                    int(recon), C.byref(h)))
+        self.h, self.op = h, op
+        nc, diff, nu, gdt = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+        check(load().exw_lts_info(h, C.byref(nc), C.byref(diff), C.byref(nu), C.byref(gdt)))
+        self.n_clusters, self.cluster_diff = nc.value, diff.value
+        self.n_updates, self.global_dt = nu.value, gdt.value
+
+    def cells(self):
+        import numpy as np
+        cl = np.zeros(self.op.n_cells, np.int32)
+        fl = np.zeros(self.op.n_cells, np.int32)
+        ip = C.POINTER(C.c_int)
+        check(load().exw_lts_cells(self.h, cl.ctypes.data_as(ip), fl.ctypes.data_as(ip)))
+        return cl, fl
+
+    def step(self, time_now, src, dst):
+        check(load().exw_lts_step(self.h, time_now, _ptr(src), _ptr(dst)))
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None:
+            load().exw_lts_destroy(self.h)
+            self.h = None
 
 
 def run(ctx: Context, problem: Problem):

@@ -15,9 +15,9 @@ LIB_PATH = os.path.join(HERE, "_lib", "libexwave_amd.so")
 
 EXW_OK, EXW_EINVAL, EXW_EDEVICE, EXW_ENOMEM, EXW_EUNSUP = 0, -1, -2, -3, -4
 FP64, FP32 = 0, 1
-EXPL_EULER, CLRK4, LSRK45R2, LSRK33R2, LSRK59R2, ADER = range(6)
+EXPL_EULER, CLRK4, LSRK45R2, LSRK33R2, LSRK59R2, ADER, ADER_LTS = range(7)
 INTEGRATORS = {"ExplEuler": EXPL_EULER, "clRK4": CLRK4, "LSRK45R2": LSRK45R2,
-               "LSRK33R2": LSRK33R2, "LSRK59R2": LSRK59R2, "ADER": ADER}
+               "LSRK33R2": LSRK33R2, "LSRK59R2": LSRK59R2, "ADER": ADER, "ADERLTS": ADER_LTS}
 TRANSFORM_NONE, TRANSFORM_REFERENCE_SINE, TRANSFORM_AXIS_2PI, TRANSFORM_PRODUCT_2PI = range(4)
 
 
@@ -72,7 +72,8 @@ class Problem(C.Structure):
                 ("integrator", C.c_int), ("cfl_number", C.c_double),
                 ("max_time_steps", C.c_int), ("final_time", C.c_double),
                 ("output_every_time", C.c_double), ("ader_recon", C.c_int),
-                ("precision", C.c_int), ("modes", Modes)]
+                ("precision", C.c_int), ("modes", Modes),
+                ("lts_max_clusters", C.c_int), ("lts_max_diff", C.c_int)]
 
 
 # (name, restype, argtypes) of every exported entry point
@@ -114,6 +115,12 @@ SIGNATURES = [
     ("exw_server_poll", _I, [_VP, C.POINTER(Result), _I]),
     ("exw_server_status", _I, [_VP, _IP, _IP, _IP, C.POINTER(C.c_size_t)]),
     ("exw_solve", _I, [_VP, C.POINTER(Request), C.POINTER(Result)]),
+    ("exw_lts_categorize", _I, [_I, _I, _DP, _IP, _D, _I, _I, _IP, _IP, _IP, _IP, _IP]),
+    ("exw_lts_create", _I, [_VP, _D, _D, _I, _I, _D, _I, C.POINTER(_VP)]),
+    ("exw_lts_destroy", _I, [_VP]),
+    ("exw_lts_info", _I, [_VP, _IP, _IP, _IP, _DP]),
+    ("exw_lts_cells", _I, [_VP, _IP, _IP]),
+    ("exw_lts_step", _I, [_VP, _D, _VP, _VP]),
 ]
 
 _lib = None

@@ -52,8 +52,9 @@ __device__ __forceinline__ constexpr int ipow_c() {
 template <int DIM, int K, typename Number, bool CART>
 __global__ void __launch_bounds__((Shape<DIM, K>::BLOCK))
     ader_ck_kernel(const DevMesh mesh, const double *__restrict__ src,
-                   const double *__restrict__ recon, const double *__restrict__ cktab, const CkTab ck, double t2, double t1,
-                   double te, double *__restrict__ tempsrc) {
+                   const double *__restrict__ recon, const double *__restrict__ cktab,
+                   const CkTab ck, double t2, double t1, double te_all, const CellSel sel,
+                   double *__restrict__ tempsrc) {
   using S = Shape<DIM, K>;
   using CS = CkShape<K>;
   constexpr int N = S::N, ND = S::ND, NF = S::NF, CPB = S::CPB;
@@ -77,15 +78,19 @@ __global__ void __launch_bounds__((Shape<DIM, K>::BLOCK))
   for (int i = tid; i < CS::TAB_SIZE && i < ck.size; i += blockDim.x) s_ck[i] = Number(cktab[i]);
   const int lc = tid / ND;
   const int q = tid - lc * ND;
-  const int cell = blockIdx.x * CPB + lc;
   const bool lane_ok = lc < CPB;
-  const bool active = lane_ok && cell < mesh.n_cells;
+  // entry -> cell (identity, or through the selection list)
+  auto cell_of = [&](int e) { return sel.list ? (e < sel.n ? sel.list[e] : -1) : e; };
+  const int cell = cell_of(blockIdx.x * CPB + lc);
+  const bool active = lane_ok && cell >= 0 && cell < mesh.n_cells;
+  const double te = active && sel.cluster ? sel.te[sel.cluster[cell]] : te_all;
 
   if constexpr (!CART) {
     constexpr int NVD = (1 << DIM) * DIM;
     for (int i = tid; i < CPB * NVD; i += blockDim.x) {
-      const int c2 = blockIdx.x * CPB + i / NVD;
-      s_V[i / NVD][i % NVD] = c2 < mesh.n_cells ? mesh.vertices[(size_t)c2 * NVD + i % NVD] : 0.0;
+      const int c2 = cell_of(blockIdx.x * CPB + i / NVD);
+      s_V[i / NVD][i % NVD] =
+          c2 >= 0 && c2 < mesh.n_cells ? mesh.vertices[(size_t)c2 * NVD + i % NVD] : 0.0;
     }
   }
   Number rho = Number(1), cc = Number(1), hh[DIM];

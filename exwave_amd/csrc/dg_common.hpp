@@ -62,6 +62,17 @@ struct Epilogue {
   const double *cell_scale;
 };
 
+// Optional cell selection of the local-time-stepping sweeps: the kernel runs
+// over cells list[0..n) instead of 0..n_cells-1, and the expansion time of a
+// cell is te[cluster[cell]] (its cluster's time level).
+constexpr int MAX_CLUSTERS = 16;
+struct CellSel {
+  const int *list;
+  int n;
+  const int *cluster;
+  double te[MAX_CLUSTERS];
+};
+
 // MI355X dispatches workgroup b to XCD b % 8.  Give every XCD one contiguous
 // range of logical blocks (a bijection for any grid size), so that cells that
 // are neighbours in the numbering share one L2.

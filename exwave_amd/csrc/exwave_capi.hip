@@ -17,6 +17,7 @@
 #include "basis.hpp"
 #include "launchers.hpp"
 #include "lsrk.hpp"
+#include "capi_internal.hpp"
 
 using namespace exw;
 
@@ -42,40 +43,6 @@ namespace exw {
 // error message of the calling thread, shared by every translation unit
 void set_last_error(const std::string &msg) { g_err = msg; }
 }  // namespace exw
-
-struct exw_ctx_s {
-  int device;
-  hipStream_t stream;
-};
-
-struct exw_op_s {
-  exw_ctx ctx;
-  int dim, degree, n_cells;
-  bool cart;
-  double minvd;
-  int *d_neighbor = nullptr;
-  double *d_rho = nullptr, *d_speed = nullptr, *d_h = nullptr, *d_vertices = nullptr;
-  double *d_tab = nullptr, *d_errtab = nullptr, *d_cktab = nullptr;
-  double *d_partial = nullptr, *d_sum = nullptr;
-  int n_partial = 0;
-  CkTab ck{};
-  std::vector<double> host_tab;
-  DevMesh view() const {
-    DevMesh m;
-    m.n_cells = n_cells;
-    m.neighbor = d_neighbor;
-    m.rho = d_rho;
-    m.speed = d_speed;
-    m.h = d_h;
-    m.vertices = d_vertices;
-    return m;
-  }
-  int64_t n_dofs() const {
-    int64_t nd = 1;
-    for (int d = 0; d < dim; ++d) nd *= degree + 1;
-    return (int64_t)n_cells * (dim + 1) * nd;
-  }
-};
 
 extern "C" {
 
@@ -228,6 +195,8 @@ int exw_op_create(exw_ctx ctx, int dim, int degree, int n_cells, const double *v
   build_tables(degree, tab, etab, ckt, ck);
   op->ck = ck;
   op->host_tab = tab;
+  op->h_vertices.assign(vertices, vertices + (size_t)n_cells * nv * dim);
+  op->h_neighbor.assign(neighbor, neighbor + (size_t)n_cells * nface);
   if ((rc = upload(op, &op->d_tab, tab.data(), tab.size())) ||
       (rc = upload(op, &op->d_errtab, etab.data(), etab.size())) ||
       (rc = upload(op, &op->d_cktab, ckt.data(), ckt.size()))) {
@@ -329,7 +298,8 @@ int exw_ader_ck(exw_op op, const double *src, const double *recon, double t2, do
     return fail(EXW_EINVAL, "exw_ader_ck: tempsrc aliases an input");
   CkLaunch l = kernels(op->dim).ck[op->degree][precision][op->cart ? 1 : 0];
   if (!l) return fail(EXW_EUNSUP, "CK kernel not compiled");
-  EXW_HIP(l(op->view(), src, recon, op->d_cktab, op->ck, t2, t1, te, tempsrc, op->ctx->stream));
+  EXW_HIP(l(op->view(), src, recon, op->d_cktab, op->ck, t2, t1, te, nullptr, tempsrc,
+             op->ctx->stream));
   return EXW_OK;
 }
 

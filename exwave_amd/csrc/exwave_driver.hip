@@ -105,7 +105,8 @@ int exw_run(exw_ctx ctx, const exw_problem *p, char *log, int log_len, double *w
                           speed.data(), &op)))
     return rc;
   const int64_t N = exw_op_n_dofs(op);
-  const int nwork = exw_time_step_work_vectors(p->integrator);
+  const bool lts = p->integrator == EXW_ADER_LTS;
+  const int nwork = lts ? 0 : exw_time_step_work_vectors(p->integrator);
   if (nwork < 0) {
     exw_op_destroy(op);
     return EXW_EINVAL;
@@ -128,6 +129,19 @@ int exw_run(exw_ctx ctx, const exw_problem *p, char *log, int log_len, double *w
   const unsigned max_step =
       p->max_time_steps == 0 ? (unsigned)std::round((T - time) / dt) : (unsigned)p->max_time_steps;
   dt = dt_cfl;
+  exw_lts L = nullptr;
+  if (lts) {
+    // WaveEquationOperationADERLTS::setup: clusters from the unrounded CFL step;
+    // the time control then advances by the slowest cluster's step
+    // (include/cluster_manager.templates.h:463)
+    if ((rc = exw_lts_create(op, cfl, dt_cfl, p->lts_max_clusters, p->lts_max_diff, 0.0,
+                             p->ader_recon, &L))) {
+      exw_free(ctx, buf);
+      exw_op_destroy(op);
+      return rc;
+    }
+    exw_lts_info(L, nullptr, nullptr, nullptr, &dt);
+  }
   int pos = 0;
   if (log && log_len > 0) log[0] = 0;
   auto output = [&]() -> int {
@@ -147,14 +161,18 @@ int exw_run(exw_ctx ctx, const exw_problem *p, char *log, int log_len, double *w
       std::swap(sol, tmp);
       exw_sync(ctx);
       auto t0 = std::chrono::steady_clock::now();
-      if ((rc = exw_time_step(op, p->integrator, dt, tmp, sol, work, p->ader_recon, p->precision)))
-        break;
+      if (lts)
+        rc = exw_lts_step(L, time, tmp, sol);  // ArbitraryHighOrderDGLTS (time_integrators.cc:54-63)
+      else
+        rc = exw_time_step(op, p->integrator, dt, tmp, sol, work, p->ader_recon, p->precision);
+      if (rc) break;
       if ((rc = exw_sync(ctx))) break;
       wall += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
       // TimeControl::at_tick (utilities.cc:89-95)
       if ((int)(time / tick) != (int)((time - dt) / tick) && (rc = output())) break;
     }
   }
+  exw_lts_destroy(L);
   exw_free(ctx, buf);
   exw_op_destroy(op);
   if (wall_seconds) *wall_seconds = wall;

@@ -2,6 +2,7 @@
 // dispatch tables of the MI355X kernels.
 #pragma once
 #include "dg_ader.hpp"
+#include "dg_lts.hpp"
 #include "dg_misc.hpp"
 #include "dg_operator.hpp"
 #include "dg_operator_cart.hpp"
@@ -13,8 +14,13 @@ constexpr int MAX_DEGREE = 8;
 // (mesh, src, device basis table, host basis table, epilogue, stream)
 using OpLaunch = hipError_t (*)(const DevMesh &, const double *, const double *, const double *,
                                 const Epilogue &, hipStream_t);
+// (mesh, src, recon, CK table, CK offsets, t2, t1, te, cell selection (NULL:
+// all cells, uniform te), out, stream)
 using CkLaunch = hipError_t (*)(const DevMesh &, const double *, const double *, const double *,
-                                const CkTab &, double, double, double, double *, hipStream_t);
+                                const CkTab &, double, double, double, const CellSel *, double *,
+                                hipStream_t);
+// (mesh, basis table, sweep arguments, stream)
+using LtsLaunch = hipError_t (*)(const DevMesh &, const double *, const LtsArgs &, hipStream_t);
 using ProjLaunch = hipError_t (*)(const DevMesh &, const double *, const DevModes &, double,
                                   double *, hipStream_t);
 using ErrLaunch = hipError_t (*)(const DevMesh &, const double *, const DevModes &, double,
@@ -25,6 +31,7 @@ using EnergyLaunch = hipError_t (*)(const DevMesh &, const double *, const doubl
 struct DimKernels {
   OpLaunch op[MAX_DEGREE + 1][2][2];  // [k][fp32][cart]
   CkLaunch ck[MAX_DEGREE + 1][2][2];
+  LtsLaunch lts[MAX_DEGREE + 1];
   ProjLaunch project[MAX_DEGREE + 1];
   ErrLaunch l2err[MAX_DEGREE + 1];
   EnergyLaunch energy[MAX_DEGREE + 1];
@@ -66,12 +73,24 @@ hipError_t launch_op(const DevMesh &m, const double *src, const double *tab,
 template <int DIM, int K, typename Number, bool CART>
 hipError_t launch_ck(const DevMesh &m, const double *src, const double *recon,
                      const double *cktab, const CkTab &ck, double t2, double t1, double te,
-                     double *out, hipStream_t s) {
+                     const CellSel *sel, double *out, hipStream_t s) {
   using S = Shape<DIM, K>;
-  const int grid = (m.n_cells + S::CPB - 1) / S::CPB;
+  CellSel cs{};
+  if (sel) cs = *sel;
+  const int n = cs.list ? cs.n : m.n_cells;
+  const int grid = (n + S::CPB - 1) / S::CPB;
   if (grid == 0) return hipSuccess;
   hipLaunchKernelGGL((ader_ck_kernel<DIM, K, Number, CART>), dim3(grid), dim3(S::BLOCK), 0, s,
-                     m, src, recon, cktab, ck, t2, t1, te, out);
+                     m, src, recon, cktab, ck, t2, t1, te, cs, out);
+  return hipGetLastError();
+}
+
+template <int DIM, int K>
+hipError_t launch_lts(const DevMesh &m, const double *tab, const LtsArgs &a, hipStream_t s) {
+  using S = Shape<DIM, K>;
+  const int grid = (a.n + S::CPB - 1) / S::CPB;
+  if (grid == 0) return hipSuccess;
+  hipLaunchKernelGGL((dg_lts_kernel<DIM, K>), dim3(grid), dim3(S::BLOCK), 0, s, m, tab, a);
   return hipGetLastError();
 }
 
@@ -112,6 +131,7 @@ void fill_dim_kernels(DimKernels &t) {
     t.ck[K][0][1] = launch_ck<DIM, K, double, true>;
     t.ck[K][1][0] = launch_ck<DIM, K, float, false>;
     t.ck[K][1][1] = launch_ck<DIM, K, float, true>;
+    t.lts[K] = launch_lts<DIM, K>;
     t.project[K] = launch_project<DIM, K>;
     t.l2err[K] = launch_l2err<DIM, K>;
     t.energy[K] = launch_energy<DIM, K>;

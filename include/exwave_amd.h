@@ -45,6 +45,7 @@ extern "C" {
 #define EXW_LSRK33R2 3
 #define EXW_LSRK59R2 4
 #define EXW_ADER 5
+#define EXW_ADER_LTS 6 /* exw_run / exw_lts_*: ADER with local time stepping */
 
 typedef struct exw_ctx_s *exw_ctx;
 typedef struct exw_op_s *exw_op;
@@ -169,6 +170,34 @@ int exw_l2_errors(exw_op op, const exw_modes *modes, double t, const double *u,
 /* E = 1/2 int rho|v|^2 + p^2/(rho c^2) with QGauss(k+1); synchronises. */
 int exw_energy(exw_op op, const double *u, double *out_host);
 
+/* ---- ADER with local time stepping --------------------------------------
+ * WaveEquationOperationADERLTS (source/wave_equation_operations.cc:1385-1692)
+ * with ClusterManager (include/cluster_manager.templates.h:24-1049): cells are
+ * grouped into clusters whose time steps are multiples 1, 1+d, 1+2d, ... of
+ * the fastest step; the clusters advance in the reference's update order and
+ * exchange face fluxes through a flux memory. */
+typedef struct exw_lts_s *exw_lts;
+/* iterate_cluster_categorization (include/cluster_manager.templates.h:24-262)
+ * on the host, no device needed: cluster[c] and flags[c] (1: has a slower
+ * face neighbour, 2: a faster one) per cell; cfl = cfl_number/k^1.5. */
+int exw_lts_categorize(int dim, int n_cells, const double *vertices, const int *neighbor,
+                       double cfl, int max_clusters, int max_diff, int *cluster, int *flags,
+                       int *n_clusters, int *cluster_diff, int *iterations);
+/* WaveEquationOperationADERLTS::setup + ClusterManager::setup (:330-484):
+ * fastest_dt = the unrounded CFL step, time0 = the time at set-up, recon =
+ * use_ader_post.  The operator must outlive the handle. */
+int exw_lts_create(exw_op op, double cfl, double fastest_dt, int max_clusters, int max_diff,
+                   double time0, int recon, exw_lts *out);
+int exw_lts_destroy(exw_lts L);
+/* global_dt = the slowest cluster's step, by which the time control advances */
+int exw_lts_info(exw_lts L, int *n_clusters, int *cluster_diff, int *n_updates,
+                 double *global_dt);
+int exw_lts_cells(exw_lts L, int *cluster, int *flags);
+/* ArbitraryHighOrderDGLTS::perform_time_step (source/time_integrators.cc:54-63):
+ * dst = state after one global step; time_now = the time after
+ * TimeControl::advance_time_step.  src and dst are distinct device vectors. */
+int exw_lts_step(exw_lts L, double time_now, const double *src, double *dst);
+
 /* ---- problem setup and the whole solve ---------------------------------- */
 /* transforms of the unit hypercube's vertices */
 #define EXW_TRANSFORM_NONE 0
@@ -198,6 +227,7 @@ typedef struct exw_problem {
   int ader_recon;      /* use_ader_post */
   int precision;       /* EXW_FP64 / EXW_FP32 operator */
   exw_modes modes;     /* initial condition and exact solution */
+  int lts_max_clusters, lts_max_diff; /* EXW_ADER_LTS: max_n_clusters, max_diff_clusters */
 } exw_problem;
 
 /* WaveEquationProblem<dim>::run (source/explicit_wave.cc:549-813): mesh,

@@ -16,9 +16,9 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _SO = os.path.join(_HERE, "_build", "libexwave_oracle.so")
 
-EXPL_EULER, CLRK4, LSRK45R2, LSRK33R2, LSRK59R2, ADER = range(6)
+EXPL_EULER, CLRK4, LSRK45R2, LSRK33R2, LSRK59R2, ADER, ADER_LTS = range(7)
 INTEGRATORS = {"ExplEuler": EXPL_EULER, "clRK4": CLRK4, "LSRK45R2": LSRK45R2,
-               "LSRK33R2": LSRK33R2, "LSRK59R2": LSRK59R2, "ADER": ADER}
+               "LSRK33R2": LSRK33R2, "LSRK59R2": LSRK59R2, "ADER": ADER, "ADERLTS": ADER_LTS}
 TRANSFORM_NONE, TRANSFORM_REFERENCE_SINE, TRANSFORM_AXIS_2PI, TRANSFORM_PRODUCT_2PI = range(4)
 
 
@@ -64,7 +64,7 @@ class Problem(C.Structure):
                 ("integrator", C.c_int), ("cfl_number", C.c_double),
                 ("max_time_steps", C.c_int), ("final_time", C.c_double),
                 ("output_every_time", C.c_double), ("ader_recon", C.c_int),
-                ("exact", Exact)]
+                ("exact", Exact), ("lts_max_clusters", C.c_int), ("lts_max_diff", C.c_int)]
 
 
 _lib = None
@@ -99,6 +99,20 @@ def lib():
         L.ox_n_dofs.argtypes = [C.POINTER(_Mesh), C.c_int]
         L.ox_gauss.argtypes = [C.c_int, dp, dp]
         L.ox_lagrange.argtypes = [C.c_int, dp, C.c_double, dp, dp]
+        ip = C.POINTER(C.c_int)
+        L.ox_lts_create.restype = C.c_void_p
+        L.ox_lts_create.argtypes = [C.POINTER(_Mesh), C.c_int, C.c_double, C.c_double, C.c_int,
+                                    C.c_int, C.c_double, C.c_int]
+        L.ox_lts_free.argtypes = [C.c_void_p]
+        for f in ("ox_lts_n_clusters", "ox_lts_cluster_diff", "ox_lts_iterations",
+                  "ox_lts_n_updates"):
+            getattr(L, f).restype = C.c_int
+            getattr(L, f).argtypes = [C.c_void_p]
+        L.ox_lts_global_dt.restype = C.c_double
+        L.ox_lts_global_dt.argtypes = [C.c_void_p]
+        L.ox_lts_schedule.argtypes = [C.c_void_p, ip, dp]
+        L.ox_lts_cells.argtypes = [C.c_void_p, ip, ip]
+        L.ox_lts_step.argtypes = [C.c_void_p, C.c_double, dp, dp]
     return _lib
 
 

@@ -326,27 +326,22 @@ static void face_flux(int dim, const double *nrm, const double *up, const double
 }
 
 typedef struct work {
-  double *r, *g, *tmp;
+  double *r, *g, *tmp, *jxw;
 } work;
 
-/* One cell of M^-1 A(src): evaluate_cell + the faces of this cell +
- * inverse mass (GAUSS_POINTS_VECTOR_OPERATION form,
- * source/wave_equation_operations.cc:640-657). */
-static void apply_cell(const ox_mesh *m, const ox_basis *b, int c, const double *src,
-                       double *dst, double sign, work *W) {
-  const int dim = m->dim, n = b->n, nf = dim + 1;
+/* evaluate_cell (source/wave_equation_operations.cc:424-456) of cell c with
+ * values u: the cell integral is added into r (unscaled by the inverse mass);
+ * jxw receives det(J) w_q at the Gauss points. */
+static void cell_volume(const ox_mesh *m, const ox_basis *b, int c, const double *u, double *r,
+                        double *jxw, work *W) {
+  const int dim = m->dim, n = b->n;
   const int nd = ipow(n, dim), nv = 1 << dim;
   const int sz[3] = {n, n, n};
   const double *V = m->vertices + (long)c * nv * dim;
-  const double *u = src + (long)c * nf * nd;
   const double rho = m->rho[c], cc = m->speed[c];
-  double *r = W->r, *g = W->g, *tmp = W->tmp;
-  memset(r, 0, sizeof(double) * nf * nd);
-
-  /* cell integral, evaluate_cell (:424-456) */
+  double *g = W->g, *tmp = W->tmp;
   const double *p = u + dim * nd;
   for (int e = 0; e < dim; ++e) tensor_apply(dim, sz, e, b->D, n, p, tmp + e * nd, 0, 0);
-  double *jxw = g + dim * nd;
   for (int q = 0; q < nd; ++q) {
     double xi[3], J[9], Ji[9];
     int t = q;
@@ -362,7 +357,7 @@ static void apply_cell(const ox_mesh *m, const ox_basis *b, int c, const double 
     for (int i = 0; i < dim; ++i) {
       double gp = 0.0;
       for (int e = 0; e < dim; ++e) gp += Ji[e * dim + i] * tmp[e * nd + q];
-      r[i * nd + q] = -gp / rho * dw;
+      r[i * nd + q] += -gp / rho * dw;
     }
     for (int e = 0; e < dim; ++e) {
       double s = 0.0;
@@ -371,54 +366,77 @@ static void apply_cell(const ox_mesh *m, const ox_basis *b, int c, const double 
     }
   }
   for (int e = 0; e < dim; ++e) tensor_apply(dim, sz, e, b->D, n, g + e * nd, r + dim * nd, 1, 1);
+}
 
-  /* faces of this cell (:480-551, :569-629), each evaluated from this side */
-  const int nt = ipow(n, dim - 1);
-  for (int f = 0; f < 2 * dim; ++f) {
-    const int d = f >> 1, s = f & 1;
-    const int nb = m->neighbor[c * 2 * dim + f];
-    const int stride_d = ipow(n, d);
-    const double *un = nb >= 0 ? src + (long)nb * nf * nd : NULL;
-    for (int t = 0; t < nt; ++t) {
-      int base = 0, tt = t;
-      double xi[3], wt = 1.0;
-      xi[d] = (double)s;
-      for (int e = 0; e < dim; ++e) {
-        if (e == d) continue;
-        int te = tt % n;
-        tt /= n;
-        base += te * ipow(n, e);
-        xi[e] = b->x[te];
-        wt *= b->w[te];
-      }
-      double J[9], Ji[9];
-      double det = multilinear_jacobian(dim, V, xi, J, Ji);
-      double nrm[3], na = 0.0;
-      for (int i = 0; i < dim; ++i) {
-        nrm[i] = Ji[d * dim + i];
-        na += nrm[i] * nrm[i];
-      }
-      na = sqrt(na);
-      for (int i = 0; i < dim; ++i) nrm[i] = (s ? 1.0 : -1.0) * nrm[i] / na;
-      double jxw_f = fabs(det) * na * wt;
-      double up[4], um[4] = {0, 0, 0, 0}, val[4];
-      for (int k = 0; k < nf; ++k) {
-        double a = 0.0, bb = 0.0;
-        for (int i = 0; i < n; ++i) {
-          a += b->face[s][i] * u[k * nd + base + i * stride_d];
-          if (un) bb += b->face[1 - s][i] * un[k * nd + base + i * stride_d];
-        }
-        up[k] = a;
-        um[k] = bb;
-      }
-      face_flux(dim, nrm, up, um, rho, cc, nb >= 0 ? m->rho[nb] : rho,
-                nb >= 0 ? m->speed[nb] : cc, nb >= 0 ? 0 : -nb, 1.0, val);
-      for (int k = 0; k < nf; ++k)
-        for (int i = 0; i < n; ++i)
-          r[k * nd + base + i * stride_d] += b->face[s][i] * val[k] * jxw_f;
+/* Face f of cell c evaluated from this side (evaluate_inner_face :480-551,
+ * evaluate_boundary_face :569-629): u = this cell's values, un = the
+ * neighbour's values (ignored at a boundary).  The test-function terms of this
+ * side, times boundary_fac `fac`, are added into r. */
+static void cell_face(const ox_mesh *m, const ox_basis *b, int c, int f, const double *u,
+                      const double *un, double fac, double *r) {
+  const int dim = m->dim, n = b->n, nf = dim + 1;
+  const int nd = ipow(n, dim), nv = 1 << dim, nt = ipow(n, dim - 1);
+  const double *V = m->vertices + (long)c * nv * dim;
+  const double rho = m->rho[c], cc = m->speed[c];
+  const int d = f >> 1, s = f & 1;
+  const int nb = m->neighbor[c * 2 * dim + f];
+  const int stride_d = ipow(n, d);
+  if (nb < 0) un = NULL;
+  for (int t = 0; t < nt; ++t) {
+    int base = 0, tt = t;
+    double xi[3], wt = 1.0;
+    xi[d] = (double)s;
+    for (int e = 0; e < dim; ++e) {
+      if (e == d) continue;
+      int te = tt % n;
+      tt /= n;
+      base += te * ipow(n, e);
+      xi[e] = b->x[te];
+      wt *= b->w[te];
     }
+    double J[9], Ji[9];
+    double det = multilinear_jacobian(dim, V, xi, J, Ji);
+    double nrm[3], na = 0.0;
+    for (int i = 0; i < dim; ++i) {
+      nrm[i] = Ji[d * dim + i];
+      na += nrm[i] * nrm[i];
+    }
+    na = sqrt(na);
+    for (int i = 0; i < dim; ++i) nrm[i] = (s ? 1.0 : -1.0) * nrm[i] / na;
+    double jxw_f = fabs(det) * na * wt;
+    double up[4], um[4] = {0, 0, 0, 0}, val[4];
+    for (int k = 0; k < nf; ++k) {
+      double a = 0.0, bb = 0.0;
+      for (int i = 0; i < n; ++i) {
+        a += b->face[s][i] * u[k * nd + base + i * stride_d];
+        if (un) bb += b->face[1 - s][i] * un[k * nd + base + i * stride_d];
+      }
+      up[k] = a;
+      um[k] = bb;
+    }
+    face_flux(dim, nrm, up, um, rho, cc, nb >= 0 ? m->rho[nb] : rho,
+              nb >= 0 ? m->speed[nb] : cc, nb >= 0 ? 0 : -nb, fac, val);
+    for (int k = 0; k < nf; ++k)
+      for (int i = 0; i < n; ++i)
+        r[k * nd + base + i * stride_d] += b->face[s][i] * val[k] * jxw_f;
   }
+}
 
+/* One cell of M^-1 A(src): evaluate_cell + the faces of this cell +
+ * inverse mass (GAUSS_POINTS_VECTOR_OPERATION form,
+ * source/wave_equation_operations.cc:640-657). */
+static void apply_cell(const ox_mesh *m, const ox_basis *b, int c, const double *src,
+                       double *dst, double sign, work *W) {
+  const int dim = m->dim, n = b->n, nf = dim + 1;
+  const int nd = ipow(n, dim);
+  const double *u = src + (long)c * nf * nd;
+  double *r = W->r, *jxw = W->jxw;
+  memset(r, 0, sizeof(double) * nf * nd);
+  cell_volume(m, b, c, u, r, jxw, W);
+  for (int f = 0; f < 2 * dim; ++f) {
+    const int nb = m->neighbor[c * 2 * dim + f];
+    cell_face(m, b, c, f, u, nb >= 0 ? src + (long)nb * nf * nd : NULL, 1.0, r);
+  }
   /* inverse mass: collocation at Gauss points -> divide by JxW */
   double *o = dst + (long)c * nf * nd;
   for (int k = 0; k < nf; ++k)
@@ -431,12 +449,14 @@ static work work_alloc(int dim, int n) {
   W.r = (double *)malloc(sizeof(double) * 4 * nd);
   W.g = (double *)malloc(sizeof(double) * 4 * nd);
   W.tmp = (double *)malloc(sizeof(double) * 16 * nd);
+  W.jxw = (double *)malloc(sizeof(double) * nd);
   return W;
 }
 static void work_free(work *W) {
   free(W->r);
   free(W->g);
   free(W->tmp);
+  free(W->jxw);
 }
 
 /* WaveEquationOperation::apply (source/wave_equation_operations.cc:665-685)
@@ -886,6 +906,479 @@ double ox_energy(const ox_mesh *m, int degree, const double *u) {
 }
 
 /* ------------------------------------------------------------------------ */
+/* ADER with local time stepping (WaveEquationOperationADERLTS,               */
+/* source/wave_equation_operations.cc:1385-1692, and ClusterManager,          */
+/* include/cluster_manager.templates.h:24-1049), restated per cell.           */
+/*                                                                            */
+/* The reference keeps its flags per cell batch of VectorizedArray width and  */
+/* its face masks per lane; batches are homogeneous in (cluster, has faster   */
+/* neighbour, has slower neighbour) because the categories of                 */
+/* iterate_cluster_categorization are handed to MatrixFree.  Every quantity   */
+/* that reaches a used result is a per-cell one, so the restatement works on  */
+/* cells (batch width 1).                                                     */
+/* ------------------------------------------------------------------------ */
+
+struct ox_lts {
+  const ox_mesh *m;
+  int degree, recon;
+  ox_basis b;
+  long N;
+  int nd, nf;
+  int n_clusters, cluster_diff, max_level;
+  double fastest_dt, global_dt, rel_tol;
+  int *cluster;          /* per cell */
+  char *faster, *slower; /* per cell: has a faster / slower face neighbour */
+  int *mult;             /* per cluster: time step multiple */
+  double *cl_time;       /* per cluster: time level */
+  int n_updates;
+  int *order;            /* cluster_update_order */
+  double *times;         /* cluster_update_times [n_updates][2] */
+  int iterations;
+  /* vectors */
+  double *state, *tempsrc, *dst, *fm, *igd, *rstate;
+  /* per-cell flags of the current pass */
+  char *U, *E, *nbrU;
+  double *cell_time;
+  /* times of the current cycle */
+  double t1fa, t2fa, t1sl, t2sl, t1sa, t2sa, dt;
+  int fm_considered;
+};
+
+static double cell_min_vertex_distance(const ox_mesh *m, int c) {
+  const int dim = m->dim, nv = 1 << dim;
+  const double *V = m->vertices + (long)c * nv * dim;
+  double mind = 1e300;
+  for (int a = 0; a < nv; ++a)
+    for (int b = a + 1; b < nv; ++b) {
+      double s = 0.0;
+      for (int d = 0; d < dim; ++d) s += (V[a * dim + d] - V[b * dim + d]) * (V[a * dim + d] - V[b * dim + d]);
+      if (s < mind) mind = s;
+    }
+  return sqrt(mind);
+}
+
+/* iterate_cluster_categorization (include/cluster_manager.templates.h:24-262)
+ * on one process: cells visited in active-cell order, ids changed in place. */
+static void lts_categorize(ox_lts *L, double cfl, int max_clusters, int max_diff) {
+  const ox_mesh *m = L->m;
+  const int n = m->n_cells, nface = 2 * m->dim;
+  double *ts = (double *)malloc(sizeof(double) * n);
+  double minimal = 1e300;
+  for (int c = 0; c < n; ++c) {
+    ts[c] = cfl * cell_min_vertex_distance(m, c); /* no wave speed here (:60) */
+    if (ts[c] < minimal) minimal = ts[c];
+  }
+  const double eps = minimal / 10.0;
+  int *id = L->cluster;
+  int nc = 0;
+  for (int c = 0; c < n; ++c) {
+    id[c] = (int)((ts[c] - eps) / minimal);
+    if (id[c] > nc) nc = id[c];
+  }
+  nc++;
+  int diff = 1;
+  if (nc > max_clusters) {
+    diff = nc / max_clusters + 1;
+    if (diff > max_diff) diff = max_diff;
+    for (int c = 0; c < n; ++c) id[c] /= diff;
+  }
+  int fs = 1, count = 0;
+  while (fs && count < 100) {
+    count++;
+    for (int cl = 0; cl < nc; ++cl)
+      for (int c = 0; c < n; ++c) {
+        if (id[c] != cl) continue;
+        int anyf = 0;
+        for (int f = 0; f < nface; ++f) {
+          int nb = m->neighbor[c * nface + f];
+          if (nb >= 0 && id[nb] < cl) anyf = 1;
+        }
+        for (int f = 0; f < nface; ++f) {
+          int nb = m->neighbor[c * nface + f];
+          if (nb >= 0 && id[nb] > cl) id[nb] = anyf ? cl : cl + 1;
+        }
+      }
+    fs = 0;
+    for (int c = 0; c < n; ++c) {
+      const int cur = id[c];
+      int fa = 0, sl = 0;
+      for (int f = 0; f < nface; ++f) {
+        int nb = m->neighbor[c * nface + f];
+        if (nb < 0) continue;
+        if (id[nb] < cur)
+          fa = 1;
+        else if (id[nb] > cur)
+          sl = 1;
+        if (id[nb] < cur - 1 || id[nb] > cur + 1) fs = 1;
+      }
+      L->faster[c] = (char)fa;
+      L->slower[c] = (char)sl;
+      if (fa && sl) fs = 1;
+    }
+  }
+  L->iterations = count;
+  nc = 0;
+  for (int c = 0; c < n; ++c)
+    if (id[c] > nc) nc = id[c];
+  L->n_clusters = nc + 1;
+  L->cluster_diff = diff;
+  free(ts);
+}
+
+/* ClusterManager::setup (include/cluster_manager.templates.h:330-484) */
+static void lts_schedule(ox_lts *L, double time0) {
+  const int nc = L->n_clusters, diff = L->cluster_diff;
+  const double D = L->fastest_dt;
+  L->mult = (int *)malloc(sizeof(int) * nc);
+  L->cl_time = (double *)malloc(sizeof(double) * nc);
+  for (int c = 0; c < nc; ++c) {
+    L->mult[c] = diff * c + 1;
+    L->cl_time[c] = time0;
+  }
+  const int max_level = diff * (nc - 1) + 1;
+  L->max_level = max_level;
+  int *tl = (int *)calloc(nc, sizeof(int));
+  int cap = 64;
+  L->order = (int *)malloc(sizeof(int) * cap);
+  L->times = (double *)malloc(sizeof(double) * 2 * cap);
+  int nu = 0;
+#define MINL(a) ((a) < max_level ? (a) : max_level)
+  for (;;) {
+    int mn = tl[0];
+    for (int c = 1; c < nc; ++c)
+      if (tl[c] < mn) mn = tl[c];
+    if (mn >= max_level) break;
+    for (int c = 0; c < nc; ++c) {
+      if (tl[c] >= max_level) continue;
+      const int mine = MINL(tl[c] + L->mult[c]);
+      int ok;
+      if (nc == 1)
+        ok = 1;
+      else if (c == 0)
+        ok = mine <= MINL(tl[c + 1] + L->mult[c + 1]);
+      else if (c == nc - 1)
+        ok = mine <= MINL(tl[c - 1] + L->mult[c - 1]);
+      else
+        ok = mine <= MINL(tl[c + 1] + L->mult[c + 1]) && mine <= MINL(tl[c - 1] + L->mult[c - 1]);
+      if (!ok) continue;
+      if (nu == cap) {
+        cap *= 2;
+        L->order = (int *)realloc(L->order, sizeof(int) * cap);
+        L->times = (double *)realloc(L->times, sizeof(double) * 2 * cap);
+      }
+      L->times[2 * nu] = tl[c] * D;
+      tl[c] += L->mult[c];
+      L->order[nu] = c;
+      L->times[2 * nu + 1] = tl[c] * D;
+      nu++;
+    }
+  }
+#undef MINL
+  free(tl);
+  L->n_updates = nu;
+  L->global_dt = D * (diff * (nc - 1) + 1);
+  for (int u = 0; u < nu; ++u) {
+    if (L->times[2 * u + 1] > max_level * D) L->times[2 * u + 1] = max_level * D;
+    L->times[2 * u] += time0;
+    L->times[2 * u + 1] += time0;
+  }
+}
+
+ox_lts *ox_lts_create(const ox_mesh *m, int degree, double cfl, double fastest_dt,
+                      int max_clusters, int max_diff, double time0, int recon) {
+  ox_lts *L = (ox_lts *)calloc(1, sizeof(ox_lts));
+  L->m = m;
+  L->degree = degree;
+  L->recon = recon;
+  ox_basis_init(&L->b, degree);
+  L->N = ox_n_dofs(m, degree);
+  L->nf = m->dim + 1;
+  L->nd = ipow(degree + 1, m->dim);
+  L->rel_tol = 1e-4; /* ClusterManager(rel_tol = 1e-4), include/cluster_manager.h:34 */
+  const int n = m->n_cells;
+  L->cluster = (int *)calloc(n, sizeof(int));
+  L->faster = (char *)calloc(n, 1);
+  L->slower = (char *)calloc(n, 1);
+  L->U = (char *)calloc(n, 1);
+  L->E = (char *)calloc(n, 1);
+  L->nbrU = (char *)calloc(n, 1);
+  L->cell_time = (double *)calloc(n, sizeof(double));
+  lts_categorize(L, cfl, max_clusters, max_diff);
+  L->fastest_dt = fastest_dt;
+  lts_schedule(L, time0);
+  const long N = L->N;
+  L->state = (double *)calloc(N, sizeof(double));
+  L->tempsrc = (double *)calloc(N, sizeof(double));
+  L->dst = (double *)calloc(N, sizeof(double));
+  L->fm = (double *)calloc(N, sizeof(double));
+  L->igd = (double *)calloc(N, sizeof(double));
+  L->rstate = (double *)calloc(N, sizeof(double));
+  return L;
+}
+
+void ox_lts_free(ox_lts *L) {
+  if (!L) return;
+  free(L->cluster);
+  free(L->faster);
+  free(L->slower);
+  free(L->U);
+  free(L->E);
+  free(L->nbrU);
+  free(L->cell_time);
+  free(L->mult);
+  free(L->cl_time);
+  free(L->order);
+  free(L->times);
+  free(L->state);
+  free(L->tempsrc);
+  free(L->dst);
+  free(L->fm);
+  free(L->igd);
+  free(L->rstate);
+  free(L);
+}
+
+int ox_lts_n_clusters(const ox_lts *L) { return L->n_clusters; }
+int ox_lts_cluster_diff(const ox_lts *L) { return L->cluster_diff; }
+int ox_lts_iterations(const ox_lts *L) { return L->iterations; }
+double ox_lts_global_dt(const ox_lts *L) { return L->global_dt; }
+int ox_lts_n_updates(const ox_lts *L) { return L->n_updates; }
+void ox_lts_schedule(const ox_lts *L, int *order, double *times) {
+  for (int u = 0; u < L->n_updates; ++u) {
+    order[u] = L->order[u];
+    times[2 * u] = L->times[2 * u];
+    times[2 * u + 1] = L->times[2 * u + 1];
+  }
+}
+void ox_lts_cells(const ox_lts *L, int *cluster, int *flags) {
+  for (int c = 0; c < L->m->n_cells; ++c) {
+    cluster[c] = L->cluster[c];
+    flags[c] = L->slower[c] + 2 * L->faster[c]; /* element_categories % 3 (:252) */
+  }
+}
+
+/* CK time integral over [t1,t2] about each flagged cell's own time level
+ * (local_apply_firstader_domain :1385-1408), into tempsrc */
+static void lts_ck_cells(ox_lts *L, const double *src, const char *flag, double t1, double t2) {
+  const ox_mesh *m = L->m;
+  const long cs = (long)L->nf * L->nd;
+  for (int c = 0; c < m->n_cells; ++c) {
+    if (!flag[c]) continue;
+    ox_mesh one = *m;
+    one.n_cells = 1;
+    one.vertices = m->vertices + (long)c * (1 << m->dim) * m->dim;
+    one.rho = m->rho + c;
+    one.speed = m->speed + c;
+    one.neighbor = m->neighbor + (long)c * 2 * m->dim;
+    ox_ader_ck(&one, L->degree, src + c * cs, L->recon ? L->igd + c * cs : NULL, t2, t1,
+               L->cell_time[c], L->tempsrc + c * cs);
+  }
+}
+
+/* contribution of face f of cell c (evaluated from tempsrc on both sides,
+ * boundary_fac -1 as in local_apply_ader_face :1514-1537) added into out */
+static void lts_face_into(ox_lts *L, int c, int f, double *out, work *W) {
+  const ox_mesh *m = L->m;
+  const long cs = (long)L->nf * L->nd;
+  const int nb = m->neighbor[c * 2 * m->dim + f];
+  memset(W->r, 0, sizeof(double) * cs);
+  cell_face(m, &L->b, c, f, L->tempsrc + c * cs, nb >= 0 ? L->tempsrc + (long)nb * cs : NULL,
+            -1.0, W->r);
+  for (long i = 0; i < cs; ++i) out[c * cs + i] += W->r[i];
+}
+
+/* ClusterManager::update_elements (include/cluster_manager.templates.h:807-1049)
+ * for cluster `ac` and the update set L->U */
+static void lts_update_elements(ox_lts *L, double *lstate, int ac, int write_fm) {
+  const ox_mesh *m = L->m;
+  const int n = m->n_cells, nface = 2 * m->dim, nc = L->n_clusters;
+  const long cs = (long)L->nf * L->nd;
+  const double tol = L->rel_tol * L->fastest_dt;
+  const int *cl = L->cluster;
+  char *U = L->U, *E = L->E, *nbrU = L->nbrU;
+  work W = work_alloc(m->dim, L->b.n);
+  for (int c = 0; c < n; ++c) {
+    nbrU[c] = 0;
+    if (U[c]) continue;
+    for (int f = 0; f < nface; ++f) {
+      int nb = m->neighbor[c * nface + f];
+      if (nb >= 0 && U[nb]) nbrU[c] = 1;
+    }
+  }
+  /* contributions across the faces to the faster (ac-1) and slower (ac+1)
+   * cluster: the actual cluster's side goes to dst, the other side to the
+   * flux memory (:862-993) */
+  for (int side = 0; side < 2; ++side) {
+    const int other = side == 0 ? ac - 1 : ac + 1;
+    if (other < 0 || other >= nc) continue;
+    const double t1 = side == 0 ? L->t1fa : L->t1sl, t2 = side == 0 ? L->t2fa : L->t2sl;
+    if (!(fabs(t2 - t1) > tol)) continue;
+    for (int c = 0; c < n; ++c)
+      E[c] = (cl[c] == ac && (side == 0 ? L->faster[c] : L->slower[c]) && U[c]) ||
+             (cl[c] == other && (side == 0 ? L->slower[c] : L->faster[c]) && nbrU[c]);
+    lts_ck_cells(L, lstate, E, t1, t2);
+    for (int c = 0; c < n; ++c) {
+      if (!E[c]) continue;
+      for (int f = 0; f < nface; ++f) {
+        int nb = m->neighbor[c * nface + f];
+        if (nb < 0 || !E[nb]) continue;
+        if (cl[c] == ac && cl[nb] != ac && nbrU[nb])
+          lts_face_into(L, c, f, L->dst, &W);
+        else if (cl[c] != ac && cl[nb] == ac && nbrU[c] && write_fm)
+          lts_face_into(L, c, f, L->fm, &W);
+      }
+    }
+  }
+  /* contribution within the actual cluster and its boundary (:996-1049) */
+  for (int c = 0; c < n; ++c) E[c] = cl[c] == ac && (U[c] || nbrU[c]);
+  lts_ck_cells(L, lstate, E, L->t1sa, L->t2sa);
+  for (int c = 0; c < n; ++c) {
+    if (!E[c]) continue;
+    for (int f = 0; f < nface; ++f) {
+      int nb = m->neighbor[c * nface + f];
+      int take;
+      if (nb < 0)
+        take = 1;
+      else
+        take = (U[c] && U[nb]) || (U[c] && E[nb] && cl[nb] == ac) || (E[c] && U[nb] && cl[c] == ac);
+      if (take) lts_face_into(L, c, f, L->dst, &W);
+    }
+  }
+  /* second ADER sweep on the update cells (local_apply_secondader_domain
+   * :1413-1497): -volume(CK over [0,dt]) + flux memory + faces, inverse mass */
+  double *V = (double *)malloc(sizeof(double) * cs);
+  for (int c = 0; c < n; ++c) {
+    if (!U[c]) continue;
+    ox_mesh one = *m;
+    one.n_cells = 1;
+    one.vertices = m->vertices + (long)c * (1 << m->dim) * m->dim;
+    one.rho = m->rho + c;
+    one.speed = m->speed + c;
+    one.neighbor = m->neighbor + (long)c * nface;
+    ox_ader_ck(&one, L->degree, lstate + c * cs, L->recon ? L->igd + c * cs : NULL, L->dt, 0.0,
+               0.0, V);
+    double *r = W.r;
+    memset(r, 0, sizeof(double) * cs);
+    cell_volume(m, &L->b, c, V, r, W.jxw, &W);
+    for (long i = 0; i < cs; ++i) r[i] = -r[i];
+    if (L->fm_considered)
+      for (long i = 0; i < cs; ++i) {
+        r[i] += L->fm[c * cs + i];
+        L->fm[c * cs + i] = 0.0;
+      }
+    for (long i = 0; i < cs; ++i) r[i] += L->dst[c * cs + i];
+    for (int k = 0; k < L->nf; ++k)
+      for (int q = 0; q < L->nd; ++q) L->dst[c * cs + k * L->nd + q] = r[k * L->nd + q] / W.jxw[q];
+  }
+  free(V);
+  /* local_state -= dst; dst = 0 (:1046-1047).  Cells outside U that received
+   * face terms are at distance >= 2 from every cell the caller reads (see
+   * ox_lts_step), and get the same subtraction as in the reference. */
+  for (long i = 0; i < L->N; ++i) {
+    lstate[i] -= L->dst[i];
+    L->dst[i] = 0.0;
+  }
+  work_free(&W);
+}
+
+/* M^-1 A(src) on the flagged cells only (reconstruct_div_grad :1593-1680) */
+static void lts_reconstruct(ox_lts *L, const double *src, const char *flag) {
+  const ox_mesh *m = L->m;
+  ox_basis *b = &L->b;
+  work W = work_alloc(m->dim, b->n);
+  for (int c = 0; c < m->n_cells; ++c)
+    if (flag[c]) apply_cell(m, b, c, src, L->igd, 1.0, &W);
+  work_free(&W);
+}
+
+/* ClusterManager::perform_time_step (include/cluster_manager.templates.h:
+ * 488-803); time_now is the time control's time after advance_time_step. */
+void ox_lts_step(ox_lts *L, double time_now, const double *src, double *dst) {
+  const ox_mesh *m = L->m;
+  const int n = m->n_cells, nc = L->n_clusters, nface = 2 * m->dim;
+  const double D = L->fastest_dt, tol = L->rel_tol * D;
+  const int *cl = L->cluster;
+  memcpy(L->state, src, sizeof(double) * L->N);
+  memset(L->dst, 0, sizeof(double) * L->N);
+  if (L->recon) ox_apply(m, L->degree, L->state, L->igd, 1.0);
+  for (int c = 0; c < n; ++c) L->cell_time[c] = L->cl_time[cl[c]];
+  char *tf = (char *)malloc(n), *Ucopy = (char *)malloc(n);
+  for (int cy = 0; cy < L->n_updates; ++cy) {
+    const int ac = L->order[cy];
+    double act = L->cl_time[ac];
+    L->fm_considered = 1;
+    for (int c = 0; c < n; ++c) L->U[c] = cl[c] == ac;
+    if (ac > 0) {
+      const double ft = L->cl_time[ac - 1];
+      L->t1fa = fmax(ft, act);
+      L->t2fa = fmin(ft + L->mult[ac - 1] * D, act + L->mult[ac] * D);
+      L->t2fa = fmin(L->t2fa, time_now);
+    } else
+      L->t1fa = L->t2fa = 0.0;
+    if (ac < nc - 1) {
+      const double st = L->cl_time[ac + 1];
+      L->t1sl = fmax(st, act);
+      L->t2sl = fmin(st + L->mult[ac + 1] * D, act + L->mult[ac] * D);
+      L->t2sl = fmin(L->t2sl, time_now);
+    } else
+      L->t1sl = L->t2sl = 0.0;
+    L->t1sa = act;
+    L->t2sa = fmin(act + L->mult[ac] * D, time_now);
+    L->dt = fmin(L->mult[ac] * D, time_now - L->cl_time[ac]);
+    lts_update_elements(L, L->state, ac, 1);
+    L->cl_time[ac] = L->times[2 * cy + 1];
+    for (int c = 0; c < n; ++c) L->cell_time[c] = L->cl_time[cl[c]];
+    if (!L->recon) continue;
+    /* reconstruction with the neighbouring clusters brought to the new time
+     * level of `ac` (:608-767); only cells next to `ac` are read by it */
+    L->fm_considered = 0;
+    memcpy(L->rstate, L->state, sizeof(double) * L->N);
+    act = L->cl_time[ac];
+    for (int side = 0; side < 2; ++side) {
+      const int other = side == 0 ? ac - 1 : ac + 1;
+      if (other < 0 || other >= nc) continue;
+      if (!(fabs(act - L->cl_time[other]) > tol)) continue;
+      for (int c = 0; c < n; ++c) tf[c] = cl[c] == other && (side == 0 ? L->slower[c] : L->faster[c]);
+      /* the reference widens the set by further cells of `other` (:631-647,
+       * :682-694); their updates are never read, the same cells are updated
+       * here for the same arithmetic */
+      for (int c = 0; c < n; ++c) {
+        Ucopy[c] = 0;
+        if (cl[c] != other) continue;
+        if (side == 0) {
+          /* the last face with a neighbour decides (:635-645) */
+          for (int f = 0; f < nface; ++f) {
+            int nb = m->neighbor[c * nface + f];
+            if (nb >= 0) Ucopy[c] = tf[nb];
+          }
+        } else {
+          for (int f = 0; f < nface; ++f) {
+            int nb = m->neighbor[c * nface + f];
+            if (nb >= 0 && tf[nb]) Ucopy[c] = 1;
+          }
+        }
+      }
+      for (int c = 0; c < n; ++c) L->U[c] = Ucopy[c] || tf[c];
+      L->t1fa = L->cl_time[other];
+      L->t2fa = act;
+      L->t1sl = L->t1sa = L->t1fa;
+      L->t2sl = L->t2sa = L->t2fa;
+      L->dt = L->t2fa - L->t1fa;
+      lts_update_elements(L, L->rstate, other, 0);
+    }
+    for (int c = 0; c < n; ++c) tf[c] = cl[c] == ac;
+    lts_reconstruct(L, L->rstate, tf);
+  }
+  for (int u = 0; u < L->n_updates; ++u) {
+    L->times[2 * u] += L->global_dt;
+    L->times[2 * u + 1] += L->global_dt;
+  }
+  free(tf);
+  free(Ucopy);
+  memcpy(dst, L->state, sizeof(double) * L->N);
+}
+
+/* ------------------------------------------------------------------------ */
 /* driver: WaveEquationProblem::run with TimeControl                          */
 /* ------------------------------------------------------------------------ */
 
@@ -905,6 +1398,15 @@ int ox_run(const ox_problem *P, char *log, int log_len, double *final_energy) {
   max_step = P->max_time_steps == 0 ? (unsigned)round((T - time) / dt) : (unsigned)P->max_time_steps;
   /* make_dofs overwrites with the unrounded CFL step (source/explicit_wave.cc:222) */
   dt = dt_cfl;
+  ox_lts *lts = NULL;
+  if (P->integrator == OX_ADER_LTS) {
+    /* WaveEquationOperationADERLTS::setup: clusters from the unrounded CFL
+     * step, then the time control advances by the slowest cluster's step
+     * (include/cluster_manager.templates.h:463) */
+    lts = ox_lts_create(m, k, cfl, dt_cfl, P->lts_max_clusters, P->lts_max_diff, 0.0,
+                        P->ader_recon);
+    dt = ox_lts_global_dt(lts);
+  }
 
   double *sol = (double *)malloc(sizeof(double) * N);
   double *tmp = (double *)malloc(sizeof(double) * N);
@@ -929,12 +1431,16 @@ int ox_run(const ox_problem *P, char *log, int log_len, double *final_energy) {
     double *sw = tmp;
     tmp = sol;
     sol = sw;
-    ox_time_step(m, k, P->integrator, dt, tmp, sol, t1, t2, P->ader_recon);
+    if (lts)
+      ox_lts_step(lts, time, tmp, sol); /* ArbitraryHighOrderDGLTS, time_integrators.cc:54-63 */
+    else
+      ox_time_step(m, k, P->integrator, dt, tmp, sol, t1, t2, P->ader_recon);
     /* TimeControl::at_tick (utilities.cc:89-95) */
     if ((int)(time / tick) != (int)((time - dt) / tick)) OX_LOG();
   }
 #undef OX_LOG
   if (final_energy) *final_energy = ox_energy(m, k, sol);
+  ox_lts_free(lts);
   free(sol);
   free(tmp);
   free(t1);

@@ -101,7 +101,8 @@ enum {
   OX_LSRK45R2 = 2,
   OX_LSRK33R2 = 3,
   OX_LSRK59R2 = 4,
-  OX_ADER = 5
+  OX_ADER = 5,
+  OX_ADER_LTS = 6 /* ox_run only: ADER with local time stepping */
 };
 void ox_time_step(const ox_mesh *m, int degree, int integrator, double dt, double *vec_n,
                   double *vec_np, double *tmp1, double *tmp2, int ader_recon);
@@ -136,10 +137,32 @@ typedef struct ox_problem {
   double final_time, output_every_time;
   int ader_recon;
   ox_exact exact;
+  int lts_max_clusters, lts_max_diff; /* max_n_clusters, max_diff_clusters */
 } ox_problem;
 int ox_run(const ox_problem *p, char *log, int log_len, double *final_state_energy);
 
 long ox_n_dofs(const ox_mesh *m, int degree);
+
+/* ADER with local time stepping: WaveEquationOperationADERLTS
+ * (source/wave_equation_operations.cc:1385-1692) + ClusterManager
+ * (include/cluster_manager.templates.h:24-1049), per cell.
+ * cfl: cfl_number/k^1.5; fastest_dt: the unrounded CFL step;
+ * time0: the time at setup. */
+typedef struct ox_lts ox_lts;
+ox_lts *ox_lts_create(const ox_mesh *m, int degree, double cfl, double fastest_dt,
+                      int max_clusters, int max_diff, double time0, int recon);
+void ox_lts_free(ox_lts *L);
+int ox_lts_n_clusters(const ox_lts *L);
+int ox_lts_cluster_diff(const ox_lts *L);
+int ox_lts_iterations(const ox_lts *L);
+double ox_lts_global_dt(const ox_lts *L);
+int ox_lts_n_updates(const ox_lts *L);
+/* order[n_updates], times[n_updates][2] (absolute, current step) */
+void ox_lts_schedule(const ox_lts *L, int *order, double *times);
+/* cluster id per cell; flags: 1 = has slower neighbour, 2 = has faster */
+void ox_lts_cells(const ox_lts *L, int *cluster, int *flags);
+/* perform_time_step: time_now = time after TimeControl::advance_time_step */
+void ox_lts_step(ox_lts *L, double time_now, const double *src, double *dst);
 
 #ifdef __cplusplus
 }

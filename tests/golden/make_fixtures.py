@@ -36,7 +36,7 @@ for fn in sorted(os.listdir(tdir)):
         m = re.match(r"#\s*mpirun:\s*(\d+)", line)
         if m:
             params["mpirun"] = int(m.group(1))
-    rows, steps, dts = [], None, []
+    rows, steps, dts, clusters, lts = [], None, [], [], {}
     for line in open(os.path.join(tdir, name + ".output")):
         m = re.match(r"\s*Time:\s*" + num + r"\s*, error p:\s*" + num + r"\s*, error p post:\s*"
                      + num + r"\s*, error v:\s*" + num + r"\s*, solution mag p:\s*" + num, line)
@@ -48,6 +48,22 @@ for fn in sorted(os.listdir(tdir)):
         m = re.match(r"\s*Time step size:\s*" + num, line)
         if m:
             dts.append(m.group(1))
+        # ADER-LTS cluster report (cells are counted in batches of the
+        # VectorizedArray width the run printed)
+        m = re.match(r"\s*cluster\s+(\d+) contains (\d+) cells and works with time step\s*"
+                     + num, line)
+        if m:
+            clusters.append([int(m.group(1)), int(m.group(2)), m.group(3)])
+        m = re.match(r"\s*n_clusters (\d+) clusterdiff (\d+) levelmax (\d+)", line)
+        if m:
+            lts.update(n_clusters=int(m.group(1)), cluster_diff=int(m.group(2)),
+                       level_max=int(m.group(3)))
+        m = re.match(r"\s*needed to iterate (\d+) times", line)
+        if m:
+            lts["iterations"] = int(m.group(1))
+        m = re.match(r"\s*Vectorization over (\d+) doubles", line)
+        if m:
+            params["n_vect"] = int(m.group(1))
         m = re.match(r"\s*Number of global active cells:\s*(\d+)", line)
         if m:
             params["n_cells"] = int(m.group(1))
@@ -56,6 +72,9 @@ for fn in sorted(os.listdir(tdir)):
             params["n_dofs"] = int(m.group(1))
     out[name] = {"params": params, "error_lines": rows, "n_steps": steps,
                  "time_step_sizes": dts}
+    if clusters:
+        lts["clusters"] = clusters
+        out[name]["lts"] = lts
 dst = os.path.join(os.path.dirname(os.path.abspath(__file__)), "reference_fixtures.json")
 with open(dst, "w") as f:
     json.dump({"source": "kronbichler/exwave tests/*.prm + tests/*.output (numbers only)",

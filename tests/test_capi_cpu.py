@@ -46,3 +46,21 @@ def test_mesh_generator_rejects_bad_input():
         X.mesh_hypercube(4, 2)
     with pytest.raises(X.ExwError):
         X.mesh_hypercube(2, 2, 0, 0, 0.0, 7)
+
+
+@pytest.mark.parametrize("dim,nint,f,k,maxc,maxd", [(2, 15, 0.2, 5, 5, 1), (2, 12, 0.25, 3, 5, 2),
+                                                    (2, 9, 0.3, 2, 3, 1), (3, 6, 0.2, 3, 5, 1),
+                                                    (3, 5, 0.25, 2, 2, 3)])
+def test_lts_categorize_matches_oracle(dim, nint, f, k, maxc, maxd):
+    """The library's host-side cluster categorisation (exw_lts_categorize) gives
+    the oracle's clusters and faster/slower flags."""
+    import exwave_amd as X
+    from tests.test_oracle import lts_info
+    V, nb = X.mesh_hypercube(dim, nint, 0, X.TRANSFORM_REFERENCE_SINE, f, 2)
+    cfl = 0.2 / k ** 1.5
+    cl, fl, nc, diff, it = X.lts_categorize(dim, V, nb, cfl, maxc, maxd)
+    m = O.Mesh(dim, nint, 0, O.TRANSFORM_REFERENCE_SINE, f, 2)
+    info = lts_info(m, k, 0.2, maxc, maxd)
+    assert nc == info["n_clusters"] and diff == info["cluster_diff"]
+    assert it == info["iterations"]
+    assert np.array_equal(cl, info["cluster"]) and np.array_equal(fl, info["flags"])

@@ -28,11 +28,13 @@ def _problem(params, transform=O.TRANSFORM_REFERENCE_SINE):
     p.output_every_time = params["output_every_time"]
     p.ader_recon = 1 if params.get("use_ader_post", False) else 0
     p.exact = O.Exact.reference(p.dim, params["membrane_modes"])
+    p.lts_max_clusters = params.get("max_n_clusters", 10)
+    p.lts_max_diff = params.get("max_diff_clusters", 7)
     return p
 
 
 FIXTURES = ["lsrk45r2_2d_cartesian", "clrk4_3d", "ader_2d_norecon_ref2", "ader_2d_recon_ref2",
-            "ader_2d_recon_ref3", "ader_3d_norecon"]
+            "ader_2d_recon_ref3", "ader_3d_norecon", "aderlts_2d_recon"]
 
 
 @pytest.mark.parametrize("name", FIXTURES)
@@ -49,7 +51,10 @@ def test_reference_fixture(fixtures, name):
                O.TRANSFORM_REFERENCE_SINE, prm["grid_transform_factor"])
     assert m.n_cells == prm["n_cells"] and m.n_dofs(k) == prm["n_dofs"]
     dt = prm["cfl_number"] / k ** 1.5 * m.min_vertex_distance_over_c
-    assert "%g" % dt == fx["time_step_sizes"][-1]
+    if "lts" in fx:  # LTS prints the slowest cluster's step; the CFL step is cluster 0's
+        assert "%g" % dt == fx["lts"]["clusters"][0][2]
+    else:
+        assert "%g" % dt == fx["time_step_sizes"][-1]
     assert len(rows) == len(fx["error_lines"])
     for (st, t, ep, ev, mp), ref in zip(rows, fx["error_lines"]):
         assert "%.2f" % t == ref[0]
@@ -236,3 +241,126 @@ def test_stability_polynomial(integ, stages, order):
         assert abs(gam[j] - 1.0 / factorial(j)) < 1e-7, (j, gam)
     if integ == O.LSRK45R2:
         assert abs(gam[5] - 0.0048544) < 1e-6
+
+
+def lts_info(m, k, cfl_number, max_clusters, max_diff, recon=1):
+    """Clusters, flags, schedule and steps of the oracle's ADER-LTS set-up."""
+    import ctypes as C
+    L = O.lib()
+    cfl = cfl_number / k ** 1.5
+    fastest = cfl * m.min_vertex_distance_over_c
+    h = L.ox_lts_create(m._m, k, cfl, fastest, max_clusters, max_diff, 0.0, recon)
+    try:
+        nc, nu = L.ox_lts_n_clusters(h), L.ox_lts_n_updates(h)
+        cl = np.zeros(m.n_cells, np.int32)
+        fl = np.zeros(m.n_cells, np.int32)
+        ip = C.POINTER(C.c_int)
+        L.ox_lts_cells(h, cl.ctypes.data_as(ip), fl.ctypes.data_as(ip))
+        order = np.zeros(nu, np.int32)
+        times = np.zeros(2 * nu)
+        L.ox_lts_schedule(h, order.ctypes.data_as(ip), O._p(times))
+        return dict(n_clusters=nc, cluster_diff=L.ox_lts_cluster_diff(h),
+                    iterations=L.ox_lts_iterations(h), global_dt=L.ox_lts_global_dt(h),
+                    fastest=fastest, cluster=cl, flags=fl, order=order,
+                    times=times.reshape(-1, 2))
+    finally:
+        L.ox_lts_free(h)
+
+
+def test_lts_clusters_fixture(fixtures):
+    """The reference's cluster report for aderlts_2d_recon: cluster count, the
+    categorisation iterations, the per-cluster time steps (multiples 1, 2, 3 of
+    the fastest step) and the cell counts, which the run prints in cell batches
+    of VectorizedArray width 2 formed per category (cluster, faster/slower
+    neighbour flags), include/cluster_manager.templates.h:252, 402-416."""
+    fx = fixtures["aderlts_2d_recon"]
+    prm, lts = fx["params"], fx["lts"]
+    k = prm["fe_degree"]
+    m = O.Mesh(2, prm["n_initial_intervals"], 0, O.TRANSFORM_REFERENCE_SINE,
+               prm["grid_transform_factor"])
+    info = lts_info(m, k, prm["cfl_number"], prm["max_n_clusters"], prm["max_diff_clusters"])
+    assert info["n_clusters"] == lts["n_clusters"]
+    assert info["cluster_diff"] == lts["cluster_diff"]
+    assert info["iterations"] == lts["iterations"]
+    nv = prm["n_vect"]
+    for c, batches, dt in lts["clusters"]:
+        cats = [int(np.sum((info["cluster"] == c) & (info["flags"] == f))) for f in range(3)]
+        assert sum((x + nv - 1) // nv for x in cats) == batches, (c, cats, batches)
+        assert "%g" % ((lts["cluster_diff"] * c + 1) * info["fastest"]) == dt
+    assert "%g" % info["global_dt"] == fx["time_step_sizes"][-1]
+
+
+def test_lts_invariants():
+    """Clustering invariants (include/cluster_manager.templates.h:100-240):
+    face neighbours differ by at most one cluster, no cell has both a faster
+    and a slower neighbour, flags match the neighbours; every cluster reaches
+    the global step exactly once its updates are summed, and each update of a
+    cluster starts where its previous one ended."""
+    for nint, f, k in [(15, 0.2, 5), (12, 0.25, 3), (9, 0.3, 2)]:
+        m = O.Mesh(2, nint, 0, O.TRANSFORM_REFERENCE_SINE, f)
+        info = lts_info(m, k, 0.2, 5, 2)
+        cl, fl, nb = info["cluster"], info["flags"], m.neighbor
+        for c in range(m.n_cells):
+            nbs = [x for x in nb[c] if x >= 0]
+            assert all(abs(cl[x] - cl[c]) <= 1 for x in nbs)
+            fa = any(cl[x] < cl[c] for x in nbs)
+            sl = any(cl[x] > cl[c] for x in nbs)
+            assert not (fa and sl)
+            assert fl[c] == int(sl) + 2 * int(fa)
+        D, nc, diff = info["fastest"], info["n_clusters"], info["cluster_diff"]
+        level = np.zeros(nc)
+        for u, c in enumerate(info["order"]):
+            assert abs(info["times"][u, 0] - level[c] * D) < 1e-12 * D * 1e3
+            level[c] = info["times"][u, 1] / D
+        assert np.allclose(level, diff * (nc - 1) + 1)
+        assert abs(info["global_dt"] - D * (diff * (nc - 1) + 1)) < 1e-15
+
+
+def test_lts_single_cluster_is_ader():
+    """With one cluster (Cartesian mesh: all cells alike) ADER-LTS is the
+    global ADER step with the same dt."""
+    m = O.Mesh(2, 4, 0, O.TRANSFORM_NONE, 0.0)
+    k = 3
+    info = lts_info(m, k, 0.2, 5, 1)
+    assert info["n_clusters"] == 1
+    e = O.Exact.make(2, [[1, 2]])
+    u0 = m.project(k, e, 0.0)
+    import ctypes as C
+    L = O.lib()
+    cfl = 0.2 / k ** 1.5
+    dt = cfl * m.min_vertex_distance_over_c
+    for recon in (0, 1):
+        h = L.ox_lts_create(m._m, k, cfl, dt, 5, 1, 0.0, recon)
+        out = np.zeros_like(u0)
+        L.ox_lts_step(h, dt, O._p(u0.copy()), O._p(out))
+        L.ox_lts_free(h)
+        ref = u0 - m.apply_ader(k, u0, dt, recon)
+        assert np.max(np.abs(out - ref)) <= 1e-13 * np.max(np.abs(ref))
+
+
+def test_lts_accuracy_two_clusters():
+    """On a deformed mesh with several clusters one LTS step (every cluster
+    advanced by its own multiple of the fastest step, fluxes between clusters
+    split in time through the flux memory) is at least as accurate against
+    the exact solution as one global ADER step of the same total length."""
+    m = O.Mesh(2, 6, 0, O.TRANSFORM_REFERENCE_SINE, 0.25, 2)
+    k = 3
+    info = lts_info(m, k, 0.2, 5, 1)
+    assert info["n_clusters"] >= 2
+    e = O.Exact.make(2, [[1, 1]])
+    u0 = m.project(k, e, 0.0)
+    import ctypes as C
+    L = O.lib()
+    cfl = 0.2 / k ** 1.5
+    dt = cfl * m.min_vertex_distance_over_c
+    h = L.ox_lts_create(m._m, k, cfl, dt, 5, 1, 0.0, 0)
+    gdt = L.ox_lts_global_dt(h)
+    out = np.zeros_like(u0)
+    L.ox_lts_step(h, gdt, O._p(u0.copy()), O._p(out))
+    L.ox_lts_free(h)
+    # one global ADER step with the slowest step is a different scheme with
+    # the same order: both approximate the exact solution at t = gdt
+    ref = u0 - m.apply_ader(k, u0, gdt, 0)
+    err_lts = m.l2_errors(k, e, gdt, out)
+    err_glob = m.l2_errors(k, e, gdt, ref)
+    assert err_lts[0] < 2 * err_glob[0] + 1e-9 and err_lts[1] < 2 * err_glob[1] + 1e-9
