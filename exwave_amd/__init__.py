@@ -15,7 +15,7 @@ from __future__ import annotations
 
 import ctypes as C
 
-from .capi import (ADER, ADER_LTS, CLRK4, EXPL_EULER, FP32, FP64, INTEGRATORS, LSRK33R2, LSRK45R2,  # noqa
+from .capi import (ADER, ADER_LTS, CLRK4, EXPL_EULER, FP32, FP64, PRECISION_AUTO, INTEGRATORS, LSRK33R2, LSRK45R2,  # noqa
                    LSRK59R2, TRANSFORM_AXIS_2PI, TRANSFORM_NONE, TRANSFORM_PRODUCT_2PI,
                    TRANSFORM_REFERENCE_SINE, Epilogue, ExwError, Modes, Problem, check, load,
                    mesh_hypercube)
@@ -49,7 +49,10 @@ class Context:
 
     def __del__(self):
         if getattr(self, "h", None) is not None:
-            load().exw_ctx_destroy(self.h)
+            try:
+                load().exw_ctx_destroy(self.h)
+            except Exception:  # interpreter shutdown
+                pass
             self.h = None
 
 
@@ -90,7 +93,10 @@ class WaveOperator:
 
     def __del__(self):
         if getattr(self, "h", None) is not None:
-            load().exw_op_destroy(self.h)
+            try:
+                load().exw_op_destroy(self.h)
+            except Exception:  # interpreter shutdown
+                pass
             self.h = None
 
     def new_vector(self, n=1):
@@ -182,7 +188,10 @@ class LocalTimeStepping:
 
     def __del__(self):
         if getattr(self, "h", None) is not None:
-            load().exw_lts_destroy(self.h)
+            try:
+                load().exw_lts_destroy(self.h)
+            except Exception:  # interpreter shutdown
+                pass
             self.h = None
 
 

@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_lib", "libexwave_amd.so")
 
 EXW_OK, EXW_EINVAL, EXW_EDEVICE, EXW_ENOMEM, EXW_EUNSUP = 0, -1, -2, -3, -4
-FP64, FP32 = 0, 1
+FP64, FP32, PRECISION_AUTO = 0, 1, 2
 EXPL_EULER, CLRK4, LSRK45R2, LSRK33R2, LSRK59R2, ADER, ADER_LTS = range(7)
 INTEGRATORS = {"ExplEuler": EXPL_EULER, "clRK4": CLRK4, "LSRK45R2": LSRK45R2,
                "LSRK33R2": LSRK33R2, "LSRK59R2": LSRK59R2, "ADER": ADER, "ADERLTS": ADER_LTS}
@@ -62,7 +62,8 @@ class Request(C.Structure):
 
 class Result(C.Structure):
     _fields_ = [("id", C.c_int64), ("n_steps", C.c_int), ("time", C.c_double),
-                ("err_p", C.c_double), ("err_v", C.c_double), ("mag_p", C.c_double)]
+                ("err_p", C.c_double), ("err_v", C.c_double), ("mag_p", C.c_double),
+                ("precision", C.c_int)]
 
 
 class Problem(C.Structure):
@@ -115,6 +116,7 @@ SIGNATURES = [
     ("exw_server_poll", _I, [_VP, C.POINTER(Result), _I]),
     ("exw_server_status", _I, [_VP, _IP, _IP, _IP, C.POINTER(C.c_size_t)]),
     ("exw_solve", _I, [_VP, C.POINTER(Request), C.POINTER(Result)]),
+    ("exw_request_initial_error", _I, [C.POINTER(Request), _DP, _DP, _DP, _IP]),
     ("exw_lts_categorize", _I, [_I, _I, _DP, _IP, _D, _I, _I, _IP, _IP, _IP, _IP, _IP]),
     ("exw_lts_create", _I, [_VP, _D, _D, _I, _I, _D, _I, C.POINTER(_VP)]),
     ("exw_lts_destroy", _I, [_VP]),

@@ -49,8 +49,9 @@ struct DevMesh {
 };
 
 // o1 = b + a1*s*k, o2 = (o1 or c) + a2*s*k with s = cell_scale[cell] (1 if
-// cell_scale is NULL); cells with s == 0 are inactive and skipped entirely
-// (the multi-instance pools of the serving loop).
+// cell_scale is NULL); cells with s == 0 are inactive and skipped entirely.
+// With cell_list the launch covers the cells cell_list[0..n_list) only (the
+// active cells of a multi-instance pool of the serving loop; Cartesian kernel).
 struct Epilogue {
   const double *b;
   double *o1;
@@ -60,6 +61,8 @@ struct Epilogue {
   double a2;
   int o2_from_o1;
   const double *cell_scale;
+  const int *cell_list;
+  int n_list;
 };
 
 // Optional cell selection of the local-time-stepping sweeps: the kernel runs

@@ -40,20 +40,21 @@ struct CartShape {
   static constexpr int BLOCK = ((CPB * LANES + 63) / 64) * 64;
 };
 
-// 1D matrices of degree K (meaning as TabOff), passed by value so that every
-// lane reads them from scalar registers.
-template <int N>
+// 1D matrices of degree K (meaning as TabOff) in the operator's arithmetic
+// type, passed by value so that every lane reads them from scalar registers
+// (no per-lane conversion in the FP32 kernel).
+template <int N, typename T>
 struct CartTab {
-  double D[N * N];
-  double DT[N * N];
-  double L[2 * N];
-  double LW[2 * N];
+  T D[N * N];
+  T DT[N * N];
+  T L[2 * N];
+  T LW[2 * N];
 };
 
 template <int DIM, int K, typename Number>
 __global__ void __launch_bounds__((CartShape<DIM, K>::BLOCK))
     dg_operator_cart_kernel(const DevMesh mesh, const double *__restrict__ src,
-                            const CartTab<K + 1> tab, const Epilogue epi) {
+                            const CartTab<K + 1, Number> tab, const Epilogue epi) {
   using S = CartShape<DIM, K>;
   constexpr int N = S::N, ND = S::ND, NT = S::NT, NF = S::NF, LANES = S::LANES, CPB = S::CPB;
   constexpr int NFACE = 2 * DIM;
@@ -67,9 +68,10 @@ __global__ void __launch_bounds__((CartShape<DIM, K>::BLOCK))
   const int tid = threadIdx.x;
   const int lc = tid / LANES;
   const int lane = tid - lc * LANES;
-  const int cell = xcd_remap(blockIdx.x, gridDim.x) * CPB + lc;
+  const int e = xcd_remap(blockIdx.x, gridDim.x) * CPB + lc;
+  const int cell = epi.cell_list ? (e < epi.n_list ? epi.cell_list[e] : -1) : e;
   const bool lane_ok = lc < CPB;
-  bool active = lane_ok && cell < mesh.n_cells;
+  bool active = lane_ok && cell >= 0 && cell < mesh.n_cells;
   double cscale = 1.0;
   if (active && epi.cell_scale) {
     cscale = epi.cell_scale[cell];
@@ -162,8 +164,8 @@ __global__ void __launch_bounds__((CartShape<DIM, K>::BLOCK))
       Number gp = Number(0), dv = Number(0);
 #pragma unroll
       for (int m = 0; m < N; ++m) {
-        gp += Number(tab.D[i * N + m]) * pl[m];
-        dv += Number(tab.DT[i * N + m]) * vl[m];
+        gp += tab.D[i * N + m] * pl[m];
+        dv += tab.DT[i * N + m] * vl[m];
       }
       kv[i] = -rho_inv * gp * ih;
       kp[i] = rc2 * dv * ih;
@@ -173,7 +175,7 @@ __global__ void __launch_bounds__((CartShape<DIM, K>::BLOCK))
       Number vo = Number(0), po = Number(0), vn = Number(0), pn = Number(0);
 #pragma unroll
       for (int i = 0; i < N; ++i) {
-        const Number ls = Number(tab.L[s * N + i]), lo = Number(tab.L[(1 - s) * N + i]);
+        const Number ls = tab.L[s * N + i], lo = tab.L[(1 - s) * N + i];
         vo += ls * vl[i];
         po += ls * pl[i];
         vn += lo * (s ? vp[i] : vm[i]);
@@ -189,7 +191,7 @@ __global__ void __launch_bounds__((CartShape<DIM, K>::BLOCK))
       val_p *= ih;
 #pragma unroll
       for (int i = 0; i < N; ++i) {
-        const Number lw = Number(tab.LW[s * N + i]);
+        const Number lw = tab.LW[s * N + i];
         kv[i] += lw * val_pn;
         kp[i] += lw * val_p;
       }

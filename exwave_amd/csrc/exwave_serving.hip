@@ -67,9 +67,91 @@ int check_request(const exw_request &r) {
   if ((r.dim != 2 && r.dim != 3) || r.degree < 1 || r.degree > MAX_DEGREE ||
       r.n_refinements < 0 || r.n_refinements > 10 || !(r.rho > 0) || !(r.c > 0) ||
       !(r.final_time > 0) || !(r.cfl_number > 0) || r.modes.n_modes < 0 || r.modes.n_modes > 8 ||
-      (r.precision != EXW_FP64 && r.precision != EXW_FP32))
+      (r.precision != EXW_FP64 && r.precision != EXW_FP32 && r.precision != EXW_PRECISION_AUTO))
     return sfail(EXW_EINVAL, "invalid serving request");
+  for (int j = 0; j < r.modes.n_modes; ++j)
+    for (int d = 0; d < r.dim; ++d)
+      if (r.modes.m[j][d] < 0 || r.modes.m[j][d] > 64)
+        return sfail(EXW_EINVAL, "invalid serving request: mode numbers must be 0..64");
   return EXW_OK;
+}
+
+// ||p0 - I p0|| with QGauss(k+2) on the 2^r-cell unit hypercube, where p0 =
+// sum_j a_j prod_d sin(m_jd pi x_d) and I is the interpolant at the Gauss
+// points (project_initial_field in the collocated basis).  With tensor
+// quadrature, prod f - prod If = sum_d (prod_{e<d} If_e)(f_d - If_d)(prod_{e>d} f_e),
+// so the squared norm is a sum of products of 1D inner products, each of
+// which involves f - If itself: no cancellation between large terms.
+void initial_error(const exw_request &r, double &e0, double &norm0) {
+  const int dim = r.dim, k = r.degree, n = k + 1, ne = k + 2, nc = 1 << r.n_refinements;
+  const int M = r.modes.n_modes;
+  const double h = 1.0 / nc, pi = 3.14159265358979323846;
+  std::vector<double> xg, wg, xe, we;
+  gauss_legendre01(n, xg, wg);
+  gauss_legendre01(ne, xe, we);
+  std::vector<double> I((size_t)ne * n);
+  for (int q = 0; q < ne; ++q) lagrange_barycentric(xg, xe[q], &I[(size_t)q * n], nullptr);
+  const size_t L = (size_t)nc * ne;
+  // per mode j and direction d: f, If and f - If at every error point
+  std::vector<double> F((size_t)M * dim * L), G(F.size()), E(F.size());
+  std::vector<double> fg(n);
+  for (int j = 0; j < M; ++j)
+    for (int d = 0; d < dim; ++d) {
+      const double m = r.modes.m[j][d];
+      const size_t o = ((size_t)j * dim + d) * L;
+      for (int c = 0; c < nc; ++c) {
+        for (int i = 0; i < n; ++i) fg[i] = std::sin(m * pi * (c + xg[i]) * h);
+        for (int q = 0; q < ne; ++q) {
+          const double f = std::sin(m * pi * (c + xe[q]) * h);
+          double g = 0.0;
+          for (int i = 0; i < n; ++i) g += I[(size_t)q * n + i] * fg[i];
+          F[o + (size_t)c * ne + q] = f;
+          G[o + (size_t)c * ne + q] = g;
+          E[o + (size_t)c * ne + q] = f - g;
+        }
+      }
+    }
+  auto inner = [&](const double *a, const double *b) {
+    double s = 0.0;
+    for (int c = 0; c < nc; ++c)
+      for (int q = 0; q < ne; ++q) s += h * we[q] * a[(size_t)c * ne + q] * b[(size_t)c * ne + q];
+    return s;
+  };
+  // factor e of the telescoping term d of mode j
+  auto factor = [&](int j, int d, int e) {
+    const size_t o = ((size_t)j * dim + e) * L;
+    return e < d ? &G[o] : (e == d ? &E[o] : &F[o]);
+  };
+  double err2 = 0.0, nrm2 = 0.0;
+  for (int j = 0; j < M; ++j)
+    for (int l = 0; l < M; ++l) {
+      const double a = r.modes.amp[j] * r.modes.amp[l];
+      double pf = 1.0;
+      for (int e = 0; e < dim; ++e)
+        pf *= inner(&F[((size_t)j * dim + e) * L], &F[((size_t)l * dim + e) * L]);
+      nrm2 += a * pf;
+      for (int d = 0; d < dim; ++d)
+        for (int d2 = 0; d2 < dim; ++d2) {
+          double t = 1.0;
+          for (int e = 0; e < dim; ++e) t *= inner(factor(j, d, e), factor(l, d2, e));
+          err2 += a * t;
+        }
+    }
+  e0 = std::sqrt(std::max(err2, 0.0));
+  norm0 = std::sqrt(std::max(nrm2, 0.0));
+}
+
+// EXW_PRECISION_AUTO: FP32 when the FP32 score clears EXW_FP32_SCORE_MIN
+int auto_precision(const exw_request &r, int n_steps, double *e0_out, double *norm_out,
+                   double *score_out) {
+  double e0, norm0;
+  initial_error(r, e0, norm0);
+  const double floor32 = std::ldexp(1.0, -24) * std::sqrt((double)n_steps) * norm0;
+  const double score = floor32 > 0 ? e0 / floor32 : 0.0;
+  if (e0_out) *e0_out = e0;
+  if (norm_out) *norm_out = norm0;
+  if (score_out) *score_out = score;
+  return score >= EXW_FP32_SCORE_MIN ? EXW_FP32 : EXW_FP64;
 }
 
 DevModes modes_of(const exw_request &r) {
@@ -114,6 +196,15 @@ struct Pool {
   bool graph_ok = false;
   std::vector<Instance *> active;
   int finishing = 0;  // solves whose error evaluation has not been polled yet
+  // the launches cover the active cells only: d_list holds them in ascending
+  // order, uploaded in stream order from a ring of pinned staging buffers
+  static constexpr int RING = 4;
+  int *d_list = nullptr;
+  int list_cap = 0, n_list = 0;
+  bool list_dirty = true;
+  int *ring[RING] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t ring_ev[RING] = {nullptr, nullptr, nullptr, nullptr};
+  int ring_cap = 0, ring_next = 0;
   std::deque<hipEvent_t> inflight;
   std::vector<hipEvent_t> event_pool;
 
@@ -171,6 +262,54 @@ void drop_graphs(Pool *p) {
     if (p->exec[i]) hipGraphExecDestroy(p->exec[i]);
   p->exec[0] = p->exec[1] = nullptr;
   p->graph_ok = false;
+}
+
+// rebuild the list of active cells (ascending) and upload it in stream order
+int update_list(Pool *p) {
+  std::vector<std::pair<int, int>> ranges;
+  int n = 0;
+  for (Instance *in : p->active) {
+    ranges.emplace_back(in->start, in->n_cells);
+    n += in->n_cells;
+  }
+  std::sort(ranges.begin(), ranges.end());
+  if (n > p->list_cap) {
+    // the in-flight steps read the old list
+    SHIP(hipStreamSynchronize(p->stream));
+    hipFree(p->d_list);
+    p->d_list = nullptr;
+    p->list_cap = std::max(n, 2 * p->list_cap);
+    if (hipMalloc((void **)&p->d_list, sizeof(int) * p->list_cap) != hipSuccess)
+      return sfail(EXW_ENOMEM, "serving: cell list allocation failed");
+  }
+  if (n > p->ring_cap) {
+    for (int i = 0; i < Pool::RING; ++i) {
+      if (p->ring_ev[i]) SHIP(hipEventSynchronize(p->ring_ev[i]));
+      if (p->ring[i]) hipHostFree(p->ring[i]);
+      p->ring[i] = nullptr;
+    }
+    p->ring_cap = std::max(n, 2 * p->ring_cap);
+    for (int i = 0; i < Pool::RING; ++i) {
+      if (hipHostMalloc((void **)&p->ring[i], sizeof(int) * p->ring_cap, hipHostMallocDefault) !=
+          hipSuccess)
+        return sfail(EXW_ENOMEM, "serving: pinned staging allocation failed");
+      if (!p->ring_ev[i]) SHIP(hipEventCreateWithFlags(&p->ring_ev[i], hipEventDisableTiming));
+    }
+  }
+  const int slot = p->ring_next;
+  p->ring_next = (slot + 1) % Pool::RING;
+  SHIP(hipEventSynchronize(p->ring_ev[slot]));  // the slot's previous upload has been read
+  int *h = p->ring[slot], k = 0;
+  for (auto &r : ranges)
+    for (int c = 0; c < r.second; ++c) h[k++] = r.first + c;
+  if (n > 0) {
+    SHIP(hipMemcpyAsync(p->d_list, h, sizeof(int) * n, hipMemcpyHostToDevice, p->stream));
+  }
+  SHIP(hipEventRecord(p->ring_ev[slot], p->stream));
+  p->n_list = n;
+  p->list_dirty = false;
+  drop_graphs(p);
+  return EXW_OK;
 }
 
 // grow the pool to new_cap cells (copies the live contents)
@@ -243,6 +382,7 @@ int grow_pool(exw_server s, Pool *p, int new_cap) {
   s->used += extra;
   p->bytes += extra;
   p->cap = new_cap;
+  p->list_dirty = true;
   drop_graphs(p);
   return EXW_OK;
 }
@@ -316,7 +456,8 @@ int enqueue_step_launches(Pool *p, int from) {
       Lsrk45::S, Lsrk45::a, Lsrk45::b, 1.0, p->reg[from], p->reg[1 - from], p->reg[2],
       [&](const double *src, const double *acc, double *y_out, double *b_out, double f1,
           double f2, bool last) {
-        Epilogue e{acc, y_out, f1, nullptr, last ? nullptr : b_out, f2, 1, p->scale};
+        Epilogue e{acc,     y_out,   f1, nullptr, last ? nullptr : b_out, f2, 1, p->scale,
+                   p->d_list, p->n_list};
         return l(m, src, p->d_tab, host_tab, e, p->stream) == hipSuccess ? 0 : EXW_EDEVICE;
       });
   if (rc) return sfail(EXW_EDEVICE, "serving step launch failed");
@@ -393,6 +534,7 @@ int admit(exw_server s, Instance *in) {
   SHIP(dim_kernels(dim).project[r.degree](p->view(start, n), p->d_tab, modes_of(r), 0.0,
                                            p->reg[p->parity] + (size_t)start * p->dpc, st));
   p->active.push_back(in);
+  p->list_dirty = true;
   return EXW_OK;
 }
 
@@ -417,6 +559,7 @@ int finalize(exw_server s, Pool *p, Instance *in) {
   SHIP(hipEventCreateWithFlags(&in->done, hipEventDisableTiming));
   SHIP(hipEventRecord(in->done, st));
   release_range(p, in->start, in->n_cells);
+  p->list_dirty = true;
   in->pool = p;
   ++p->finishing;
   s->finishing.push_back(in);
@@ -435,6 +578,10 @@ int release_idle_pools(exw_server s) {
     }
     drop_graphs(p);
     free_pool_arrays(p);
+    hipFree(p->d_list);
+    p->d_list = nullptr;
+    p->list_cap = 0;
+    p->list_dirty = true;
     s->used -= p->bytes;
     p->bytes = 0;
     p->cap = 0;
@@ -476,6 +623,11 @@ int exw_server_destroy(exw_server s) {
     hipFree(p->d_etab);
     for (auto e : p->event_pool) hipEventDestroy(e);
     for (auto e : p->inflight) hipEventDestroy(e);
+    hipFree(p->d_list);
+    for (int i = 0; i < Pool::RING; ++i) {
+      if (p->ring[i]) hipHostFree(p->ring[i]);
+      if (p->ring_ev[i]) hipEventDestroy(p->ring_ev[i]);
+    }
     for (auto *in : p->active) delete in;
     hipStreamDestroy(p->stream);
     delete p;
@@ -498,6 +650,8 @@ int exw_server_submit(exw_server s, const exw_request *r, int64_t *id) {
   in->id = s->next_id++;
   in->req = *r;
   in->n_steps = solve_steps(*r, in->dt, in->t_end);
+  if (r->precision == EXW_PRECISION_AUTO)
+    in->req.precision = auto_precision(*r, in->n_steps, nullptr, nullptr, nullptr);
   int nc = 0;
   exw_mesh_hypercube(r->dim, 1 << r->n_refinements, 0, EXW_TRANSFORM_NONE, 0.0, 2, &nc, nullptr,
                      nullptr);
@@ -546,6 +700,8 @@ int exw_server_progress(exw_server s, int max_steps) {
         if (!oldest) oldest = p;
         continue;
       }
+      if (p->list_dirty)
+        if (int rc = update_list(p)) return rc;
       if (!p->graph_ok)
         if (int rc = capture_graphs(p)) return rc;
       SHIP(hipGraphLaunch(p->exec[p->parity], p->stream));
@@ -604,6 +760,7 @@ int exw_server_poll(exw_server s, exw_result *out, int max_results) {
       r.err_p = std::sqrt(in->h_sum[0]);
       r.err_v = std::sqrt(in->h_sum[1]);
       r.mag_p = std::sqrt(in->h_sum[2]);
+      r.precision = in->req.precision;
       hipFree(in->d_partial);
       hipHostFree(in->h_sum);
       hipEventDestroy(in->done);
@@ -655,10 +812,13 @@ int exw_solve(exw_ctx ctx, const exw_request *r, exw_result *out) {
   double *sol = buf, *tmp = buf + N, *work = buf + 2 * N;
   double dt, t_end;
   const int steps = solve_steps(*r, dt, t_end);
+  const int prec = r->precision == EXW_PRECISION_AUTO
+                       ? auto_precision(*r, steps, nullptr, nullptr, nullptr)
+                       : r->precision;
   rc = exw_project(op, &r->modes, 0.0, sol);
   for (int s = 0; s < steps && !rc; ++s) {
     std::swap(sol, tmp);
-    rc = exw_time_step(op, EXW_LSRK45R2, dt, tmp, sol, work, 0, r->precision);
+    rc = exw_time_step(op, EXW_LSRK45R2, dt, tmp, sol, work, 0, prec);
   }
   double err[3] = {0, 0, 0};
   if (!rc) rc = exw_l2_errors(op, &r->modes, t_end, sol, err);
@@ -671,6 +831,18 @@ int exw_solve(exw_ctx ctx, const exw_request *r, exw_result *out) {
   out->err_p = err[0];
   out->err_v = err[1];
   out->mag_p = err[2];
+  out->precision = prec;
+  return EXW_OK;
+}
+
+int exw_request_initial_error(const exw_request *r, double *e0, double *norm_p0, double *score,
+                              int *precision) {
+  if (!r) return sfail(EXW_EINVAL, "exw_request_initial_error: null request");
+  if (int rc = check_request(*r)) return rc;
+  double dt, t_end;
+  const int steps = solve_steps(*r, dt, t_end);
+  const int p = auto_precision(*r, steps, e0, norm_p0, score);
+  if (precision) *precision = p;
   return EXW_OK;
 }
 

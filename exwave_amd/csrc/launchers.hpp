@@ -47,21 +47,23 @@ hipError_t launch_op(const DevMesh &m, const double *src, const double *tab,
     using CS = CartShape<DIM, K>;
     constexpr int N = K + 1;
     using TO = TabOff<N>;
-    CartTab<N> ct;
+    CartTab<N, Number> ct;
     for (int i = 0; i < N * N; ++i) {
-      ct.D[i] = host_tab[TO::D + i];
-      ct.DT[i] = host_tab[TO::DT + i];
+      ct.D[i] = Number(host_tab[TO::D + i]);
+      ct.DT[i] = Number(host_tab[TO::DT + i]);
     }
     for (int i = 0; i < 2 * N; ++i) {
-      ct.L[i] = host_tab[TO::L + i];
-      ct.LW[i] = host_tab[TO::LW + i];
+      ct.L[i] = Number(host_tab[TO::L + i]);
+      ct.LW[i] = Number(host_tab[TO::LW + i]);
     }
-    const int grid = (m.n_cells + CS::CPB - 1) / CS::CPB;
+    const int n = e.cell_list ? e.n_list : m.n_cells;
+    const int grid = (n + CS::CPB - 1) / CS::CPB;
     if (grid == 0) return hipSuccess;
     hipLaunchKernelGGL((dg_operator_cart_kernel<DIM, K, Number>), dim3(grid), dim3(CS::BLOCK), 0,
                        s, m, src, ct, e);
   } else {
     using S = Shape<DIM, K>;
+    if (e.cell_list) return hipErrorInvalidValue;  // cell lists: Cartesian kernel only
     const int grid = (m.n_cells + S::CPB - 1) / S::CPB;
     if (grid == 0) return hipSuccess;
     hipLaunchKernelGGL((dg_operator_kernel<DIM, K, Number, CART>), dim3(grid), dim3(S::BLOCK), 0,

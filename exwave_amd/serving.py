@@ -11,7 +11,7 @@ from __future__ import annotations
 import ctypes as C
 import time
 
-from .capi import FP32, FP64, Modes, Request, Result, check, load
+from .capi import FP32, FP64, PRECISION_AUTO, Modes, Request, Result, check, load
 
 
 def make_request(q, precision=FP64):
@@ -59,10 +59,20 @@ class Server:
 
 
 def _res(r):
-    return dict(n_steps=r.n_steps, time=r.time, err_p=r.err_p, err_v=r.err_v, mag_p=r.mag_p)
+    return dict(n_steps=r.n_steps, time=r.time, err_p=r.err_p, err_v=r.err_v, mag_p=r.mag_p,
+                precision=r.precision)
 
 
-def run_server(ctx, reqs, precision_of=lambda q: FP32, hbm_budget_bytes=64 << 30,
+def initial_error(q):
+    """exw_request_initial_error: (E0, ||p0||, FP32 score, selected precision)."""
+    r = make_request(q, PRECISION_AUTO)
+    e0, n0, sc, pr = C.c_double(), C.c_double(), C.c_double(), C.c_int()
+    check(load().exw_request_initial_error(C.byref(r), C.byref(e0), C.byref(n0), C.byref(sc),
+                                           C.byref(pr)))
+    return e0.value, n0.value, sc.value, pr.value
+
+
+def run_server(ctx, reqs, precision_of=lambda q: PRECISION_AUTO, hbm_budget_bytes=64 << 30,
                realtime=True, steps_per_call=4, stall_timeout=120.0):
     """Returns ({request id: result dict with 'latency'}, makespan seconds)."""
     srv = Server(ctx, hbm_budget_bytes)

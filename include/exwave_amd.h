@@ -252,14 +252,24 @@ typedef struct exw_request {
   double rho, c;
   exw_modes modes;
   double final_time, cfl_number;
-  int precision; /* EXW_FP64 / EXW_FP32 operator */
+  int precision; /* EXW_FP64 / EXW_FP32 operator, or EXW_PRECISION_AUTO */
 } exw_request;
+
+/* EXW_PRECISION_AUTO: the FP32 operator is used when the solve's own
+ * discretisation error is far above what FP32 round-off adds over its steps:
+ *   E0 / (2^-24 sqrt(n_steps) ||p0||) >= EXW_FP32_SCORE_MIN,
+ * with E0 = ||p0 - I p0|| the error of the initial Gauss-point interpolant
+ * (exw_request_initial_error) and ||p0|| the norm of the initial pressure;
+ * otherwise FP64.  The result then stays within 1.1x of the FP64 error. */
+#define EXW_PRECISION_AUTO 2
+#define EXW_FP32_SCORE_MIN 10.0
 
 typedef struct exw_result {
   int64_t id;
   int n_steps;
   double time;                 /* final time reached (>= final_time) */
   double err_p, err_v, mag_p;  /* L2 errors against the exact modes, QGauss(k+2) */
+  int precision;               /* operator arithmetic the solve ran with */
 } exw_result;
 
 typedef struct exw_server_s *exw_server;
@@ -287,6 +297,11 @@ int exw_server_status(exw_server s, int *n_queued, int *n_active, int *n_finishi
 /* The same solve alone, one launch sequence per step (the one-at-a-time path
  * the serving loop is compared with).  Synchronises. */
 int exw_solve(exw_ctx ctx, const exw_request *r, exw_result *out);
+/* host only: E0 = ||p0 - I p0|| (QGauss(k+2)) and ||p0|| of a request, in
+ * closed form (separable modes, no device work); *score = the FP32 score of
+ * EXW_PRECISION_AUTO and *precision the operator it selects. */
+int exw_request_initial_error(const exw_request *r, double *e0, double *norm_p0, double *score,
+                              int *precision);
 
 #ifdef __cplusplus
 }

@@ -64,3 +64,30 @@ def test_lts_categorize_matches_oracle(dim, nint, f, k, maxc, maxd):
     assert nc == info["n_clusters"] and diff == info["cluster_diff"]
     assert it == info["iterations"]
     assert np.array_equal(cl, info["cluster"]) and np.array_equal(fl, info["flags"])
+
+
+@pytest.mark.parametrize("dim,k,r,modes,amps", [(2, 3, 3, [[2, 1]], [1.0]), (3, 2, 2, [[1, 1, 2]], [1.0]),
+                                                (2, 6, 4, [[3, 4]], [1.0]),
+                                                (2, 4, 3, [[1, 1], [2, 3]], [1.0, -0.5]),
+                                                (3, 5, 2, [[1, 2, 1], [2, 1, 3]], [0.7, 0.3])])
+def test_request_initial_error_matches_oracle(dim, k, r, modes, amps):
+    """The closed-form initial interpolation error and norm used by the
+    automatic precision choice equal the oracle's L2 error of the projected
+    initial field at t = 0 (QGauss(k+2))."""
+    import ctypes as C
+    import exwave_amd.capi as capi
+    req = capi.Request()
+    req.dim, req.degree, req.n_refinements = dim, k, r
+    req.rho = req.c = 1.0
+    req.modes = capi.Modes.make(dim, modes, amps)
+    req.final_time, req.cfl_number, req.precision = 0.5, 0.2, capi.PRECISION_AUTO
+    e0, n0, sc, pr = C.c_double(), C.c_double(), C.c_double(), C.c_int()
+    capi.check(capi.load().exw_request_initial_error(C.byref(req), C.byref(e0), C.byref(n0),
+                                                     C.byref(sc), C.byref(pr)))
+    m = O.Mesh(dim, 1 << r, 0, O.TRANSFORM_NONE, 0.0, 2)
+    e = O.Exact.make(dim, modes, amps)
+    u = m.project(k, e, 0.0)
+    err = m.l2_errors(k, e, 0.0, u)
+    assert abs(e0.value - err[0]) <= 1e-9 * err[0] + 1e-15, (e0.value, err[0])
+    assert abs(n0.value - err[2]) <= 1e-12 * err[2]
+    assert pr.value == (capi.FP32 if sc.value >= 10.0 else capi.FP64)
