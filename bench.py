@@ -14,9 +14,22 @@ LSRK45 time step = 5 fused operator+update sweeps.
 Prints one JSON line (see the keys below).  `roofline` is computed for the
 fused stage kernel from algorithmic bytes (28.8 B/DoF per stage on average:
 18 FP64 vector passes per LSRK45 step, plus 64 B of per-cell data) over its
-average launch time measured with HIP events on the launch stream.
-`cpu_baseline` times the CPU oracle (a C/OpenMP restatement, not deal.II) on
-a bounded sample of the same workload.
+average launch time measured with HIP events on the launch stream; `traffic`
+is the measured HBM bytes per launch from the committed rocprofv3 counter
+summary of this workload (profiles/), when present.  `cpu_baseline` times the
+CPU oracle (a C/OpenMP restatement, not deal.II) on a bounded sample of the
+same workload.
+
+`serving` (unless --no-serving): the 512-request serving stream (seeded: dim
+2/3, k 2..6, refinements 3..6 / 2..4, rho and c log-uniform on [0.5, 2],
+modes 1..4, T in [0.5, 2], Poisson arrivals at 200/s), replayed in real time
+through the continuously batched serving loop with the automatic FP32/FP64
+choice, against the plain FP64 one-solve-at-a-time path (exw_solve per
+request) in the same run: solves/s, p50/p99 latency (completion - arrival),
+and how many results stay within 1.1x of the one-at-a-time FP64 L2 errors.
+
+    python bench.py --copy-calibration    # plain copy kernel of known size,
+                                          # for scaling FETCH_SIZE/WRITE_SIZE
 """
 from __future__ import annotations
 
@@ -45,7 +58,122 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target CPU time of the oracle sample")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serving-cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--copy-calibration", action="store_true")
     return ap.parse_args()
+
+
+TRAFFIC_FILE = os.path.join(HERE, "profiles", "dg_operator_cart_3d_k4_64cube_counters.md")
+
+
+def measured_traffic(kernel_key):
+    """HBM bytes per launch of `kernel_key` from the committed counter summary
+    (a line 'traffic_bytes_per_launch <kernel_key> <bytes>')."""
+    try:
+        for line in open(TRAFFIC_FILE):
+            f = line.split()
+            if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel_key:
+                return int(float(f[2]))
+    except OSError:
+        pass
+    return None
+
+
+def copy_calibration():
+    """A plain device copy of 2^27 doubles (1 GiB read, 1 GiB write), five
+    times, for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = 1 << 27
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration": {"bytes_read": 8 * n, "bytes_written": 8 * n,
+                                           "launches": 5}}), flush=True)
+
+
+def serving_cpu_baseline(reqs, target_s):
+    """The CPU oracle solving requests of the same stream (in arrival order,
+    LSRK45, FP64) for about target_s; the stream's whole work divided by the
+    measured DoF-stage rate gives the CPU solves/s."""
+    import numpy as np
+
+    import oracle as O
+    from exwave_amd import serving_stream as SS
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    done_work, solved, steps_done = 0.0, 0, 0
+    for q in reqs:
+        m = O.Mesh(q["dim"], 1 << q["n_refinements"], 0, O.TRANSFORM_NONE, 0.0, 2)
+        m.set_material(np.full(m.n_cells, q["rho"]), np.full(m.n_cells, q["c"]))
+        e = O.Exact.make(q["dim"], [q["modes"]], rho=q["rho"], c=q["c"])
+        u = m.project(q["degree"], e, 0.0)
+        h = 1.0 / (1 << q["n_refinements"])
+        dt = q["cfl"] / q["degree"] ** 1.5 * h / q["c"]
+        n = SS.n_steps(q)
+        for s in range(n):
+            vnp = np.zeros_like(u)
+            m.time_step(q["degree"], O.LSRK45R2, dt, u, vnp)
+            u = vnp
+            done_work += 5 * SS.n_dofs(q)
+            steps_done += 1
+            if time.perf_counter() - t0 > target_s:
+                break
+        else:
+            solved += 1
+            continue
+        break
+    el = time.perf_counter() - t0
+    rate = done_work / el
+    total = sum(5.0 * SS.n_dofs(q) * SS.n_steps(q) for q in reqs)
+    return {"solves_per_s": round(len(reqs) / (total / rate), 4), "unit": "solves/s",
+            "cores": threads, "kind": "port",
+            "sample": f"oracle LSRK45 FP64 on the stream's first requests in arrival order: "
+                      f"{solved} solved + {steps_done} steps in {el:.1f} s = "
+                      f"{rate / 1e9:.3f} GDoF-stages/s, extrapolated to the stream's "
+                      f"{total:.3e} DoF-stages; OpenMP {threads} threads; CPU restatement, "
+                      "not deal.II"}
+
+
+def serving_measure(ctx, cpu_seconds, want_cpu):
+    import numpy as np
+
+    import exwave_amd as X
+    from exwave_amd import serving as S
+    from exwave_amd import serving_stream as SS
+    reqs = SS.make_stream()
+    # warm-up: one tiny solve per (dim, degree) pool and precision (builds graphs,
+    # loads kernels), outside the measurement
+    warm = [dict(id=10000 + 10 * d + k, arrival=0.0, dim=d, degree=k, n_refinements=2, rho=1.0,
+                 c=1.0, modes=[1] * d, final_time=0.02, cfl=0.2)
+            for d in (2, 3) for k in range(2, 7)]
+    S.run_one_at_a_time(ctx, warm, X.FP64)
+    S.run_server(ctx, warm, lambda q: X.FP64, realtime=False)
+    S.run_server(ctx, warm, lambda q: X.FP32, realtime=False)
+    solo, t_solo = S.run_one_at_a_time(ctx, reqs, X.FP64)
+    res, makespan = S.run_server(ctx, reqs, lambda q: X.PRECISION_AUTO, realtime=True)
+    lat = np.array([res[q["id"]]["latency"] for q in reqs])
+    within = sum(res[q["id"]]["err_p"] <= 1.1 * solo[q["id"]]["err_p"] and
+                 res[q["id"]]["err_v"] <= 1.1 * solo[q["id"]]["err_v"] for q in reqs)
+    worst = max(max(res[q["id"]]["err_p"] / solo[q["id"]]["err_p"],
+                    res[q["id"]]["err_v"] / solo[q["id"]]["err_v"]) for q in reqs)
+    n32 = sum(res[q["id"]]["precision"] == X.FP32 for q in reqs)
+    out = {"requests": len(reqs), "arrival_rate_per_s": 200.0,
+           "solves_per_s": round(len(reqs) / makespan, 3), "makespan_s": round(makespan, 3),
+           "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
+           "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
+           "one_at_a_time_fp64_solves_per_s": round(len(reqs) / t_solo, 3),
+           "one_at_a_time_fp64_s": round(t_solo, 3),
+           "speedup_vs_one_at_a_time": round(t_solo / makespan, 3),
+           "within_1p1x_fp64_error": f"{within}/{len(reqs)}",
+           "worst_error_ratio": round(worst, 4), "fp32_solves": n32,
+           "precision": "auto (FP32 operator where its round-off stays far below the "
+                        "discretisation error, else FP64; vectors and RK in FP64)"}
+    if want_cpu:
+        out["cpu_baseline"] = serving_cpu_baseline(reqs, cpu_seconds)
+    return out
 
 
 def cpu_baseline(n1d, k, target_s):
@@ -95,6 +223,9 @@ def cpu_baseline(n1d, k, target_s):
 
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); the "
               "multi-GPU sharding is a design note in DESIGN.md", file=sys.stderr)
@@ -165,17 +296,23 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
                      "traffic": None,
-                     "kernel": f"dg_operator_kernel<3,{k},{'double' if prec == X.FP64 else 'float'},cartesian>",
+                     "kernel": f"dg_operator_cart_kernel<3,{k},{'double' if prec == X.FP64 else 'float'}>",
                      "algorithmic_bytes_per_launch": int(bytes_per_launch),
                      "avg_launch_ms": round(stage_ms, 5)},
         "wall_s": round(wall, 4),
         "l2_error_after": {"t": t_end, "p": err[0], "v": err[1], "mag_p": err[2]},
         "energy_ratio": E1 / E0,
     }
+    key = f"3d_k{k}_n{n1d}_{'fp64' if prec == X.FP64 else 'fp32'}"
+    out["roofline"]["traffic"] = measured_traffic(key)
+    if out["roofline"]["traffic"]:
+        out["roofline"]["traffic_source"] = os.path.relpath(TRAFFIC_FILE, HERE)
+    del vec_n, vec_np, work
+    torch.cuda.empty_cache()
     if not args.no_cpu_baseline:
-        del vec_n, vec_np, work
-        torch.cuda.empty_cache()
         out["cpu_baseline"] = cpu_baseline(n1d, k, args.cpu_seconds)
+    if not args.no_serving:
+        out["serving"] = serving_measure(ctx, args.serving_cpu_seconds, not args.no_cpu_baseline)
     print(json.dumps(out), flush=True)
 
 
