@@ -61,6 +61,9 @@ def parse():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serving-cpu-seconds", type=float, default=10.0)
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--nonaffine", action="store_true",
+                    help="non-affine 32^3/48^3, k=4/6, FP32 vs FP64 operator (errors + GDoF/s)")
+    ap.add_argument("--nonaffine-steps", type=int, default=40)
     return ap.parse_args()
 
 
@@ -92,6 +95,84 @@ def copy_calibration():
     torch.cuda.synchronize()
     print(json.dumps({"copy_calibration": {"bytes_read": 8 * n, "bytes_written": 8 * n,
                                            "launches": 5}}), flush=True)
+
+
+def nonaffine_measure(steps):
+    """Non-affine 3D membranes at 32^3 and 48^3 cells, k = 4 and 6: LSRK45 with
+    the FP64 operator and with the FP32 operator (FP64 vectors, RK updates and
+    norms) from the same projected initial field.
+
+    map "axis": x_i += 0.05 prod_{e != i} sin(2 pi x_e) (the literal node map;
+      it moves the boundary, so no closed-form solution): FP32 vs FP64 relative
+      L2 difference of the final states, and for the 32^3 k=4 case the FP64 and
+      FP32 states against the CPU oracle after 2 steps;
+    map "product": x_i += 0.05 prod_e sin(2 pi x_e) (keeps the unit cube):
+      L2 errors of both against the analytical membrane solution.
+    GDoF/s per RK stage of the general-geometry fused kernel for each."""
+    import numpy as np
+    import torch
+
+    import exwave_amd as X
+    ctx = X.Context(0)
+    stream = torch.cuda.current_stream()
+    modes = X.Modes.make(DIM, [[1, 1, 1], [2, 3, 1]])
+    out = []
+    for k in (4, 6):
+        for n1d in (32, 48):
+            for tname, tr in (("axis", X.TRANSFORM_AXIS_2PI), ("product", X.TRANSFORM_PRODUCT_2PI)):
+                op = X.WaveOperator.hypercube(ctx, DIM, k, n1d, 0, tr, 0.05, 2)
+                dt = 0.2 / k ** 1.5 * op.min_vertex_distance_over_c
+                rec = {"degree": k, "cells_per_dir": n1d, "map": tname, "n_dofs": op.n_dofs,
+                       "steps": steps, "dt": dt}
+                finals = {}
+                for pname, prec in (("fp64", X.FP64), ("fp32", X.FP32)):
+                    a, b, w = op.new_vector(), op.new_vector(), op.new_vector()
+                    op.project(modes, 0.0, b)
+                    ev0 = torch.cuda.Event(enable_timing=True)
+                    ev1 = torch.cuda.Event(enable_timing=True)
+                    ev0.record(stream)
+                    for _ in range(steps):
+                        a, b = b, a
+                        op.time_step(X.LSRK45R2, dt, a, b, w, precision=prec)
+                    ev1.record(stream)
+                    torch.cuda.synchronize()
+                    ms = ev0.elapsed_time(ev1)
+                    rec[f"gdofs_{pname}"] = round(op.n_dofs * 5 * steps / (ms / 1e3) / 1e9, 2)
+                    if tname == "product":
+                        e = op.l2_errors(modes, steps * dt, b)
+                        rec[f"err_p_{pname}"], rec[f"err_v_{pname}"] = e[0], e[1]
+                    finals[pname] = b
+                    del a, w
+                d = finals["fp32"] - finals["fp64"]
+                rec["fp32_vs_fp64_rel_l2"] = float(torch.linalg.vector_norm(d) /
+                                                   torch.linalg.vector_norm(finals["fp64"]))
+                del finals, d
+                torch.cuda.empty_cache()
+                out.append(rec)
+                print(json.dumps(rec), file=sys.stderr, flush=True)
+    # oracle cross-check of the literal map at 32^3, k=4 (two steps)
+    import oracle as O
+    k, n1d, st = 4, 32, 2
+    op = X.WaveOperator.hypercube(ctx, DIM, k, n1d, 0, X.TRANSFORM_AXIS_2PI, 0.05, 2)
+    m = O.Mesh(DIM, n1d, 0, O.TRANSFORM_AXIS_2PI, 0.05, 2)
+    dt = 0.2 / k ** 1.5 * m.min_vertex_distance_over_c
+    e = O.Exact.make(DIM, [[1, 1, 1], [2, 3, 1]])
+    u = m.project(k, e, 0.0)
+    for _ in range(st):
+        vnp = np.zeros_like(u)
+        m.time_step(k, O.LSRK45R2, dt, u, vnp)
+        u = vnp
+    chk = {"case": "oracle check, map axis, 32^3, k=4, 2 steps"}
+    for pname, prec in (("fp64", X.FP64), ("fp32", X.FP32)):
+        a, b, w = op.new_vector(), op.new_vector(), op.new_vector()
+        op.project(modes, 0.0, b)
+        for _ in range(st):
+            a, b = b, a
+            op.time_step(X.LSRK45R2, dt, a, b, w, precision=prec)
+        g = b.cpu().numpy()
+        chk[f"{pname}_rel_max_vs_oracle"] = float(np.max(np.abs(g - u)) / np.max(np.abs(u)))
+        chk[f"{pname}_rel_l2_vs_oracle"] = float(np.linalg.norm(g - u) / np.linalg.norm(u))
+    print(json.dumps({"nonaffine": out, "oracle_check": chk}), flush=True)
 
 
 def serving_cpu_baseline(reqs, target_s):
@@ -225,6 +306,9 @@ def main():
     args = parse()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.nonaffine:
+        nonaffine_measure(args.nonaffine_steps)
         return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); the "
