@@ -4,81 +4,23 @@
 #include <vector>
 
 #include "dg_ader.hpp"
+#include "gauss_cx.hpp"
 
 namespace exw {
 
-// n-point Gauss-Legendre rule on [0,1] (Newton iteration on P_n; ascending).
+// The 1D rules are computed by constexpr code so that the same arithmetic
+// yields the host tables (build_tables) and the compile-time tables of the
+// Cartesian kernel (CartConst in gauss_cx.hpp).
 inline void gauss_legendre01(int n, std::vector<double> &x, std::vector<double> &w) {
   x.assign(n, 0.0);
   w.assign(n, 0.0);
-  for (int i = 0; i < n; ++i) {
-    // Chebyshev-like initial guess for the i-th root in descending order
-    double z = std::cos(M_PI * (i + 0.75) / (n + 0.5)), dp = 1.0;
-    for (int it = 0; it < 200; ++it) {
-      double p0 = 1.0, p1 = z;  // P_0, P_1
-      for (int j = 2; j <= n; ++j) {
-        const double p2 = ((2.0 * j - 1.0) * z * p1 - (j - 1.0) * p0) / j;
-        p0 = p1;
-        p1 = p2;
-      }
-      const double pn = n == 0 ? 1.0 : (n == 1 ? z : p1);
-      const double pnm1 = n == 1 ? 1.0 : p0;
-      dp = n * (z * pn - pnm1) / (z * z - 1.0);
-      const double dz = pn / dp;
-      z -= dz;
-      if (std::fabs(dz) < 1e-17) break;
-    }
-    {
-      double p0 = 1.0, p1 = z;
-      for (int j = 2; j <= n; ++j) {
-        const double p2 = ((2.0 * j - 1.0) * z * p1 - (j - 1.0) * p0) / j;
-        p0 = p1;
-        p1 = p2;
-      }
-      const double pn = n == 1 ? z : p1, pnm1 = n == 1 ? 1.0 : p0;
-      dp = n * (z * pn - pnm1) / (z * z - 1.0);
-    }
-    x[n - 1 - i] = 0.5 * (1.0 + z);
-    w[n - 1 - i] = 1.0 / ((1.0 - z * z) * dp * dp);  // 2/((1-z^2)P'^2) scaled by 1/2
-  }
+  cx::gauss01(n, x.data(), w.data());
 }
 
 // Barycentric Lagrange basis on `nodes`: values l_i(x) and derivatives l_i'(x).
 inline void lagrange_barycentric(const std::vector<double> &nodes, double x, double *val,
                                  double *der) {
-  const int n = (int)nodes.size();
-  std::vector<double> bw(n, 1.0);
-  for (int i = 0; i < n; ++i)
-    for (int j = 0; j < n; ++j)
-      if (j != i) bw[i] /= (nodes[i] - nodes[j]);
-  int hit = -1;
-  for (int i = 0; i < n; ++i)
-    if (x == nodes[i]) hit = i;
-  if (hit >= 0) {
-    // at a node: l_i = delta, l_i'(x_k) = bw_i/bw_k/(x_k-x_i), l_k'(x_k) = -sum_{i!=k}
-    for (int i = 0; i < n; ++i) {
-      if (val) val[i] = i == hit ? 1.0 : 0.0;
-      if (der && i != hit) der[i] = bw[i] / bw[hit] / (x - nodes[i]);
-    }
-    if (der) {
-      double s = 0.0;
-      for (int i = 0; i < n; ++i)
-        if (i != hit) s += der[i];
-      der[hit] = -s;
-    }
-    return;
-  }
-  double s = 0.0, sd = 0.0;
-  for (int i = 0; i < n; ++i) {
-    s += bw[i] / (x - nodes[i]);
-    sd += bw[i] / ((x - nodes[i]) * (x - nodes[i]));
-  }
-  for (int i = 0; i < n; ++i) {
-    const double li = bw[i] / (x - nodes[i]) / s;
-    if (val) val[i] = li;
-    // d/dx [ (bw_i/(x-x_i)) / s ] = li * ( -1/(x-x_i) + sd/s )
-    if (der) der[i] = li * (-1.0 / (x - nodes[i]) + sd / s);
-  }
+  cx::lagrange(int(nodes.size()), nodes.data(), x, val, der);
 }
 
 

@@ -157,26 +157,55 @@ __device__ __forceinline__ void multilinear_point(const VT *V, const T *xi, T *x
 //   boundary id 1: lambda = v+.n/tau + p+ ; 2: lambda = 0 ; 3: absorbing
 //   test values: v: (p+ - lambda)/rho n ; p: rho c^2 (-v+.n + tau (lambda - p+))
 // nv_m is the neighbour's v.n with the SAME normal (v-.n).
+//
+// lambda is linear in the traces: lambda = l[0] v+.n + l[1] p+ + l[2] v-.n + l[3] p-.
 template <typename T>
-__device__ __forceinline__ void flux(T nvp, T pp, T nv_m, T pm, T rho_p, T c_p, T rho_m, T c_m,
-                                     int bid, T &val_pn, T &val_p) {
+__device__ __forceinline__ void flux_lambda(T rho_p, T c_p, T rho_m, T c_m, int bid, T *l) {
   const T tau_p = T(1) / (c_p * rho_p);
-  T lambda;
+  l[0] = l[1] = l[2] = l[3] = T(0);
   if (bid == 0) {
     const T tau_m = T(1) / (c_m * rho_m);
-    lambda = (nvp - nv_m + tau_p * pp + tau_m * pm) / (tau_p + tau_m);
+    const T inv = T(1) / (tau_p + tau_m);
+    l[0] = inv;
+    l[1] = inv * tau_p;
+    l[2] = -inv;
+    l[3] = inv * tau_m;
   } else if (bid == 1) {
-    lambda = nvp / tau_p + pp;
-  } else if (bid == 2) {
-    lambda = T(0);
-  } else {
+    l[0] = T(1) / tau_p;
+    l[1] = T(1);
+  } else if (bid == 3) {
     // first-order absorbing condition, the reference's expression with the
     // exterior impedance equal to the interior one
     const T tau_e = T(1) / (c_p * rho_p);
-    lambda = tau_p / (tau_p + tau_e) * pp + T(1) / (tau_p + tau_e) * nvp;
-  }
+    l[0] = T(1) / (tau_p + tau_e);
+    l[1] = tau_p / (tau_p + tau_e);
+  }  // bid 2: lambda = 0
+}
+
+template <typename T>
+__device__ __forceinline__ void flux(T nvp, T pp, T nv_m, T pm, T rho_p, T c_p, T rho_m, T c_m,
+                                     int bid, T &val_pn, T &val_p) {
+  T l[4];
+  flux_lambda<T>(rho_p, c_p, rho_m, c_m, bid, l);
+  const T tau_p = T(1) / (c_p * rho_p);
+  const T lambda = l[0] * nvp + l[1] * pp + l[2] * nv_m + l[3] * pm;
   val_pn = (pp - lambda) / rho_p;  // multiplies n_d for the velocity test values
   val_p = c_p * c_p * rho_p * (-nvp + tau_p * (lambda - pp));
+}
+
+// The same flux as a linear map of (own v.n, own p, neighbour v.n, neighbour
+// p) -> (v test value, p test value): a[j], b[j] multiply input j.
+template <typename T>
+__device__ __forceinline__ void flux_matrix(T rho_p, T c_p, T rho_m, T c_m, int bid, T *a, T *b) {
+  T l[4];
+  flux_lambda<T>(rho_p, c_p, rho_m, c_m, bid, l);
+  const T irho = T(1) / rho_p, tau_p = T(1) / (c_p * rho_p), z = c_p * c_p * rho_p;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const T epp = j == 1 ? T(1) : T(0), env = j == 0 ? T(1) : T(0);
+    a[j] = (epp - l[j]) * irho;
+    b[j] = z * (-env + tau_p * (l[j] - epp));
+  }
 }
 
 }  // namespace exw

@@ -24,6 +24,7 @@
 // one read + one write per value and line instead of N reads per derivative.
 #pragma once
 #include "dg_common.hpp"
+#include "gauss_cx.hpp"
 
 namespace exw {
 
@@ -40,30 +41,24 @@ struct CartShape {
   static constexpr int BLOCK = ((CPB * LANES + 63) / 64) * 64;
 };
 
-// 1D matrices of degree K (meaning as TabOff) in the operator's arithmetic
-// type, passed by value so that every lane reads them from scalar registers
-// (no per-lane conversion in the FP32 kernel).
-template <int N, typename T>
-struct CartTab {
-  T D[N * N];
-  T DT[N * N];
-  T L[2 * N];
-  T LW[2 * N];
-};
-
 template <int DIM, int K, typename Number>
 __global__ void __launch_bounds__((CartShape<DIM, K>::BLOCK))
     dg_operator_cart_kernel(const DevMesh mesh, const double *__restrict__ src,
-                            const CartTab<K + 1, Number> tab, const Epilogue epi) {
+                            const Epilogue epi) {
   using S = CartShape<DIM, K>;
   constexpr int N = S::N, ND = S::ND, NT = S::NT, NF = S::NF, LANES = S::LANES, CPB = S::CPB;
   constexpr int NFACE = 2 * DIM;
+  // the 1D matrices are compile-time constants (folded into the unrolled loops)
+  constexpr const CartConst<N> &T = CartTable<N>::value;
   __shared__ Number s_u[CPB][NF][ND];
   // neighbour (v_n, p) nodes of face f; phase B overwrites [2d][0] with the
   // lines' dv_d/dt and [2d][1] with their dp/dt share
   __shared__ Number s_nb[CPB][NFACE][2][ND];
-  __shared__ int s_nbr[CPB][NFACE];
-  __shared__ Number s_nmat[CPB][NFACE][2];
+  // per face: the numerical flux as a linear map of the traces (own v_n, own
+  // p, neighbour v_n, neighbour p) -> (v lifting, p lifting), with 1/h and the
+  // normal's sign folded in; per direction: -1/(rho h), rho c^2/h
+  __shared__ Number s_fc[CPB][NFACE][8];
+  __shared__ Number s_vc[CPB][DIM][2];
 
   const int tid = threadIdx.x;
   const int lc = tid / LANES;
@@ -108,19 +103,31 @@ __global__ void __launch_bounds__((CartShape<DIM, K>::BLOCK))
       nv[f][1] = un[DIM * ND + lane];
     }
   }
-  Number rho = Number(1), cc = Number(1);
-  if (active) {
-    rho = Number(mesh.rho[cell]);
-    cc = Number(mesh.speed[cell]);
-  }
   if (active && lane < NFACE) {
+    // face lane: columns of the linear flux map (evaluate_inner_face /
+    // evaluate_boundary_face, source/wave_equation_operations.cc:480-629)
+    const int f = lane, d = f >> 1;
     int nb = nbr[0];
 #pragma unroll
-    for (int f = 1; f < NFACE; ++f)
-      if (lane == f) nb = nbr[f];
-    s_nbr[lc][lane] = nb;
-    s_nmat[lc][lane][0] = nb >= 0 ? Number(mesh.rho[nb]) : rho;
-    s_nmat[lc][lane][1] = nb >= 0 ? Number(mesh.speed[nb]) : cc;
+    for (int g = 1; g < NFACE; ++g)
+      if (lane == g) nb = nbr[g];
+    const double rho = mesh.rho[cell], cc = mesh.speed[cell];
+    const double rho_m = nb >= 0 ? mesh.rho[nb] : rho, c_m = nb >= 0 ? mesh.speed[nb] : cc;
+    const double ih = 1.0 / mesh.h[(size_t)cell * DIM + d];
+    const double sg = (f & 1) ? 1.0 : -1.0;
+    double a[4], b[4];
+    flux_matrix<double>(rho, cc, rho_m, c_m, nb >= 0 ? 0 : -nb, a, b);
+    // inputs (v_d, p, neighbour v_d, neighbour p): v.n = sg v_d on both sides
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const double in = (j == 0 || j == 2) ? sg : 1.0;
+      s_fc[lc][f][j] = Number(a[j] * in * sg * ih);
+      s_fc[lc][f][4 + j] = Number(b[j] * in * ih);
+    }
+    if ((f & 1) == 0) {  // the cell term's factors of direction d
+      s_vc[lc][d][0] = Number(-ih / rho);
+      s_vc[lc][d][1] = Number(rho * cc * cc * ih);
+    }
   }
   if (node_ok) {
 #pragma unroll
@@ -145,18 +152,13 @@ __global__ void __launch_bounds__((CartShape<DIM, K>::BLOCK))
       base = d == 0 ? t0 * N + t1 * N * N : (d == 1 ? t0 + t1 * N * N : t0 + t1 * N);
       sd = d == 0 ? 1 : (d == 1 ? N : N * N);
     }
-    const Number hd = Number(mesh.h[(size_t)cell * DIM + d]);
-    const Number ih = Number(1) / hd, rho_inv = Number(1) / rho, rc2 = rho * cc * cc;
-    Number vl[N], pl[N], vm[N], pm[N], vp[N], pp[N];
+    const Number cvv = s_vc[lc][d][0], cvp = s_vc[lc][d][1];
+    Number vl[N], pl[N];
 #pragma unroll
     for (int i = 0; i < N; ++i) {
       const int n = base + i * sd;
       vl[i] = s_u[lc][d][n];
       pl[i] = s_u[lc][DIM][n];
-      vm[i] = s_nb[lc][2 * d][0][n];
-      pm[i] = s_nb[lc][2 * d][1][n];
-      vp[i] = s_nb[lc][2 * d + 1][0][n];
-      pp[i] = s_nb[lc][2 * d + 1][1][n];
     }
     Number kv[N], kp[N];
 #pragma unroll
@@ -164,36 +166,33 @@ __global__ void __launch_bounds__((CartShape<DIM, K>::BLOCK))
       Number gp = Number(0), dv = Number(0);
 #pragma unroll
       for (int m = 0; m < N; ++m) {
-        gp += tab.D[i * N + m] * pl[m];
-        dv += tab.DT[i * N + m] * vl[m];
+        gp += Number(T.D[i * N + m]) * pl[m];
+        dv += Number(T.DT[i * N + m]) * vl[m];
       }
-      kv[i] = -rho_inv * gp * ih;
-      kp[i] = rc2 * dv * ih;
+      kv[i] = cvv * gp;
+      kp[i] = cvp * dv;
     }
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
+      const int f = 2 * d + s;
       Number vo = Number(0), po = Number(0), vn = Number(0), pn = Number(0);
 #pragma unroll
       for (int i = 0; i < N; ++i) {
-        const Number ls = tab.L[s * N + i], lo = tab.L[(1 - s) * N + i];
+        const int n = base + i * sd;
+        const Number ls = Number(T.L[s * N + i]), lo = Number(T.L[(1 - s) * N + i]);
         vo += ls * vl[i];
         po += ls * pl[i];
-        vn += lo * (s ? vp[i] : vm[i]);
-        pn += lo * (s ? pp[i] : pm[i]);
+        vn += lo * s_nb[lc][f][0][n];
+        pn += lo * s_nb[lc][f][1][n];
       }
-      const int f = 2 * d + s;
-      const int nb = s_nbr[lc][f];
-      const Number sg = s ? Number(1) : Number(-1);
-      Number val_pn, val_p;
-      flux<Number>(sg * vo, po, sg * vn, pn, rho, cc, s_nmat[lc][f][0], s_nmat[lc][f][1],
-                   nb >= 0 ? 0 : -nb, val_pn, val_p);
-      val_pn *= sg * ih;
-      val_p *= ih;
+      const Number *c = s_fc[lc][f];
+      const Number fv = c[0] * vo + c[1] * po + c[2] * vn + c[3] * pn;
+      const Number fp = c[4] * vo + c[5] * po + c[6] * vn + c[7] * pn;
 #pragma unroll
       for (int i = 0; i < N; ++i) {
-        const Number lw = tab.LW[s * N + i];
-        kv[i] += lw * val_pn;
-        kp[i] += lw * val_p;
+        const Number lw = Number(T.LW[s * N + i]);
+        kv[i] += lw * fv;
+        kp[i] += lw * fp;
       }
     }
 #pragma unroll

@@ -43,24 +43,13 @@ template <int DIM, int K, typename Number, bool CART>
 hipError_t launch_op(const DevMesh &m, const double *src, const double *tab,
                      const double *host_tab, const Epilogue &e, hipStream_t s) {
   if constexpr (CART) {
-    // box cells: node/line/node kernel with the 1D matrices in the kernel arguments
+    // box cells: node/line/node kernel with compile-time 1D matrices
     using CS = CartShape<DIM, K>;
-    constexpr int N = K + 1;
-    using TO = TabOff<N>;
-    CartTab<N, Number> ct;
-    for (int i = 0; i < N * N; ++i) {
-      ct.D[i] = Number(host_tab[TO::D + i]);
-      ct.DT[i] = Number(host_tab[TO::DT + i]);
-    }
-    for (int i = 0; i < 2 * N; ++i) {
-      ct.L[i] = Number(host_tab[TO::L + i]);
-      ct.LW[i] = Number(host_tab[TO::LW + i]);
-    }
     const int n = e.cell_list ? e.n_list : m.n_cells;
     const int grid = (n + CS::CPB - 1) / CS::CPB;
     if (grid == 0) return hipSuccess;
     hipLaunchKernelGGL((dg_operator_cart_kernel<DIM, K, Number>), dim3(grid), dim3(CS::BLOCK), 0,
-                       s, m, src, ct, e);
+                       s, m, src, e);
   } else {
     using S = Shape<DIM, K>;
     if (e.cell_list) return hipErrorInvalidValue;  // cell lists: Cartesian kernel only
