@@ -28,6 +28,10 @@
 
 namespace exw {
 
+#ifndef EXW_CART_GROUP
+#define EXW_CART_GROUP 128
+#endif
+
 template <int DIM, int K>
 struct CartShape {
   static constexpr int N = K + 1;
@@ -35,11 +39,16 @@ struct CartShape {
   static constexpr int NT = DIM == 2 ? N : N * N;  // lines per direction
   static constexpr int NF = DIM + 1;
   static constexpr int NLINE = DIM * NT;           // lines per cell
-  static constexpr int LANES = ND > NLINE ? ND : NLINE;
-  // cells per workgroup: about 128 lanes per group (more groups in flight)
-  static constexpr int CPB = LANES >= 128 ? 1 : 128 / LANES;
+  // node lanes per cell: cells of >= 48 nodes occupy whole waves (their
+  // neighbour rows stay wave-uniform scalar loads); smaller cells are packed
+  static constexpr bool WAVE_ALIGNED = ND >= 48;
+  static constexpr int LANES = WAVE_ALIGNED ? ((ND + 63) / 64) * 64 : (ND > NLINE ? ND : NLINE);
+  // cells per workgroup: about EXW_CART_GROUP lanes per group
+  static constexpr int CPB = LANES >= EXW_CART_GROUP ? 1 : EXW_CART_GROUP / LANES;
   static constexpr int BLOCK = ((CPB * LANES + 63) / 64) * 64;
+  static_assert(CPB * NLINE <= BLOCK, "the packed line phase must fit the group");
 };
+
 
 template <int DIM, int K, typename Number>
 __global__ void __launch_bounds__((CartShape<DIM, K>::BLOCK))
@@ -59,6 +68,7 @@ __global__ void __launch_bounds__((CartShape<DIM, K>::BLOCK))
   // normal's sign folded in; per direction: -1/(rho h), rho c^2/h
   __shared__ Number s_fc[CPB][NFACE][8];
   __shared__ Number s_vc[CPB][DIM][2];
+  __shared__ int s_act[CPB];
 
   const int tid = threadIdx.x;
   const int lc = tid / LANES;
@@ -73,6 +83,7 @@ __global__ void __launch_bounds__((CartShape<DIM, K>::BLOCK))
     active = cscale != 0.0;
   }
   const bool node_ok = active && lane < ND;
+  if (lane_ok && lane == 0) s_act[lc] = active ? 1 : 0;
   const double ea1 = epi.a1 * cscale, ea2 = epi.a2 * cscale;
   const size_t cbase = (size_t)cell * NF * ND;
 
@@ -140,9 +151,10 @@ __global__ void __launch_bounds__((CartShape<DIM, K>::BLOCK))
   }
   __syncthreads();
 
-  // ---------------- B: one lane per line ----------------
-  if (active && lane < S::NLINE) {
-    const int d = lane / NT, t = lane - d * NT;
+  // ---------------- B: one lane per line, the group's lines packed ----------------
+  const int lcB = tid / S::NLINE, line = tid - lcB * S::NLINE;
+  if (lcB < CPB && s_act[lcB]) {
+    const int d = line / NT, t = line - d * NT;
     int base, sd;
     if constexpr (DIM == 2) {
       base = d == 0 ? t * N : t;
@@ -152,13 +164,13 @@ __global__ void __launch_bounds__((CartShape<DIM, K>::BLOCK))
       base = d == 0 ? t0 * N + t1 * N * N : (d == 1 ? t0 + t1 * N * N : t0 + t1 * N);
       sd = d == 0 ? 1 : (d == 1 ? N : N * N);
     }
-    const Number cvv = s_vc[lc][d][0], cvp = s_vc[lc][d][1];
+    const Number cvv = s_vc[lcB][d][0], cvp = s_vc[lcB][d][1];
     Number vl[N], pl[N];
 #pragma unroll
     for (int i = 0; i < N; ++i) {
       const int n = base + i * sd;
-      vl[i] = s_u[lc][d][n];
-      pl[i] = s_u[lc][DIM][n];
+      vl[i] = s_u[lcB][d][n];
+      pl[i] = s_u[lcB][DIM][n];
     }
     Number kv[N], kp[N];
 #pragma unroll
@@ -182,10 +194,10 @@ __global__ void __launch_bounds__((CartShape<DIM, K>::BLOCK))
         const Number ls = Number(T.L[s * N + i]), lo = Number(T.L[(1 - s) * N + i]);
         vo += ls * vl[i];
         po += ls * pl[i];
-        vn += lo * s_nb[lc][f][0][n];
-        pn += lo * s_nb[lc][f][1][n];
+        vn += lo * s_nb[lcB][f][0][n];
+        pn += lo * s_nb[lcB][f][1][n];
       }
-      const Number *c = s_fc[lc][f];
+      const Number *c = s_fc[lcB][f];
       const Number fv = c[0] * vo + c[1] * po + c[2] * vn + c[3] * pn;
       const Number fp = c[4] * vo + c[5] * po + c[6] * vn + c[7] * pn;
 #pragma unroll
@@ -198,8 +210,8 @@ __global__ void __launch_bounds__((CartShape<DIM, K>::BLOCK))
 #pragma unroll
     for (int i = 0; i < N; ++i) {
       const int n = base + i * sd;
-      s_nb[lc][2 * d][0][n] = kv[i];
-      s_nb[lc][2 * d][1][n] = kp[i];
+      s_nb[lcB][2 * d][0][n] = kv[i];
+      s_nb[lcB][2 * d][1][n] = kp[i];
     }
   }
   __syncthreads();
