@@ -685,8 +685,7 @@ int exw_server_progress(exw_server s, int max_steps) {
   int n_active = 0;
   for (int it = 0; it < max_steps; ++it) {
     n_active = 0;
-    int launched = 0;
-    Pool *oldest = nullptr;
+    int launched = 0, waiting = 0;
     for (auto &kv : s->pools) {
       Pool *p = kv.second;
       if (p->active.empty()) continue;
@@ -697,7 +696,7 @@ int exw_server_progress(exw_server s, int max_steps) {
         p->inflight.pop_front();
       }
       if ((int)p->inflight.size() >= s->lookahead) {
-        if (!oldest) oldest = p;
+        ++waiting;
         continue;
       }
       if (p->list_dirty)
@@ -728,12 +727,10 @@ int exw_server_progress(exw_server s, int max_steps) {
       }
     }
     if (n_active == 0) break;
-    if (!launched && oldest) {
-      // every active pool has a full look-ahead: wait for the oldest step of one
-      hipEvent_t e = oldest->inflight.front();
-      SHIP(hipEventSynchronize(e));
-      oldest->inflight.pop_front();
-      oldest->event_pool.push_back(e);
+    if (!launched && waiting) {
+      // every active pool has a full look-ahead: return to the caller, which
+      // polls and calls again (no pool is waited for while others could step)
+      break;
     }
   }
   // retire completed look-ahead events without blocking
