@@ -159,26 +159,39 @@ __device__ __forceinline__ void multilinear_point(const VT *V, const T *xi, T *x
 // nv_m is the neighbour's v.n with the SAME normal (v-.n).
 //
 // lambda is linear in the traces: lambda = l[0] v+.n + l[1] p+ + l[2] v-.n + l[3] p-.
+//
+// Reciprocals: the hardware reciprocal refined by two Newton steps (within
+// an ulp or two of the correctly rounded quotient, at half the instructions
+// of an IEEE division).
+__device__ __forceinline__ double recip(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+  r = __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+  return r;
+}
+__device__ __forceinline__ float recip(float x) { return 1.0f / x; }
+
 template <typename T>
 __device__ __forceinline__ void flux_lambda(T rho_p, T c_p, T rho_m, T c_m, int bid, T *l) {
-  const T tau_p = T(1) / (c_p * rho_p);
+  const T tau_p = recip(c_p * rho_p);
   l[0] = l[1] = l[2] = l[3] = T(0);
   if (bid == 0) {
-    const T tau_m = T(1) / (c_m * rho_m);
-    const T inv = T(1) / (tau_p + tau_m);
+    const T tau_m = recip(c_m * rho_m);
+    const T inv = recip(tau_p + tau_m);
     l[0] = inv;
     l[1] = inv * tau_p;
     l[2] = -inv;
     l[3] = inv * tau_m;
   } else if (bid == 1) {
-    l[0] = T(1) / tau_p;
+    l[0] = c_p * rho_p;  // 1/tau
     l[1] = T(1);
   } else if (bid == 3) {
     // first-order absorbing condition, the reference's expression with the
     // exterior impedance equal to the interior one
-    const T tau_e = T(1) / (c_p * rho_p);
-    l[0] = T(1) / (tau_p + tau_e);
-    l[1] = tau_p / (tau_p + tau_e);
+    const T tau_e = recip(c_p * rho_p);
+    const T inv = recip(tau_p + tau_e);
+    l[0] = inv;
+    l[1] = tau_p * inv;
   }  // bid 2: lambda = 0
 }
 
@@ -187,9 +200,9 @@ __device__ __forceinline__ void flux(T nvp, T pp, T nv_m, T pm, T rho_p, T c_p, 
                                      int bid, T &val_pn, T &val_p) {
   T l[4];
   flux_lambda<T>(rho_p, c_p, rho_m, c_m, bid, l);
-  const T tau_p = T(1) / (c_p * rho_p);
+  const T tau_p = recip(c_p * rho_p);
   const T lambda = l[0] * nvp + l[1] * pp + l[2] * nv_m + l[3] * pm;
-  val_pn = (pp - lambda) / rho_p;  // multiplies n_d for the velocity test values
+  val_pn = (pp - lambda) * recip(rho_p);  // multiplies n_d for the velocity test values
   val_p = c_p * c_p * rho_p * (-nvp + tau_p * (lambda - pp));
 }
 
@@ -199,7 +212,7 @@ template <typename T>
 __device__ __forceinline__ void flux_matrix(T rho_p, T c_p, T rho_m, T c_m, int bid, T *a, T *b) {
   T l[4];
   flux_lambda<T>(rho_p, c_p, rho_m, c_m, bid, l);
-  const T irho = T(1) / rho_p, tau_p = T(1) / (c_p * rho_p), z = c_p * c_p * rho_p;
+  const T irho = recip(rho_p), tau_p = recip(c_p * rho_p), z = c_p * c_p * rho_p;
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     const T epp = j == 1 ? T(1) : T(0), env = j == 0 ? T(1) : T(0);

@@ -71,7 +71,10 @@ __global__ void __launch_bounds__((CartShape<DIM, K>::BLOCK))
   __shared__ int s_act[CPB];
 
   const int tid = threadIdx.x;
-  const int lc = tid / LANES;
+  int lc = tid / LANES;
+  // a wave-aligned cell slot is uniform over the wave: say so, so that the
+  // cell's index, neighbour row and material are scalar loads
+  if constexpr (S::WAVE_ALIGNED) lc = __builtin_amdgcn_readfirstlane(lc);
   const int lane = tid - lc * LANES;
   const int e = xcd_remap(blockIdx.x, gridDim.x) * CPB + lc;
   const int cell = epi.cell_list ? (e < epi.n_list ? epi.cell_list[e] : -1) : e;
@@ -124,7 +127,7 @@ __global__ void __launch_bounds__((CartShape<DIM, K>::BLOCK))
       if (lane == g) nb = nbr[g];
     const double rho = mesh.rho[cell], cc = mesh.speed[cell];
     const double rho_m = nb >= 0 ? mesh.rho[nb] : rho, c_m = nb >= 0 ? mesh.speed[nb] : cc;
-    const double ih = 1.0 / mesh.h[(size_t)cell * DIM + d];
+    const double ih = recip(mesh.h[(size_t)cell * DIM + d]);
     const double sg = (f & 1) ? 1.0 : -1.0;
     double a[4], b[4];
     flux_matrix<double>(rho, cc, rho_m, c_m, nb >= 0 ? 0 : -nb, a, b);
@@ -136,7 +139,7 @@ __global__ void __launch_bounds__((CartShape<DIM, K>::BLOCK))
       s_fc[lc][f][4 + j] = Number(b[j] * in * ih);
     }
     if ((f & 1) == 0) {  // the cell term's factors of direction d
-      s_vc[lc][d][0] = Number(-ih / rho);
+      s_vc[lc][d][0] = Number(-ih * recip(rho));
       s_vc[lc][d][1] = Number(rho * cc * cc * ih);
     }
   }
