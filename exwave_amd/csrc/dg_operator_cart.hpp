@@ -96,26 +96,28 @@ __global__ void __launch_bounds__((CartShape<DIM, K>::BLOCK))
   int nbr[NFACE];
 #pragma unroll
   for (int f = 0; f < NFACE; ++f) nbr[f] = active ? mesh.neighbor[(size_t)cell * NFACE + f] : -1;
+  // every load below is unconditional and in bounds (no branch between the
+  // loads, so all of them are in flight together): lanes past the last node
+  // read the last node, inactive groups read cell 0, and boundary faces read
+  // the cell itself -- their neighbour inputs have zero flux coefficients
+  const int lane_c = lane < ND ? lane : ND - 1;
+  const int cell_c = active ? cell : 0;
+  const size_t cbase_c = (size_t)cell_c * NF * ND;
   double uo[NF], bv[NF], nv[NFACE][2];
 #pragma unroll
-  for (int f = 0; f < NF; ++f) uo[f] = bv[f] = 0.0;
-  if (node_ok) {
+  for (int f = 0; f < NF; ++f) {
+    uo[f] = src[cbase_c + f * ND + lane_c];
+    bv[f] = 0.0;
+  }
+  if (epi.b) {
 #pragma unroll
-    for (int f = 0; f < NF; ++f) uo[f] = src[cbase + f * ND + lane];
-    if (epi.b) {
-#pragma unroll
-      for (int f = 0; f < NF; ++f)
-        bv[f] = __builtin_nontemporal_load(epi.b + cbase + f * ND + lane);
-    }
+    for (int f = 0; f < NF; ++f) bv[f] = __builtin_nontemporal_load(epi.b + cbase_c + f * ND + lane_c);
   }
 #pragma unroll
   for (int f = 0; f < NFACE; ++f) {
-    nv[f][0] = nv[f][1] = 0.0;
-    if (node_ok && nbr[f] >= 0) {
-      const double *un = src + (size_t)nbr[f] * NF * ND;
-      nv[f][0] = un[(f >> 1) * ND + lane];
-      nv[f][1] = un[DIM * ND + lane];
-    }
+    const double *un = src + (size_t)(nbr[f] >= 0 ? nbr[f] : cell_c) * NF * ND;
+    nv[f][0] = un[(f >> 1) * ND + lane_c];
+    nv[f][1] = un[DIM * ND + lane_c];
   }
   if (active && lane < NFACE) {
     // face lane: columns of the linear flux map (evaluate_inner_face /
@@ -143,13 +145,13 @@ __global__ void __launch_bounds__((CartShape<DIM, K>::BLOCK))
       s_vc[lc][d][1] = Number(rho * cc * cc * ih);
     }
   }
-  if (node_ok) {
+  if (lane_ok) {  // lanes past the last node store the last node's (same) values
 #pragma unroll
-    for (int f = 0; f < NF; ++f) s_u[lc][f][lane] = Number(uo[f]);
+    for (int f = 0; f < NF; ++f) s_u[lc][f][lane_c] = Number(uo[f]);
 #pragma unroll
     for (int f = 0; f < NFACE; ++f) {
-      s_nb[lc][f][0][lane] = Number(nv[f][0]);
-      s_nb[lc][f][1][lane] = Number(nv[f][1]);
+      s_nb[lc][f][0][lane_c] = Number(nv[f][0]);
+      s_nb[lc][f][1][lane_c] = Number(nv[f][1]);
     }
   }
   __syncthreads();
