@@ -46,7 +46,7 @@ hipError_t launch_op(const DevMesh &m, const double *src, const double *tab,
     // box cells: node/line/node kernel with compile-time 1D matrices
     using CS = CartShape<DIM, K>;
     const int n = e.cell_list ? e.n_list : m.n_cells;
-    const int grid = (n + CS::CPB - 1) / CS::CPB;
+    const int grid = (n + CS::CPB * CS::ITER - 1) / (CS::CPB * CS::ITER);
     if (grid == 0) return hipSuccess;
     hipLaunchKernelGGL((dg_operator_cart_kernel<DIM, K, Number>), dim3(grid), dim3(CS::BLOCK), 0,
                        s, m, src, e);

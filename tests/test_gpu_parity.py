@@ -90,6 +90,34 @@ def test_apply_parity(X, ctx, case):
     assert _rel(_host(dst), want) <= FP32_TOL
 
 
+@pytest.mark.parametrize("dim,nint,k,prec", [(3, 4, 4, "fp64"), (3, 4, 3, "fp32"), (2, 6, 6, "fp64"),
+                                             (2, 7, 2, "fp64"), (3, 3, 6, "fp64")])
+def test_apply_any_cell_order(X, ctx, dim, nint, k, prec):
+    """The Cartesian kernel on a randomly renumbered mesh (any cell order, e.g.
+    the tiled order of the benchmark) against the oracle on the lexicographic
+    mesh, through the same permutation."""
+    _say("any cell order", dim, nint, k, prec)
+    om = O.Mesh(dim, nint, 0, 0, 0.0, 2)
+    rng = np.random.default_rng(11)
+    rho, c = rng.uniform(0.5, 2.0, om.n_cells), rng.uniform(0.5, 2.0, om.n_cells)
+    om.set_material(rho, c)
+    perm = rng.permutation(om.n_cells)          # new -> old
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(perm.size)
+    nb = om.neighbor[perm].copy()
+    nb[nb >= 0] = inv[nb[nb >= 0]]
+    op = X.WaveOperator(ctx, dim, k, om.vertices[perm], nb, rho[perm], c[perm])
+    assert op.is_cartesian
+    dpc = om.n_dofs(k) // om.n_cells
+    src = rng.standard_normal(om.n_dofs(k))
+    ref = om.apply(k, src)
+    src_p = src.reshape(om.n_cells, dpc)[perm].reshape(-1)
+    d = _dev(np.zeros_like(src))
+    op.apply(_dev(src_p), d, precision=X.FP64 if prec == "fp64" else X.FP32)
+    got = _host(d).reshape(om.n_cells, dpc)[inv].reshape(-1)
+    assert _rel(got, ref) <= (FP64_TOL if prec == "fp64" else FP32_TOL)
+
+
 def test_rk_stage_matches_updater(X, ctx):
     """exw_rk_stage == apply + RKVectorUpdater (source/time_integrators.cc:128-155)."""
     dim, k = 3, 4
