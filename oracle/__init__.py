@@ -64,7 +64,8 @@ class Problem(C.Structure):
                 ("integrator", C.c_int), ("cfl_number", C.c_double),
                 ("max_time_steps", C.c_int), ("final_time", C.c_double),
                 ("output_every_time", C.c_double), ("ader_recon", C.c_int),
-                ("exact", Exact), ("lts_max_clusters", C.c_int), ("lts_max_diff", C.c_int)]
+                ("exact", Exact), ("lts_max_clusters", C.c_int), ("lts_max_diff", C.c_int),
+                ("post_pressure", C.c_int)]
 
 
 _lib = None
@@ -113,6 +114,10 @@ def lib():
         L.ox_lts_schedule.argtypes = [C.c_void_p, ip, dp]
         L.ox_lts_cells.argtypes = [C.c_void_p, ip, ip]
         L.ox_lts_step.argtypes = [C.c_void_p, C.c_double, dp, dp]
+        L.ox_gauss_lobatto.argtypes = [C.c_int, dp]
+        L.ox_post_pressure.argtypes = [C.POINTER(_Mesh), C.c_int, dp, dp]
+        L.ox_post_error.restype = C.c_double
+        L.ox_post_error.argtypes = [C.POINTER(_Mesh), C.c_int, C.POINTER(Exact), C.c_double, dp]
     return _lib
 
 
@@ -203,17 +208,34 @@ class Mesh:
     def energy(self, degree, u):
         return lib().ox_energy(self._m, degree, _p(u))
 
+    def post_pressure(self, degree, u):
+        """compute_post_pressure: n_cells x (k+2)^dim Gauss-Lobatto nodal values."""
+        out = np.zeros(self.n_cells * (degree + 2) ** self.dim)
+        lib().ox_post_pressure(self._m, degree, _p(u), _p(out))
+        return out
+
+    def post_error(self, degree, exact, t, post):
+        return lib().ox_post_error(self._m, degree, C.byref(exact), t, _p(post))
+
 
 def run(problem: Problem):
-    """Run a whole solve; returns (n_steps, [(step, time, err_p, err_v, mag_p)], energy)."""
+    """Run a whole solve; returns (n_steps, [(step, time, err_p, err_v, mag_p)], energy);
+    with problem.post_pressure set the rows carry a sixth entry, error p post."""
     buf = C.create_string_buffer(1 << 16)
     en = C.c_double(0.0)
     steps = lib().ox_run(C.byref(problem), buf, len(buf), C.byref(en))
     rows = []
     for line in buf.value.decode().strip().splitlines():
-        s, t, ep, ev, mp = line.split()
-        rows.append((int(s), float(t), float(ep), float(ev), float(mp)))
+        s, t, ep, ev, mp, epp = line.split()
+        row = (int(s), float(t), float(ep), float(ev), float(mp))
+        rows.append(row + (float(epp),) if problem.post_pressure else row)
     return steps, rows, en.value
+
+
+def gauss_lobatto(n):
+    x = np.zeros(n)
+    lib().ox_gauss_lobatto(n, _p(x))
+    return x
 
 
 def gauss(n):

@@ -906,6 +906,260 @@ double ox_energy(const ox_mesh *m, int degree, const double *u) {
 }
 
 /* ------------------------------------------------------------------------ */
+/* post-processed pressure (compute_post_pressure,                            */
+/* source/wave_equation_operations.cc:786-848)                                */
+/* ------------------------------------------------------------------------ */
+
+/* n-point Gauss-Lobatto points on [0,1] (ends and the roots of P'_{n-1}) */
+void ox_gauss_lobatto(int n, double *x) {
+  const int m = n - 1;
+  x[0] = 0.0;
+  x[n - 1] = 1.0;
+  for (int i = 1; i < n - 1; ++i) {
+    double z = -cos(M_PI * i / m);
+    for (int it = 0; it < 100; ++it) {
+      double p0 = 1.0, p1 = z;
+      for (int j = 2; j <= m; ++j) {
+        const double p2 = ((2.0 * j - 1.0) * z * p1 - (j - 1.0) * p0) / j;
+        p0 = p1;
+        p1 = p2;
+      }
+      const double pm = m == 1 ? z : p1, pm1 = m == 1 ? 1.0 : p0;
+      const double d1 = m * (z * pm - pm1) / (z * z - 1.0);          /* P'_m */
+      const double d2 = (2.0 * z * d1 - m * (m + 1.0) * pm) / (1.0 - z * z); /* P''_m */
+      const double dz = d1 / d2;
+      z -= dz;
+      if (fabs(dz) < 1e-16) break;
+    }
+    x[i] = 0.5 * (1.0 + z);
+  }
+}
+
+/* tensor interpolation of one field: A (n_out x n_in) along every direction */
+static void tensor_interp(int dim, int n_in, int n_out, const double *A, const double *in,
+                          double *out, double *w1, double *w2) {
+  int sz[3] = {n_in, n_in, n_in};
+  const double *src = in;
+  double *bufs[2] = {w1, w2};
+  for (int d = 0; d < dim; ++d) {
+    double *o = (d == dim - 1) ? out : bufs[d & 1];
+    tensor_apply(dim, sz, d, A, n_out, src, o, 0, 0);
+    sz[d] = n_out;
+    src = o;
+  }
+}
+
+/* reference gradient (dim components) at the nq^dim points of a field on the
+ * np^dim nodes: along e the derivative matrix Pd, along the others Pv */
+static void tensor_grad(int dim, int np, int nq, const double *Pv, const double *Pd,
+                        const double *in, double *g, double *w1, double *w2) {
+  for (int e = 0; e < dim; ++e) {
+    int sz[3] = {np, np, np};
+    const double *src = in;
+    double *bufs[2] = {w1, w2};
+    for (int d = 0; d < dim; ++d) {
+      double *o = (d == dim - 1) ? g + (long)e * ipow(nq, dim) : bufs[d & 1];
+      tensor_apply(dim, sz, d, d == e ? Pd : Pv, nq, src, o, 0, 0);
+      sz[d] = nq;
+      src = o;
+    }
+  }
+}
+
+/* the transpose of tensor_grad: sum_e grad_e^T f_e, added into out (np^dim) */
+static void tensor_grad_t(int dim, int np, int nq, const double *Pv, const double *Pd,
+                          const double *f, double *out, double *w1, double *w2) {
+  const int npd = ipow(np, dim);
+  for (int i = 0; i < npd; ++i) out[i] = 0.0;
+  for (int e = 0; e < dim; ++e) {
+    int sz[3] = {nq, nq, nq};
+    const double *src = f + (long)e * ipow(nq, dim);
+    double *bufs[2] = {w1, w2};
+    for (int d = dim - 1; d >= 0; --d) {
+      double *o = bufs[d & 1];
+      tensor_apply(dim, sz, d, d == e ? Pd : Pv, np, src, o, 1, 0);
+      sz[d] = np;
+      src = o;
+    }
+    for (int i = 0; i < npd; ++i) out[i] += src[i];
+  }
+}
+
+void ox_post_pressure(const ox_mesh *m, int degree, const double *solution, double *post) {
+  const int dim = m->dim, n = degree + 1, np = degree + 2, nq = degree + 2, nf = dim + 1;
+  const int nd = ipow(n, dim), npd = ipow(np, dim), nqd = ipow(nq, dim), nv = 1 << dim;
+  const long N = ox_n_dofs(m, degree);
+  double *tmp = (double *)malloc(sizeof(double) * N);
+  ox_apply(m, degree, solution, tmp, 1.0); /* :795 */
+  ox_basis b;
+  ox_basis_init(&b, degree);
+  double xq[OX_MAX_N], wq[OX_MAX_N], xp[OX_MAX_N];
+  ox_gauss(nq, xq, wq);
+  ox_gauss_lobatto(np, xp);
+  double Iq[OX_MAX_N * OX_MAX_N], Pv[OX_MAX_N * OX_MAX_N], Pd[OX_MAX_N * OX_MAX_N];
+  for (int q = 0; q < nq; ++q) {
+    ox_lagrange(n, b.x, xq[q], Iq + q * n, NULL);
+    ox_lagrange(np, xp, xq[q], Pv + q * np, Pd + q * np);
+  }
+#pragma omp parallel
+  {
+    const int W = nqd > npd ? nqd : npd;
+    double *w1 = (double *)malloc(sizeof(double) * W), *w2 = (double *)malloc(sizeof(double) * W);
+    double *vt = (double *)malloc(sizeof(double) * dim * nqd);
+    double *ph = (double *)malloc(sizeof(double) * nqd);
+    double *G = (double *)malloc(sizeof(double) * dim * dim * nqd);
+    double *jw = (double *)malloc(sizeof(double) * nqd);
+    double *fr = (double *)malloc(sizeof(double) * dim * nqd);
+    double *gr = (double *)malloc(sizeof(double) * dim * nqd);
+    double *rhs = (double *)malloc(sizeof(double) * npd);
+    double *u = (double *)malloc(sizeof(double) * npd);
+    double *r = (double *)malloc(sizeof(double) * npd);
+    double *pp = (double *)malloc(sizeof(double) * npd);
+    double *Ap = (double *)malloc(sizeof(double) * npd);
+#pragma omp for schedule(static)
+    for (int c = 0; c < m->n_cells; ++c) {
+      const double *V = m->vertices + (long)c * nv * dim;
+      const double rho = m->rho[c];
+      for (int d = 0; d < dim; ++d)
+        tensor_interp(dim, n, nq, Iq, tmp + ((long)c * nf + d) * nd, vt + (long)d * nqd, w1, w2);
+      tensor_interp(dim, n, nq, Iq, solution + ((long)c * nf + dim) * nd, ph, w1, w2);
+      /* geometry at QGauss(k+2); rhs integrand (grad phi, -rho v_t) in reference form */
+      for (int q = 0; q < nqd; ++q) {
+        double xi[3], J[9], Ji[9], w = 1.0;
+        int t = q;
+        for (int d = 0; d < dim; ++d) {
+          xi[d] = xq[t % nq];
+          w *= wq[t % nq];
+          t /= nq;
+        }
+        const double det = multilinear_jacobian(dim, V, xi, J, Ji);
+        jw[q] = det * w;
+        for (int e = 0; e < dim; ++e) {
+          double s = 0.0;
+          for (int i = 0; i < dim; ++i) s += Ji[e * dim + i] * (-rho * vt[(long)i * nqd + q]);
+          fr[(long)e * nqd + q] = s * det * w;
+          for (int e2 = 0; e2 < dim; ++e2) {
+            double g = 0.0;
+            for (int i = 0; i < dim; ++i) g += Ji[e * dim + i] * Ji[e2 * dim + i];
+            G[((long)e * dim + e2) * nqd + q] = g * det * w;
+          }
+        }
+      }
+      tensor_grad_t(dim, np, nq, Pv, Pd, fr, rhs, w1, w2);
+      rhs[0] = 0.0; /* :808-811 */
+      /* CG on the Laplacian with the first unknown fixed (the reference's
+       * MatrixPostProcessing::vmult, :750-769) */
+#define OX_POST_APPLY(in, out)                                                           \
+  do {                                                                                   \
+    const double keep = (in)[0];                                                         \
+    (in)[0] = 0.0;                                                                       \
+    tensor_grad(dim, np, nq, Pv, Pd, in, gr, w1, w2);                                    \
+    for (int q = 0; q < nqd; ++q) {                                                      \
+      double g[3];                                                                       \
+      for (int e = 0; e < dim; ++e) {                                                    \
+        g[e] = 0.0;                                                                      \
+        for (int e2 = 0; e2 < dim; ++e2) g[e] += G[((long)e * dim + e2) * nqd + q] * gr[(long)e2 * nqd + q]; \
+      }                                                                                  \
+      for (int e = 0; e < dim; ++e) fr[(long)e * nqd + q] = g[e];                        \
+    }                                                                                    \
+    tensor_grad_t(dim, np, nq, Pv, Pd, fr, out, w1, w2);                                 \
+    (out)[0] = keep;                                                                     \
+    (in)[0] = keep;                                                                      \
+  } while (0)
+      double rr = 0.0, r0;
+      for (int i = 0; i < npd; ++i) {
+        u[i] = 0.0;
+        r[i] = rhs[i];
+        pp[i] = r[i];
+        rr += r[i] * r[i];
+      }
+      r0 = rr;
+      for (int it = 0; it < 20 * npd && rr > 1e-30 * r0 && rr > 1e-60; ++it) {
+        OX_POST_APPLY(pp, Ap);
+        double pAp = 0.0;
+        for (int i = 0; i < npd; ++i) pAp += pp[i] * Ap[i];
+        const double alpha = rr / pAp;
+        double rn = 0.0;
+        for (int i = 0; i < npd; ++i) {
+          u[i] += alpha * pp[i];
+          r[i] -= alpha * Ap[i];
+          rn += r[i] * r[i];
+        }
+        const double beta = rn / rr;
+        rr = rn;
+        for (int i = 0; i < npd; ++i) pp[i] = r[i] + beta * pp[i];
+      }
+#undef OX_POST_APPLY
+      /* constant offset from the cell average of the pressure (:813-837) */
+      double vol = 0.0, correct = 0.0, prelim = 0.0;
+      tensor_interp(dim, np, nq, Pv, u, gr, w1, w2);
+      for (int q = 0; q < nqd; ++q) {
+        vol += jw[q];
+        correct += jw[q] * ph[q];
+        prelim += jw[q] * gr[q];
+      }
+      const double adjust = (correct - prelim) / vol;
+      double *o = post + (long)c * npd;
+      for (int i = 0; i < npd; ++i) o[i] = u[i] + adjust;
+    }
+    free(w1);
+    free(w2);
+    free(vt);
+    free(ph);
+    free(G);
+    free(jw);
+    free(fr);
+    free(gr);
+    free(rhs);
+    free(u);
+    free(r);
+    free(pp);
+    free(Ap);
+  }
+  free(tmp);
+}
+
+/* L2 error of the post-processed pressure with QGauss(k+3)
+ * (source/explicit_wave.cc:477-484) */
+double ox_post_error(const ox_mesh *m, int degree, const ox_exact *e, double t,
+                     const double *post) {
+  const int dim = m->dim, np = degree + 2, nq = degree + 3, npd = ipow(np, dim),
+            nqd = ipow(nq, dim), nv = 1 << dim;
+  double xq[OX_MAX_N + 2], wq[OX_MAX_N + 2], xp[OX_MAX_N], Pv[(OX_MAX_N + 2) * OX_MAX_N];
+  ox_gauss(nq, xq, wq);
+  ox_gauss_lobatto(np, xp);
+  for (int q = 0; q < nq; ++q) ox_lagrange(np, xp, xq[q], Pv + q * np, NULL);
+  double err = 0.0;
+#pragma omp parallel reduction(+ : err)
+  {
+    double *w1 = (double *)malloc(sizeof(double) * nqd), *w2 = (double *)malloc(sizeof(double) * nqd);
+    double *pq = (double *)malloc(sizeof(double) * nqd);
+#pragma omp for schedule(static)
+    for (int c = 0; c < m->n_cells; ++c) {
+      const double *V = m->vertices + (long)c * nv * dim;
+      tensor_interp(dim, np, nq, Pv, post + (long)c * npd, pq, w1, w2);
+      for (int q = 0; q < nqd; ++q) {
+        double xi[3], x[3], J[9], Ji[9], w = 1.0;
+        int tq = q;
+        for (int d = 0; d < dim; ++d) {
+          xi[d] = xq[tq % nq];
+          w *= wq[tq % nq];
+          tq /= nq;
+        }
+        multilinear_point(dim, V, xi, x);
+        const double jxw = fabs(multilinear_jacobian(dim, V, xi, J, Ji)) * w;
+        const double dp = pq[q] - ox_exact_value(e, x, t, dim);
+        err += dp * dp * jxw;
+      }
+    }
+    free(w1);
+    free(w2);
+    free(pq);
+  }
+  return sqrt(err);
+}
+
+/* ------------------------------------------------------------------------ */
 /* ADER with local time stepping (WaveEquationOperationADERLTS,               */
 /* source/wave_equation_operations.cc:1385-1692, and ClusterManager,          */
 /* include/cluster_manager.templates.h:24-1049), restated per cell.           */
@@ -1412,16 +1666,23 @@ int ox_run(const ox_problem *P, char *log, int log_len, double *final_energy) {
   double *tmp = (double *)malloc(sizeof(double) * N);
   double *t1 = (double *)malloc(sizeof(double) * N);
   double *t2 = (double *)malloc(sizeof(double) * N);
+  double *post = P->post_pressure
+                     ? (double *)malloc(sizeof(double) * m->n_cells * ipow(k + 2, P->dim))
+                     : NULL;
   ox_project(m, k, &P->exact, 0.0, sol);
   int pos = 0;
   if (log && log_len > 0) log[0] = 0;
 #define OX_LOG()                                                                           \
   do {                                                                                     \
-    double err[3];                                                                         \
+    double err[3], epp = -1.0;                                                             \
     ox_l2_errors(m, k, &P->exact, time, sol, err);                                         \
+    if (P->post_pressure) {                                                                \
+      ox_post_pressure(m, k, sol, post);                                                   \
+      epp = ox_post_error(m, k, &P->exact, time, post);                                    \
+    }                                                                                      \
     if (log && pos < log_len)                                                              \
-      pos += snprintf(log + pos, log_len - pos, "%u %.17g %.10e %.10e %.10e\n", step, time, \
-                      err[0], err[1], err[2]);                                             \
+      pos += snprintf(log + pos, log_len - pos, "%u %.17g %.10e %.10e %.10e %.10e\n", step,  \
+                      time, err[0], err[1], err[2], epp);                                  \
   } while (0)
   OX_LOG();
   /* while (!done()) : time >= final_time || step > max_step (utilities.cc:84-87) */
@@ -1441,6 +1702,7 @@ int ox_run(const ox_problem *P, char *log, int log_len, double *final_energy) {
 #undef OX_LOG
   if (final_energy) *final_energy = ox_energy(m, k, sol);
   ox_lts_free(lts);
+  free(post);
   free(sol);
   free(tmp);
   free(t1);

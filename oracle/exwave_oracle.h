@@ -124,8 +124,8 @@ void ox_l2_errors(const ox_mesh *m, int degree, const ox_exact *e, double t, con
 double ox_energy(const ox_mesh *m, int degree, const double *u);
 
 /* Whole solve following WaveEquationProblem::run (source/explicit_wave.cc:549-813).
- * Writes one line per output tick into `log` (reference 'error' line format,
- * with 'error p post' reported as -1 because post-processing is not restated).
+ * Writes one line per output tick into `log`: step, time, error p, error v,
+ * solution mag p, error p post (-1 unless post_pressure is set).
  * Returns the number of steps performed. */
 typedef struct ox_problem {
   int dim, degree, n_intervals, n_refinements, transform;
@@ -138,10 +138,23 @@ typedef struct ox_problem {
   int ader_recon;
   ox_exact exact;
   int lts_max_clusters, lts_max_diff; /* max_n_clusters, max_diff_clusters */
+  int post_pressure;                   /* also log 'error p post' (else -1) */
 } ox_problem;
 int ox_run(const ox_problem *p, char *log, int log_len, double *final_state_energy);
 
 long ox_n_dofs(const ox_mesh *m, int degree);
+
+/* Post-processed pressure (compute_post_pressure,
+ * source/wave_equation_operations.cc:786-848): per cell the local Neumann
+ * problem (grad phi, grad p*) = (grad phi, -rho dv/dt) on Q_{k+1} (Gauss-Lobatto
+ * nodal basis, (k+2)^dim values per cell, QGauss(k+2)), its first unknown
+ * fixed, the constant set so the cell average equals that of p_h.
+ * post: n_cells * (k+2)^dim values. */
+void ox_gauss_lobatto(int n, double *x);
+void ox_post_pressure(const ox_mesh *m, int degree, const double *solution, double *post);
+/* ||p* - p|| with QGauss(k+3) (source/explicit_wave.cc:477-484) */
+double ox_post_error(const ox_mesh *m, int degree, const ox_exact *e, double t,
+                     const double *post);
 
 /* ADER with local time stepping: WaveEquationOperationADERLTS
  * (source/wave_equation_operations.cc:1385-1692) + ClusterManager
