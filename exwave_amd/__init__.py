@@ -142,6 +142,27 @@ class WaveOperator:
         check(load().exw_energy(self.h, _ptr(u), C.byref(out)))
         return out.value
 
+    @property
+    def n_post(self):
+        return load().exw_post_n_values(self.h)
+
+    def post_pressure(self, solution, tmp, post, iterations=None):
+        """compute_post_pressure: tmp <- apply(solution), post <- p* (n_post values,
+        (k+2)^dim Gauss-Lobatto nodal values per cell); iterations: optional
+        int32 device tensor of n_cells CG iteration counts."""
+        it = None
+        if iterations is not None:
+            import torch
+            assert iterations.dtype == torch.int32 and iterations.is_cuda
+            assert iterations.is_contiguous() and iterations.numel() >= self.n_cells
+            it = C.c_void_p(iterations.data_ptr())
+        check(load().exw_post_pressure(self.h, _ptr(solution), _ptr(tmp), _ptr(post), it))
+
+    def post_error(self, modes: Modes, t, post):
+        out = C.c_double(0.0)
+        check(load().exw_post_error(self.h, C.byref(modes), t, _ptr(post), C.byref(out)))
+        return out.value
+
 
 def lts_categorize(dim, vertices, neighbor, cfl, max_clusters, max_diff):
     """exw_lts_categorize (host only): (cluster ids, flags, n_clusters,
@@ -196,12 +217,14 @@ class LocalTimeStepping:
 
 
 def run(ctx: Context, problem: Problem):
-    """exw_run: returns (n_steps, [(step, time, err_p, err_v, mag_p)], wall_seconds)."""
+    """exw_run: returns (n_steps, [(step, time, err_p, err_v, mag_p)], wall_seconds);
+    with problem.post_pressure set the rows carry a sixth entry, error p post."""
     buf = C.create_string_buffer(1 << 16)
     wall = C.c_double(0.0)
     steps = check(load().exw_run(ctx.h, C.byref(problem), buf, len(buf), C.byref(wall)))
     rows = []
     for line in buf.value.decode().strip().splitlines():
-        s, t, ep, ev, mp = line.split()
-        rows.append((int(s), float(t), float(ep), float(ev), float(mp)))
+        s, t, ep, ev, mp, epp = line.split()
+        row = (int(s), float(t), float(ep), float(ev), float(mp))
+        rows.append(row + (float(epp),) if problem.post_pressure else row)
     return steps, rows, wall.value

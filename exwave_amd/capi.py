@@ -74,7 +74,8 @@ class Problem(C.Structure):
                 ("max_time_steps", C.c_int), ("final_time", C.c_double),
                 ("output_every_time", C.c_double), ("ader_recon", C.c_int),
                 ("precision", C.c_int), ("modes", Modes),
-                ("lts_max_clusters", C.c_int), ("lts_max_diff", C.c_int)]
+                ("lts_max_clusters", C.c_int), ("lts_max_diff", C.c_int),
+                ("post_pressure", C.c_int)]
 
 
 # (name, restype, argtypes) of every exported entry point
@@ -107,6 +108,9 @@ SIGNATURES = [
     ("exw_project", _I, [_VP, C.POINTER(Modes), _D, _VP]),
     ("exw_l2_errors", _I, [_VP, C.POINTER(Modes), _D, _VP, _DP]),
     ("exw_energy", _I, [_VP, _VP, _DP]),
+    ("exw_post_n_values", C.c_int64, [_VP]),
+    ("exw_post_pressure", _I, [_VP, _VP, _VP, _VP, _VP]),
+    ("exw_post_error", _I, [_VP, C.POINTER(Modes), _D, _VP, _DP]),
     ("exw_mesh_hypercube", _I, [_I, _I, _I, _I, _D, _I, _IP, _DP, _IP]),
     ("exw_run", _I, [_VP, C.POINTER(Problem), C.c_char_p, _I, _DP]),
     ("exw_server_create", _I, [_VP, C.c_size_t, C.POINTER(_VP)]),

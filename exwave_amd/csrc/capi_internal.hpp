@@ -25,7 +25,7 @@ struct exw_op_s {
   double minvd;
   int *d_neighbor = nullptr;
   double *d_rho = nullptr, *d_speed = nullptr, *d_h = nullptr, *d_vertices = nullptr;
-  double *d_tab = nullptr, *d_errtab = nullptr, *d_cktab = nullptr;
+  double *d_tab = nullptr, *d_errtab = nullptr, *d_cktab = nullptr, *d_posttab = nullptr;
   double *d_partial = nullptr, *d_sum = nullptr;
   int n_partial = 0;
   exw::CkTab ck{};
@@ -42,6 +42,11 @@ struct exw_op_s {
     m.h = d_h;
     m.vertices = d_vertices;
     return m;
+  }
+  int64_t n_post() const {  // post-processed pressure: (k+2)^dim values per cell
+    int64_t np = 1;
+    for (int d = 0; d < dim; ++d) np *= degree + 2;
+    return (int64_t)n_cells * np;
   }
   int64_t n_dofs() const {
     int64_t nd = 1;

@@ -114,9 +114,33 @@ static void free_op(exw_op op) {
   hipFree(op->d_tab);
   hipFree(op->d_errtab);
   hipFree(op->d_cktab);
+  hipFree(op->d_posttab);
   hipFree(op->d_partial);
   hipFree(op->d_sum);
   delete op;
+}
+
+// 1D matrices of the post-processing (layout PostTab<k>): QGauss(k+2) points
+// and weights, the Gauss-Lobatto(k+2) Lagrange basis and the QGauss(k+2)
+// collocation derivative at those points, the QGauss(k+1) (solution) basis at
+// them, and the QGauss(k+3) error rule with the Gauss-Lobatto basis there.
+static std::vector<double> build_post_table(int k) {
+  const int N = k + 1, P = k + 2, E = k + 3;
+  std::vector<double> t(2 * P + 2 * P * P + P * N + 2 * E + E * P);
+  double xg[16], wg[16], xl[16];
+  double *xq = t.data(), *wq = xq + P, *PV = wq + P, *DQ = PV + P * P, *IQ = DQ + P * P,
+         *xe = IQ + P * N, *we = xe + E, *IE = we + E;
+  cx::gauss01(N, xg, wg);
+  cx::gauss01(P, xq, wq);
+  cx::gauss_lobatto01(P, xl);
+  cx::gauss01(E, xe, we);
+  for (int q = 0; q < P; ++q) {
+    cx::lagrange(P, xl, xq[q], PV + q * P, nullptr);
+    cx::lagrange(P, xq, xq[q], nullptr, DQ + q * P);
+    cx::lagrange(N, xg, xq[q], IQ + q * N, nullptr);
+  }
+  for (int q = 0; q < E; ++q) cx::lagrange(P, xl, xe[q], IE + q * P, nullptr);
+  return t;
 }
 
 extern "C++" {
@@ -202,6 +226,13 @@ int exw_op_create(exw_ctx ctx, int dim, int degree, int n_cells, const double *v
       (rc = upload(op, &op->d_cktab, ckt.data(), ckt.size()))) {
     free_op(op);
     return rc;
+  }
+  {
+    const std::vector<double> pt = build_post_table(degree);
+    if ((rc = upload(op, &op->d_posttab, pt.data(), pt.size()))) {
+      free_op(op);
+      return rc;
+    }
   }
   op->n_partial = 1024;
   if (hipMalloc((void **)&op->d_partial, sizeof(double) * 3 * op->n_partial) != hipSuccess ||
@@ -448,6 +479,41 @@ int exw_l2_errors(exw_op op, const exw_modes *modes, double t, const double *u,
   EXW_HIP(hipMemcpyAsync(s, op->d_sum, sizeof(s), hipMemcpyDeviceToHost, op->ctx->stream));
   EXW_HIP(hipStreamSynchronize(op->ctx->stream));
   for (int k = 0; k < 3; ++k) out_host[k] = std::sqrt(s[k]);
+  return EXW_OK;
+}
+
+int64_t exw_post_n_values(exw_op op) { return op ? op->n_post() : -1; }
+
+int exw_post_pressure(exw_op op, const double *solution, double *tmp, double *post,
+                      int *iterations) {
+  if (!op || !solution || !tmp || !post)
+    return fail(EXW_EINVAL, "exw_post_pressure: null argument");
+  if (tmp == solution) return fail(EXW_EINVAL, "exw_post_pressure: tmp aliases solution");
+  PostLaunch l = kernels(op->dim).post[op->degree];
+  if (!l) return fail(EXW_EUNSUP, "post-processing kernel not compiled");
+  // compute_post_pressure calls apply(solution, tmp_vector) first (:795)
+  if (int rc = exw_op_apply(op, solution, tmp, EXW_FP64)) return rc;
+  int np = 1;
+  for (int d = 0; d < op->dim; ++d) np *= op->degree + 2;
+  const PostSolver cfg{EXW_POST_ABS_TOL, EXW_POST_REL_TOL, EXW_POST_MAX_ITER_FACTOR * np};
+  EXW_HIP(l(op->view(), op->d_posttab, solution, tmp, post, iterations, cfg, op->ctx->stream));
+  return EXW_OK;
+}
+
+int exw_post_error(exw_op op, const exw_modes *modes, double t, const double *post,
+                   double *out_host) {
+  if (!op || !modes || !post || !out_host) return fail(EXW_EINVAL, "exw_post_error: null argument");
+  if (modes->n_modes < 0 || modes->n_modes > 8) return fail(EXW_EINVAL, "n_modes must be 0..8");
+  const int nb = op->n_partial;
+  EXW_HIP(kernels(op->dim).post_error[op->degree](op->view(), op->d_posttab, to_dev(modes), t,
+                                                   post, op->d_partial, nb, op->ctx->stream));
+  hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(256), 0, op->ctx->stream, op->d_partial,
+                     nb, 1, op->d_sum);
+  EXW_HIP(hipGetLastError());
+  double s = 0.0;
+  EXW_HIP(hipMemcpyAsync(&s, op->d_sum, sizeof(double), hipMemcpyDeviceToHost, op->ctx->stream));
+  EXW_HIP(hipStreamSynchronize(op->ctx->stream));
+  *out_host = std::sqrt(s);
   return EXW_OK;
 }
 

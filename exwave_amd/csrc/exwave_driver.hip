@@ -144,13 +144,26 @@ int exw_run(exw_ctx ctx, const exw_problem *p, char *log, int log_len, double *w
   }
   int pos = 0;
   if (log && log_len > 0) log[0] = 0;
+  double *post = nullptr;
+  if (p->post_pressure && (rc = exw_alloc(ctx, sizeof(double) * exw_post_n_values(op),
+                                          (void **)&post))) {
+    exw_lts_destroy(L);
+    exw_free(ctx, buf);
+    exw_op_destroy(op);
+    return rc;
+  }
   auto output = [&]() -> int {
-    double err[3];
+    double err[3], epp = -1.0;
     int r = exw_l2_errors(op, &p->modes, time, sol, err);
     if (r) return r;
+    // output_results: compute_post_pressure into tmp_solutions (explicit_wave.cc:476);
+    // tmp is free between steps (the next step swaps it with sol first)
+    if (post && ((r = exw_post_pressure(op, sol, tmp, post, nullptr)) ||
+                 (r = exw_post_error(op, &p->modes, time, post, &epp))))
+      return r;
     if (log && pos < log_len)
-      pos += std::snprintf(log + pos, log_len - pos, "%u %.17g %.10e %.10e %.10e\n", step, time,
-                           err[0], err[1], err[2]);
+      pos += std::snprintf(log + pos, log_len - pos, "%u %.17g %.10e %.10e %.10e %.10e\n", step,
+                           time, err[0], err[1], err[2], epp);
     return 0;
   };
   double wall = 0.0;
@@ -173,6 +186,7 @@ int exw_run(exw_ctx ctx, const exw_problem *p, char *log, int log_len, double *w
     }
   }
   exw_lts_destroy(L);
+  if (post) exw_free(ctx, post);
   exw_free(ctx, buf);
   exw_op_destroy(op);
   if (wall_seconds) *wall_seconds = wall;

@@ -51,6 +51,31 @@ constexpr void gauss01(int n, double *x, double *w) {
   }
 }
 
+// n-point Gauss-Lobatto points on [0,1], ascending: the ends and the roots of
+// P'_{n-1} (Newton iteration on P'_{n-1} from the Chebyshev-Lobatto points)
+constexpr void gauss_lobatto01(int n, double *x) {
+  const int m = n - 1;
+  x[0] = 0.0;
+  x[n - 1] = 1.0;
+  for (int i = 1; i < n - 1; ++i) {
+    double z = -cos_(3.14159265358979323846 * i / m);
+    for (int it = 0; it < 100; ++it) {
+      double p0 = 1.0, p1 = z;
+      for (int j = 2; j <= m; ++j) {
+        const double p2 = ((2.0 * j - 1.0) * z * p1 - (j - 1.0) * p0) / j;
+        p0 = p1;
+        p1 = p2;
+      }
+      const double d1 = m * (z * p1 - p0) / (z * z - 1.0);              // P'_m
+      const double d2 = (2.0 * z * d1 - m * (m + 1.0) * p1) / (1.0 - z * z);  // P''_m
+      const double dz = d1 / d2;
+      z -= dz;
+      if (fabs_(dz) < 1e-17) break;
+    }
+    x[i] = 0.5 * (1.0 + z);
+  }
+}
+
 // barycentric Lagrange basis on nodes[0..n): values l_i(x) and derivatives l_i'(x)
 constexpr void lagrange(int n, const double *nodes, double x, double *val, double *der) {
   double bw[16] = {};

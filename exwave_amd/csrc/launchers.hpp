@@ -6,6 +6,7 @@
 #include "dg_misc.hpp"
 #include "dg_operator.hpp"
 #include "dg_operator_cart.hpp"
+#include "dg_post.hpp"
 
 namespace exw {
 
@@ -27,6 +28,12 @@ using ErrLaunch = hipError_t (*)(const DevMesh &, const double *, const DevModes
                                  const double *, double *, int, hipStream_t);
 using EnergyLaunch = hipError_t (*)(const DevMesh &, const double *, const double *, double *,
                                     int, hipStream_t);
+// (mesh, post table, solution, apply(solution), post, iterations, solver, stream)
+using PostLaunch = hipError_t (*)(const DevMesh &, const double *, const double *, const double *,
+                                  double *, int *, const PostSolver &, hipStream_t);
+// (mesh, post table, modes, t, post, partials, n blocks, stream)
+using PostErrLaunch = hipError_t (*)(const DevMesh &, const double *, const DevModes &, double,
+                                     const double *, double *, int, hipStream_t);
 
 struct DimKernels {
   OpLaunch op[MAX_DEGREE + 1][2][2];  // [k][fp32][cart]
@@ -35,6 +42,8 @@ struct DimKernels {
   ProjLaunch project[MAX_DEGREE + 1];
   ErrLaunch l2err[MAX_DEGREE + 1];
   EnergyLaunch energy[MAX_DEGREE + 1];
+  PostLaunch post[MAX_DEGREE + 1];
+  PostErrLaunch post_error[MAX_DEGREE + 1];
   int block[MAX_DEGREE + 1];
   int cells_per_block[MAX_DEGREE + 1];
 };
@@ -112,6 +121,25 @@ hipError_t launch_energy(const DevMesh &m, const double *tab, const double *u, d
 }
 
 template <int DIM, int K>
+hipError_t launch_post(const DevMesh &m, const double *tab, const double *sol, const double *dudt,
+                       double *post, int *iterations, const PostSolver &cfg, hipStream_t s) {
+  using S = PostShape<DIM, K>;
+  const int grid = (m.n_cells + S::CPB - 1) / S::CPB;
+  if (grid == 0) return hipSuccess;
+  hipLaunchKernelGGL((post_pressure_kernel<DIM, K>), dim3(grid), dim3(S::BLOCK), 0, s, m, tab,
+                     sol, dudt, post, iterations, cfg);
+  return hipGetLastError();
+}
+
+template <int DIM, int K>
+hipError_t launch_post_error(const DevMesh &m, const double *tab, const DevModes &md, double t,
+                             const double *post, double *partial, int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL((post_error_kernel<DIM, K>), dim3(nblocks), dim3(256), 0, s, m, tab, md, t,
+                     post, partial);
+  return hipGetLastError();
+}
+
+template <int DIM, int K>
 void fill_dim_kernels(DimKernels &t) {
   if constexpr (K >= 1) {
     t.op[K][0][0] = launch_op<DIM, K, double, false>;
@@ -126,6 +154,8 @@ void fill_dim_kernels(DimKernels &t) {
     t.project[K] = launch_project<DIM, K>;
     t.l2err[K] = launch_l2err<DIM, K>;
     t.energy[K] = launch_energy<DIM, K>;
+    t.post[K] = launch_post<DIM, K>;
+    t.post_error[K] = launch_post_error<DIM, K>;
     t.block[K] = Shape<DIM, K>::BLOCK;
     t.cells_per_block[K] = Shape<DIM, K>::CPB;
     fill_dim_kernels<DIM, K - 1>(t);

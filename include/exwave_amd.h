@@ -214,6 +214,32 @@ int exw_lts_step(exw_lts L, double time_now, const double *src, double *dst);
 int exw_mesh_hypercube(int dim, int n_intervals, int n_refinements, int transform, double factor,
                        int boundary_id, int *n_cells, double *vertices, int *neighbor);
 
+/* ---- post-processed pressure ----------------------------------------------
+ * WaveEquationOperation::compute_post_pressure
+ * (source/wave_equation_operations.cc:786-848): tmp <- apply(solution) (the
+ * reference's tmp_vector, n_dofs values), then per cell the local Neumann
+ * problem (grad phi, grad p*) = (grad phi, -rho v_t) on Q_{k+1}, with the
+ * constant set so the cell average of p* equals that of p.  post receives
+ * exw_post_n_values(op) = n_cells * (k+2)^dim values, per cell the nodal values
+ * at the (k+2)^dim Gauss-Lobatto points (the reference's fe_post_disp,
+ * source/explicit_wave.cc:165), xi_0 fastest.  iterations (device, n_cells
+ * ints, may be NULL) receives each cell's CG iteration count.
+ * The local CG runs on the zero-sum subspace of the Neumann problem (same p*
+ * as the reference's first-unknown constraint, far better conditioned) and
+ * stops per cell at |r| < EXW_POST_ABS_TOL, |r|/|r0| < EXW_POST_REL_TOL or
+ * after EXW_POST_MAX_ITER_FACTOR * (k+2)^dim + 1 iterations (the reference:
+ * 1e-20, 1e-11, 6 n, :798-799).  Asynchronous on the context's stream. */
+#define EXW_POST_ABS_TOL 1e-20
+#define EXW_POST_REL_TOL 1e-13
+#define EXW_POST_MAX_ITER_FACTOR 6
+int64_t exw_post_n_values(exw_op op);
+int exw_post_pressure(exw_op op, const double *solution, double *tmp, double *post,
+                      int *iterations);
+/* ||p* - p_exact||_L2 with QGauss(k+3) (the 'error p post' of
+ * source/explicit_wave.cc:477-484); synchronises. */
+int exw_post_error(exw_op op, const exw_modes *modes, double t, const double *post,
+                   double *err_host);
+
 /* Parameters of one solve (the keys of the reference's prm files that the
  * hot path uses, source/parameters.cc:36-100). */
 typedef struct exw_problem {
@@ -228,13 +254,14 @@ typedef struct exw_problem {
   int precision;       /* EXW_FP64 / EXW_FP32 operator */
   exw_modes modes;     /* initial condition and exact solution */
   int lts_max_clusters, lts_max_diff; /* EXW_ADER_LTS: max_n_clusters, max_diff_clusters */
+  int post_pressure;   /* also report 'error p post' at every tick (else -1) */
 } exw_problem;
 
 /* WaveEquationProblem<dim>::run (source/explicit_wave.cc:549-813): mesh,
  * operator, projection, TimeControl with the reference's dt rules
  * (source/utilities.cc:31-95, source/explicit_wave.cc:222), the time loop and
  * the L2 errors at every output tick.  Writes one line per tick
- *   "<step> <time> <error p> <error v> <solution mag p>\n"
+ *   "<step> <time> <error p> <error v> <solution mag p> <error p post>\n"
  * into log (if not NULL).  Returns the number of steps (>= 0) or an error
  * code.  wall_seconds (optional) receives the time-loop wall time excluding
  * the error evaluations. */

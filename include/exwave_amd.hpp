@@ -175,6 +175,14 @@ class WaveEquationOperationBase {
   virtual exw_op handle() const = 0;
   virtual const Context &context() const = 0;
   virtual int precision() const = 0;
+  // compute_post_pressure (source/wave_equation_operations.cc:786-848):
+  // tmp_vector <- apply(solution), post_pressure <- p* ((k+2)^dim Gauss-Lobatto
+  // nodal values per cell, exw_post_n_values)
+  virtual void compute_post_pressure(const DeviceVector &solution, DeviceVector &tmp_vector,
+                                     DeviceVector &post_pressure) const {
+    check(exw_post_pressure(handle(), solution.data(), tmp_vector.data(), post_pressure.data(),
+                            nullptr));
+  }
   virtual unsigned int cluster_id(unsigned int) const { return (unsigned)-1; }
   virtual value_type time_step(unsigned int) const { return get_time_control().get_time_step(); }
 };

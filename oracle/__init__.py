@@ -116,6 +116,7 @@ def lib():
         L.ox_lts_step.argtypes = [C.c_void_p, C.c_double, dp, dp]
         L.ox_gauss_lobatto.argtypes = [C.c_int, dp]
         L.ox_post_pressure.argtypes = [C.POINTER(_Mesh), C.c_int, dp, dp]
+        L.ox_post_set_solver.argtypes = [C.c_double, C.c_double, C.c_int, C.c_void_p, C.c_int]
         L.ox_post_error.restype = C.c_double
         L.ox_post_error.argtypes = [C.POINTER(_Mesh), C.c_int, C.POINTER(Exact), C.c_double, dp]
     return _lib
@@ -208,11 +209,21 @@ class Mesh:
     def energy(self, degree, u):
         return lib().ox_energy(self._m, degree, _p(u))
 
-    def post_pressure(self, degree, u):
-        """compute_post_pressure: n_cells x (k+2)^dim Gauss-Lobatto nodal values."""
+    def post_pressure(self, degree, u, rel_tol=1e-11, abs_tol=1e-20, max_iter_factor=6,
+                      iterations=False, mean_free=False):
+        """compute_post_pressure: n_cells x (k+2)^dim Gauss-Lobatto nodal values
+        (and each cell's CG iteration count if `iterations`); mean_free solves
+        the same local problem on the zero-sum subspace instead of fixing the
+        first unknown."""
         out = np.zeros(self.n_cells * (degree + 2) ** self.dim)
-        lib().ox_post_pressure(self._m, degree, _p(u), _p(out))
-        return out
+        its = np.zeros(self.n_cells, dtype=np.int32)
+        L = lib()
+        L.ox_post_set_solver(abs_tol, rel_tol, max_iter_factor, its.ctypes.data, int(mean_free))
+        try:
+            L.ox_post_pressure(self._m, degree, _p(u), _p(out))
+        finally:
+            L.ox_post_set_solver(1e-20, 1e-11, 6, None, 0)
+        return (out, its) if iterations else out
 
     def post_error(self, degree, exact, t, post):
         return lib().ox_post_error(self._m, degree, C.byref(exact), t, _p(post))

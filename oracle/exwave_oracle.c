@@ -8,6 +8,7 @@
  */
 #include "exwave_oracle.h"
 
+#include <float.h>
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -985,6 +986,22 @@ static void tensor_grad_t(int dim, int np, int nq, const double *Pv, const doubl
   }
 }
 
+/* per-cell CG tolerances of compute_post_pressure (:798-799): absolute,
+ * relative, and the iteration cap as a multiple of the unknowns */
+static double ox_post_tol[2] = {1e-20, 1e-11};
+static int ox_post_max_iter_factor = 6;
+static int *ox_post_iterations = NULL;
+static int ox_post_mode = 0;
+
+void ox_post_set_solver(double abs_tol, double rel_tol, int max_iter_factor, int *iterations,
+                        int mode) {
+  ox_post_mode = mode;
+  ox_post_tol[0] = abs_tol;
+  ox_post_tol[1] = rel_tol;
+  ox_post_max_iter_factor = max_iter_factor;
+  ox_post_iterations = iterations;
+}
+
 void ox_post_pressure(const ox_mesh *m, int degree, const double *solution, double *post) {
   const int dim = m->dim, n = degree + 1, np = degree + 2, nq = degree + 2, nf = dim + 1;
   const int nd = ipow(n, dim), npd = ipow(np, dim), nqd = ipow(nq, dim), nv = 1 << dim;
@@ -1046,6 +1063,7 @@ void ox_post_pressure(const ox_mesh *m, int degree, const double *solution, doub
         }
       }
       tensor_grad_t(dim, np, nq, Pv, Pd, fr, rhs, w1, w2);
+      const double rhs0 = rhs[0];
       rhs[0] = 0.0; /* :808-811 */
       /* CG on the Laplacian with the first unknown fixed (the reference's
        * MatrixPostProcessing::vmult, :750-769) */
@@ -1066,30 +1084,72 @@ void ox_post_pressure(const ox_mesh *m, int degree, const double *solution, doub
     (out)[0] = keep;                                                                     \
     (in)[0] = keep;                                                                      \
   } while (0)
-      double rr = 0.0, r0;
+      /* SolverCGvect::solve (include/elementwise_cg.templates.h:66-145) for one
+       * cell: zero start, identity preconditioner, stop on |r| < abs_tol,
+       * |r|/|r0| < rel_tol or after max_iter iterations.  Mode 1 instead runs
+       * CG on the Neumann problem itself, restricted to vectors with zero sum
+       * (the matrix's null space is the constant vector): the same p* after
+       * the average adjustment below, with a far smaller condition number. */
+      const int mean_free = ox_post_mode == 1;
+      if (mean_free) {
+        rhs[0] = rhs0;
+        double mean = 0.0;
+        for (int i = 0; i < npd; ++i) mean += rhs[i];
+        mean /= npd;
+        for (int i = 0; i < npd; ++i) rhs[i] -= mean;
+      }
+#define OX_POST_NEUMANN(in, out)                                                         \
+  do {                                                                                   \
+    tensor_grad(dim, np, nq, Pv, Pd, in, gr, w1, w2);                                    \
+    for (int q = 0; q < nqd; ++q) {                                                      \
+      double g[3];                                                                       \
+      for (int e = 0; e < dim; ++e) {                                                    \
+        g[e] = 0.0;                                                                      \
+        for (int e2 = 0; e2 < dim; ++e2) g[e] += G[((long)e * dim + e2) * nqd + q] * gr[(long)e2 * nqd + q]; \
+      }                                                                                  \
+      for (int e = 0; e < dim; ++e) fr[(long)e * nqd + q] = g[e];                        \
+    }                                                                                    \
+    tensor_grad_t(dim, np, nq, Pv, Pd, fr, out, w1, w2);                                 \
+  } while (0)
+      double rr = 0.0;
       for (int i = 0; i < npd; ++i) {
         u[i] = 0.0;
         r[i] = rhs[i];
         pp[i] = r[i];
         rr += r[i] * r[i];
       }
-      r0 = rr;
-      for (int it = 0; it < 20 * npd && rr > 1e-30 * r0 && rr > 1e-60; ++it) {
-        OX_POST_APPLY(pp, Ap);
+      const double nr0 = sqrt(rr);
+      int it = 0;
+      while (1) {
+        if (mean_free)
+          OX_POST_NEUMANN(pp, Ap);
+        else
+          OX_POST_APPLY(pp, Ap);
         double pAp = 0.0;
         for (int i = 0; i < npd; ++i) pAp += pp[i] * Ap[i];
+        /* adjust_division_by_zero (include/elementwise_cg.templates.h:44-61) */
+        if (pAp < fmax(1e-5 * ox_post_tol[0] * ox_post_tol[0], 1e4 * DBL_MIN)) pAp = 1.0;
         const double alpha = rr / pAp;
-        double rn = 0.0;
+        double rn = 0.0, rs = 0.0;
         for (int i = 0; i < npd; ++i) {
           u[i] += alpha * pp[i];
           r[i] -= alpha * Ap[i];
-          rn += r[i] * r[i];
+          rs += r[i];
         }
+        if (mean_free) /* keep the residual in the range of the matrix */
+          for (int i = 0; i < npd; ++i) r[i] -= rs / npd;
+        for (int i = 0; i < npd; ++i) rn += r[i] * r[i];
+        ++it;
+        if (sqrt(rn) < ox_post_tol[0] || sqrt(rn) < ox_post_tol[1] * nr0 ||
+            it > ox_post_max_iter_factor * npd)
+          break;
         const double beta = rn / rr;
         rr = rn;
         for (int i = 0; i < npd; ++i) pp[i] = r[i] + beta * pp[i];
       }
+      if (ox_post_iterations) ox_post_iterations[c] = it;
 #undef OX_POST_APPLY
+#undef OX_POST_NEUMANN
       /* constant offset from the cell average of the pressure (:813-837) */
       double vol = 0.0, correct = 0.0, prelim = 0.0;
       tensor_interp(dim, np, nq, Pv, u, gr, w1, w2);

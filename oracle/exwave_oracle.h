@@ -152,6 +152,12 @@ long ox_n_dofs(const ox_mesh *m, int degree);
  * post: n_cells * (k+2)^dim values. */
 void ox_gauss_lobatto(int n, double *x);
 void ox_post_pressure(const ox_mesh *m, int degree, const double *solution, double *post);
+/* the per-cell CG's stopping rule (default: the reference's 1e-20, 1e-11, 6 n);
+ * iterations, if not NULL, receives each cell's iteration count; mode 0 is the
+ * reference's first-unknown constraint, mode 1 CG on the zero-sum subspace of
+ * the Neumann problem (same p*, better conditioned) */
+void ox_post_set_solver(double abs_tol, double rel_tol, int max_iter_factor, int *iterations,
+                        int mode);
 /* ||p* - p|| with QGauss(k+3) (source/explicit_wave.cc:477-484) */
 double ox_post_error(const ox_mesh *m, int degree, const ox_exact *e, double t,
                      const double *post);

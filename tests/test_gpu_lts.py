@@ -73,7 +73,7 @@ def test_lts_steps_match_oracle(X, ctx, case, recon):
 
 def test_lts_fixture_end_to_end(X, ctx, fixtures):
     """exw_run with the reference's aderlts_2d_recon parameters reproduces every
-    'error' line (4 printed digits) and the step count."""
+    'error' line (4 printed digits, error p post included) and the step count."""
     fx = fixtures["aderlts_2d_recon"]
     prm = fx["params"]
     op_ = _problem(prm)
@@ -85,9 +85,11 @@ def test_lts_fixture_end_to_end(X, ctx, fixtures):
         setattr(p, name, getattr(op_, name))
     p.precision = X.FP64
     p.modes = X.Modes.make(prm["dimension"], [[prm["membrane_modes"]] * prm["dimension"]])
+    p.post_pressure = 1
     steps, rows, _ = X.run(ctx, p)
     assert steps == fx["n_steps"]
     assert len(rows) == len(fx["error_lines"])
-    for (st, t, ep, ev, mp), ref in zip(rows, fx["error_lines"]):
+    for (st, t, ep, ev, mp, epp), ref in zip(rows, fx["error_lines"]):
         assert "%.2f" % t == ref[0]
-        assert ["%.4e" % ep, "%.4e" % ev, "%.4e" % mp] == [ref[1], ref[3], ref[4]], (t, ref)
+        got = ["%.4e" % ep, "%.4e" % epp, "%.4e" % ev, "%.4e" % mp]
+        assert got == ref[1:5], (t, got, ref)

@@ -212,7 +212,8 @@ FIXTURES = ["lsrk45r2_2d_cartesian", "clrk4_3d", "ader_2d_norecon_ref2", "ader_2
 @pytest.mark.parametrize("name", FIXTURES)
 def test_reference_fixture_on_gpu(X, ctx, fixtures, name):
     """The reference's regression tests run end to end on the GPU (exw_run):
-    every 'error' line at 4 printed digits, number of steps."""
+    every 'error' line (error p, error p post, error v, solution mag p) at 4
+    printed digits, number of steps."""
     _say("fixture on gpu", name)
     fx = fixtures[name]
     prm = fx["params"]
@@ -227,11 +228,14 @@ def test_reference_fixture_on_gpu(X, ctx, fixtures, name):
     p.ader_recon = 1 if prm.get("use_ader_post") else 0
     p.precision = X.FP64
     p.modes = X.Modes.reference(p.dim, prm["membrane_modes"])
+    p.post_pressure = 1
     steps, rows, _ = X.run(ctx, p)
     assert steps == fx["n_steps"]
-    for (st, t, ep, ev, mp), ref in zip(rows, fx["error_lines"]):
+    assert len(rows) == len(fx["error_lines"])
+    for (st, t, ep, ev, mp, epp), ref in zip(rows, fx["error_lines"]):
         assert "%.2f" % t == ref[0]
-        assert ["%.4e" % ep, "%.4e" % ev, "%.4e" % mp] == [ref[1], ref[3], ref[4]], (t, ref)
+        got = ["%.4e" % ep, "%.4e" % epp, "%.4e" % ev, "%.4e" % mp]
+        assert got == ref[1:5], (t, got, ref)
 
 
 def test_error_paths(X, ctx):
