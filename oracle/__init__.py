@@ -16,9 +16,10 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _SO = os.path.join(_HERE, "_build", "libexwave_oracle.so")
 
-EXPL_EULER, CLRK4, LSRK45R2, LSRK33R2, LSRK59R2, ADER, ADER_LTS = range(7)
+EXPL_EULER, CLRK4, LSRK45R2, LSRK33R2, LSRK59R2, ADER, ADER_LTS, LSRK45R3, SSPRK, ADER_ADCONFULL = range(10)
 INTEGRATORS = {"ExplEuler": EXPL_EULER, "clRK4": CLRK4, "LSRK45R2": LSRK45R2,
-               "LSRK33R2": LSRK33R2, "LSRK59R2": LSRK59R2, "ADER": ADER, "ADERLTS": ADER_LTS}
+               "LSRK33R2": LSRK33R2, "LSRK59R2": LSRK59R2, "ADER": ADER, "ADERLTS": ADER_LTS,
+               "LSRK45R3": LSRK45R3, "SSPRK": SSPRK, "ADERADCONFULL": ADER_ADCONFULL}
 TRANSFORM_NONE, TRANSFORM_REFERENCE_SINE, TRANSFORM_AXIS_2PI, TRANSFORM_PRODUCT_2PI = range(4)
 
 

@@ -687,6 +687,35 @@ void ox_apply_ader(const ox_mesh *m, int degree, const double *src, double *dst,
 /* time integrators (source/time_integrators.cc)                             */
 /* ------------------------------------------------------------------------ */
 
+/* Shu-Osher coefficients of the optimal SSPRK with 8 stages and order 4
+ * (S. J. Ruuth, Math. Comp. 75 (2006) 183-207), as tabulated in
+ * SSPRK::initialize_coeffs (source/time_integrators.cc:726-858); row s-1
+ * forms stage s from stages 0..s-1 */
+static const double ox_ssprk84_a[8][8] = {
+    {1.},
+    {0.538569155333175, 0.461430844666825},
+    {0.004485387460763, 0., 0.995514612539237},
+    {0.164495299288580, 0.016875060685979, 0., 0.818629640025440},
+    {0.426933682982668, 0.157047028197878, 0.023164224070770, 0., 0.392855064748685},
+    {0.082083400476958, 0.000000039091042, 0.033974171137350, 0.005505195713107, 0.,
+     0.878437193581543},
+    {0.006736365648625, 0.010581829625529, 0.009353386191951, 0.101886062556838,
+     0.000023428364930, 0., 0.871418927612128},
+    {0.071115287415749, 0.018677648343953, 0.007902408660034, 0.319384027162348,
+     0.007121989995845, 0.001631615692736, 0., 0.574167022729334}};
+static const double ox_ssprk84_b[8][8] = {
+    {0.282318339066479},
+    {0., 0.130270389660380},
+    {0.003963092203460, 0., 0.281052031928487},
+    {0.000038019518678, 0.004764139104512, 0., 0.231114160282572},
+    {0.000019921336144, 0.044337256156151, 0.006539685265423, 0., 0.110910189373703},
+    {0.000000034006679, 0.000000011036118, 0.009591531566657, 0.001554217709960, 0.,
+     0.247998929466160},
+    {0.013159891155054, 0.002987444564164, 0.002640632454359, 0.028764303955070,
+     0.000006614257074, 0., 0.246017544274548},
+    {0.000000010647874, 0.005273042658132, 0.002230994887525, 0.090167968072837,
+     0.002010668386475, 0.000460635032368, 0., 0.162097880203691}};
+
 static void lsrk_2reg(const ox_mesh *m, int degree, int S, const double *a, const double *bcoef,
                       double dt, double *vec_n, double *vec_np, double *tmp1) {
   long N = ox_n_dofs(m, degree);
@@ -759,6 +788,80 @@ void ox_time_step(const ox_mesh *m, int degree, int integrator, double dt, doubl
                          393957816125. / 7825732611452.,   720647959663. / 6565743875477.,
                          3559252274877. / 14424734981077.};
     lsrk_2reg(m, degree, 9, a, b, dt, vec_n, vec_np, tmp1);
+  } else if (integrator == OX_LSRK45R3) {
+    /* LowStorageRK45Reg3::perform_time_step (:366-425), its vector operations
+     * in order; vec_n is overwritten (stage 3 writes into it) */
+    const double a21 = 2365592473904. / 8146167614645., a32 = 4278267785271. / 6823155464066.,
+                 a43 = 2789585899612. / 8986505720531., a54 = 15310836689591. / 24358012670437.,
+                 a31 = -722262345248. / 10870640012513., a42 = 1365858020701. / 8494387045469.,
+                 a53 = 3819021186. / 2763618202291.;
+    const double b1 = 846876320697. / 6523801458457., b2 = 3032295699695. / 12397907741132.,
+                 b3 = 612618101729. / 6534652265123., b4 = 1155491934595. / 2954287928812.,
+                 b5 = 707644755468. / 5028292464395.;
+    const double h = dt;
+    ox_apply(m, degree, vec_n, vec_np, 1.0);
+    for (long i = 0; i < N; ++i) {
+      vec_n[i] += a21 * h * vec_np[i];
+      tmp1[i] = vec_n[i] + (b1 - a21) * h * vec_np[i];
+    }
+    ox_apply(m, degree, vec_n, tmp2, 1.0);
+    for (long i = 0; i < N; ++i) {
+      tmp1[i] += a32 * h * tmp2[i] + (a31 - b1) * h * vec_np[i];
+      vec_np[i] = (b1 - a31) * h * vec_np[i] + tmp1[i];
+      vec_np[i] += (b2 - a32) * h * tmp2[i];
+    }
+    ox_apply(m, degree, tmp1, vec_n, 1.0);
+    for (long i = 0; i < N; ++i) {
+      vec_np[i] += a43 * h * vec_n[i] + (a42 - b2) * h * tmp2[i];
+      tmp2[i] = (b2 - a42) * h * tmp2[i] + vec_np[i];
+      tmp2[i] += (b3 - a43) * h * vec_n[i];
+    }
+    ox_apply(m, degree, vec_np, tmp1, 1.0);
+    for (long i = 0; i < N; ++i) {
+      tmp2[i] += a54 * h * tmp1[i] + (a53 - b3) * h * vec_n[i];
+      vec_np[i] = tmp2[i];
+      vec_np[i] += (b3 - a53) * h * vec_n[i] + (b4 - a54) * h * tmp1[i];
+    }
+    ox_apply(m, degree, tmp2, tmp1, 1.0);
+    for (long i = 0; i < N; ++i) vec_np[i] += b5 * h * tmp1[i];
+  } else if (integrator == OX_SSPRK) {
+    /* SSPRK::perform_time_step (:440-479) with order 4, 8 stages: u_0 = vec_n,
+     * u_s = sum_{l<s} A[s-1][l] u_l + dt B[s-1][l] F(u_l), vec_np = u_8 */
+    const int S = 8;
+    double *u = (double *)malloc(sizeof(double) * N * (2 * S + 1)), *F = u + N * (S + 1);
+    memcpy(u, vec_n, sizeof(double) * N);
+    for (int st = 1; st <= S; ++st) {
+      ox_apply(m, degree, u + (long)(st - 1) * N, F + (long)(st - 1) * N, 1.0);
+      double *us = u + (long)st * N;
+      for (long i = 0; i < N; ++i) us[i] = 0.0;
+      for (int l = 0; l < st; ++l) {
+        const double al = ox_ssprk84_a[st - 1][l], bl = ox_ssprk84_b[st - 1][l] * dt;
+        const double *ul = u + (long)l * N, *Fl = F + (long)l * N;
+        for (long i = 0; i < N; ++i) us[i] += al * ul[i] + bl * Fl[i];
+      }
+    }
+    memcpy(vec_np, u + (long)S * N, sizeof(double) * N);
+    free(u);
+  } else if (integrator == OX_ADER_ADCONFULL) {
+    /* WaveEquationOperationADERADCONFULL::apply_ader (wave_equation_operations.cc:
+     * 689-735): the Taylor series of the full operator; then ArbitraryHighOrderDG
+     * (time_integrators.cc:37-50) vec_np = vec_n - apply_ader(vec_n) */
+    double *tempsrc = tmp1, *tempvals = tmp2;
+    ox_apply(m, degree, vec_n, vec_np, 1.0);
+    for (long i = 0; i < N; ++i) {
+      tempsrc[i] = -dt * vec_np[i];
+      tempvals[i] = -vec_np[i];
+    }
+    double fac = -0.5 * dt * dt;
+    for (int k = 1; k <= degree; ++k) {
+      ox_apply(m, degree, tempvals, vec_np, 1.0);
+      for (long i = 0; i < N; ++i) {
+        tempsrc[i] += -fac * vec_np[i];
+        tempvals[i] = -vec_np[i];
+      }
+      fac *= -dt / (k + 2);
+    }
+    for (long i = 0; i < N; ++i) vec_np[i] = -tempsrc[i] + vec_n[i];
   } else if (integrator == OX_ADER) {
     /* ArbitraryHighOrderDG (:37-50): vec_np = vec_n - apply_ader(vec_n);
      * tmp1 and tmp2 must be adjacent (2N scratch) -> use both explicitly */

@@ -102,7 +102,10 @@ enum {
   OX_LSRK33R2 = 3,
   OX_LSRK59R2 = 4,
   OX_ADER = 5,
-  OX_ADER_LTS = 6 /* ox_run only: ADER with local time stepping */
+  OX_ADER_LTS = 6, /* ox_run only: ADER with local time stepping */
+  OX_LSRK45R3 = 7,
+  OX_SSPRK = 8,        /* SSPRK(8 stages, order 4), the reference's 'SSPRK' */
+  OX_ADER_ADCONFULL = 9
 };
 void ox_time_step(const ox_mesh *m, int degree, int integrator, double dt, double *vec_n,
                   double *vec_np, double *tmp1, double *tmp2, int ader_recon);

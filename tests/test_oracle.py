@@ -34,7 +34,7 @@ def _problem(params, transform=O.TRANSFORM_REFERENCE_SINE):
 
 
 FIXTURES = ["lsrk45r2_2d_cartesian", "clrk4_3d", "ader_2d_norecon_ref2", "ader_2d_recon_ref2",
-            "ader_2d_recon_ref3", "ader_3d_norecon", "aderlts_2d_recon"]
+            "ader_2d_recon_ref3", "ader_3d_norecon", "aderlts_2d_recon", "aderadconfull_2d"]
 
 
 @pytest.mark.parametrize("name", FIXTURES)
@@ -174,7 +174,8 @@ def _dense_operator(m, k):
 
 
 @pytest.mark.parametrize("integ,order", [(O.EXPL_EULER, 1), (O.CLRK4, 4), (O.LSRK45R2, 4),
-                                         (O.LSRK33R2, 3), (O.LSRK59R2, 5)])
+                                         (O.LSRK33R2, 3), (O.LSRK59R2, 5), (O.LSRK45R3, 4),
+                                         (O.SSPRK, 4), (O.ADER_ADCONFULL, 3)])
 def test_integrator_order(integ, order):
     """One step of each integrator vs exp(dt L): the local error scales like
     dt^(order+1) (a wrong tableau entry breaks an order condition)."""
@@ -214,7 +215,8 @@ def test_ader_ck_taylor():
 
 @pytest.mark.parametrize("integ,stages,order", [(O.EXPL_EULER, 1, 1), (O.CLRK4, 4, 4),
                                                 (O.LSRK45R2, 5, 4), (O.LSRK33R2, 3, 3),
-                                                (O.LSRK59R2, 9, 5)])
+                                                (O.LSRK59R2, 9, 5), (O.LSRK45R3, 5, 4),
+                                                (O.SSPRK, 8, 4), (O.ADER_ADCONFULL, 3, 3)])
 def test_stability_polynomial(integ, stages, order):
     """For u' = L u one step is R(dt L) u with R of degree `stages`; recover the
     coefficients gamma_j by a Vandermonde fit over several dt and require
@@ -244,6 +246,20 @@ def test_stability_polynomial(integ, stages, order):
         assert abs(gam[j] - 1.0 / factorial(j)) < 1e-7, (j, gam)
     if integ == O.LSRK45R2:
         assert abs(gam[5] - 0.0048544) < 1e-6
+    if integ == O.ADER_ADCONFULL:  # k = 2: exactly the cubic Taylor polynomial of exp
+        assert abs(gam[3] - 1.0 / 6.0) < 1e-7
+
+
+def test_ssprk84_shu_osher_rows():
+    """Each stage of the Shu-Osher form is a combination of the earlier stages
+    whose A row sums to one, so a step with dt = 0 returns u itself (a
+    mistyped A entry breaks it; the B entries are pinned by the order tests)."""
+    m = O.Mesh(2, 2, 0, O.TRANSFORM_NONE, 0.0, 2)
+    k = 2
+    u0 = _random_fields(m, k, 3)
+    vn, vnp = u0.copy(), np.zeros_like(u0)
+    m.time_step(k, O.SSPRK, 0.0, vn, vnp)
+    assert np.max(np.abs(vnp - u0)) <= 1e-14 * np.max(np.abs(u0))
 
 
 def lts_info(m, k, cfl_number, max_clusters, max_diff, recon=1):
