@@ -15,7 +15,8 @@ from __future__ import annotations
 
 import ctypes as C
 
-from .capi import (ADER, ADER_LTS, CLRK4, EXPL_EULER, FP32, FP64, PRECISION_AUTO, INTEGRATORS, LSRK33R2, LSRK45R2,  # noqa
+from .capi import (ADER, ADER_ADCONFULL, ADER_LTS, CLRK4, EXPL_EULER, FP32, FP64,  # noqa
+                   PRECISION_AUTO, INTEGRATORS, LSRK33R2, LSRK45R2, LSRK45R3, SSPRK,
                    LSRK59R2, TRANSFORM_AXIS_2PI, TRANSFORM_NONE, TRANSFORM_PRODUCT_2PI,
                    TRANSFORM_REFERENCE_SINE, Epilogue, ExwError, Modes, Problem, check, load,
                    mesh_hypercube)

@@ -15,9 +15,11 @@ LIB_PATH = os.path.join(HERE, "_lib", "libexwave_amd.so")
 
 EXW_OK, EXW_EINVAL, EXW_EDEVICE, EXW_ENOMEM, EXW_EUNSUP = 0, -1, -2, -3, -4
 FP64, FP32, PRECISION_AUTO = 0, 1, 2
-EXPL_EULER, CLRK4, LSRK45R2, LSRK33R2, LSRK59R2, ADER, ADER_LTS = range(7)
+(EXPL_EULER, CLRK4, LSRK45R2, LSRK33R2, LSRK59R2, ADER, ADER_LTS, LSRK45R3, SSPRK,
+ ADER_ADCONFULL) = range(10)
 INTEGRATORS = {"ExplEuler": EXPL_EULER, "clRK4": CLRK4, "LSRK45R2": LSRK45R2,
-               "LSRK33R2": LSRK33R2, "LSRK59R2": LSRK59R2, "ADER": ADER, "ADERLTS": ADER_LTS}
+               "LSRK33R2": LSRK33R2, "LSRK59R2": LSRK59R2, "ADER": ADER, "ADERLTS": ADER_LTS,
+               "LSRK45R3": LSRK45R3, "SSPRK": SSPRK, "ADERADCONFULL": ADER_ADCONFULL}
 TRANSFORM_NONE, TRANSFORM_REFERENCE_SINE, TRANSFORM_AXIS_2PI, TRANSFORM_PRODUCT_2PI = range(4)
 
 

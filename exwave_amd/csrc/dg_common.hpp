@@ -48,8 +48,10 @@ struct DevMesh {
   const double *__restrict__ vertices;   // general: [n_cells][2^dim][dim]
 };
 
-// o1 = b + a1*s*k, o2 = (o1 or c) + a2*s*k with s = cell_scale[cell] (1 if
-// cell_scale is NULL); cells with s == 0 are inactive and skipped entirely.
+// o1 = b + a1*s*k, o2 = (o1 or c) + a2*s*k, o3 = (o1 or c) + a3*s*k (if o3 is
+// not NULL; needs o2) with s = cell_scale[cell] (1 if cell_scale is NULL);
+// cells with s == 0 are inactive and skipped entirely.  b and c are read
+// before any output of the same entry is written, so outputs may alias them.
 // With cell_list the launch covers the cells cell_list[0..n_list) only (the
 // active cells of a multi-instance pool of the serving loop; Cartesian kernel).
 struct Epilogue {
@@ -63,6 +65,8 @@ struct Epilogue {
   const double *cell_scale;
   const int *cell_list;
   int n_list;
+  double *o3;
+  double a3;
 };
 
 // Optional cell selection of the local-time-stepping sweeps: the kernel runs

@@ -59,7 +59,7 @@ __global__ void __launch_bounds__((Shape<DIM, K>::BLOCK))
     cscale = epi.cell_scale[cell];
     active = cscale != 0.0;
   }
-  const double ea1 = epi.a1 * cscale, ea2 = epi.a2 * cscale;
+  const double ea1 = epi.a1 * cscale, ea2 = epi.a2 * cscale, ea3 = epi.a3 * cscale;
   const size_t cbase = (size_t)cell * NF * ND;
 
   // ---------------- phase 1: independent loads ----------------
@@ -362,8 +362,11 @@ __global__ void __launch_bounds__((Shape<DIM, K>::BLOCK))
       const double kk = double(f < DIM ? kv[f < DIM ? f : 0] : kp);
       const double o1 = bv[f] + ea1 * kk;
       __builtin_nontemporal_store(o1, epi.o1 + idx);
-      if (epi.o2)
-        __builtin_nontemporal_store((epi.o2_from_o1 ? o1 : cv[f]) + ea2 * kk, epi.o2 + idx);
+      if (epi.o2) {
+        const double c = epi.o2_from_o1 ? o1 : cv[f];
+        __builtin_nontemporal_store(c + ea2 * kk, epi.o2 + idx);
+        if (epi.o3) __builtin_nontemporal_store(c + ea3 * kk, epi.o3 + idx);
+      }
     }
   }
 }

@@ -238,7 +238,8 @@ __global__ void __launch_bounds__((CartShape<DIM, K>::BLOCK))
 
     // ---------------- C: one lane per node, fused epilogue ----------------
     if (active[it] && lane < ND) {
-      const double ea1 = epi.a1 * cscale[it], ea2 = epi.a2 * cscale[it];
+      const double ea1 = epi.a1 * cscale[it], ea2 = epi.a2 * cscale[it],
+                   ea3 = epi.a3 * cscale[it];
       const size_t cbase = (size_t)cell[it] * NF * ND;
       Number k[NF];
       Number kp = s_nb[lc][0][1][lane];
@@ -254,9 +255,11 @@ __global__ void __launch_bounds__((CartShape<DIM, K>::BLOCK))
         const double kk = double(k[f]);
         const double o1 = bv[it][f] + ea1 * kk;
         __builtin_nontemporal_store(o1, epi.o1 + idx);
-        if (epi.o2)
-          __builtin_nontemporal_store((epi.o2_from_o1 ? o1 : epi.c[idx]) + ea2 * kk,
-                                      epi.o2 + idx);
+        if (epi.o2) {
+          const double cv = epi.o2_from_o1 ? o1 : epi.c[idx];
+          __builtin_nontemporal_store(cv + ea2 * kk, epi.o2 + idx);
+          if (epi.o3) __builtin_nontemporal_store(cv + ea3 * kk, epi.o3 + idx);
+        }
       }
     }
   }

@@ -254,6 +254,14 @@ int exw_op_n_cells(exw_op op) { return op ? op->n_cells : -1; }
 int exw_op_is_cartesian(exw_op op) { return op ? (int)op->cart : -1; }
 double exw_op_min_vertex_distance_over_c(exw_op op) { return op ? op->minvd : -1.0; }
 
+// one operator sweep with the library-internal epilogue (three outputs)
+static int launch_epilogue(exw_op op, const double *src, const Epilogue &ep, int precision) {
+  OpLaunch l = kernels(op->dim).op[op->degree][precision][op->cart ? 1 : 0];
+  if (!l) return fail(EXW_EUNSUP, "operator kernel not compiled");
+  EXW_HIP(l(op->view(), src, op->d_tab, op->host_tab.data(), ep, op->ctx->stream));
+  return EXW_OK;
+}
+
 static int check_prec(int precision) {
   if (precision != EXW_FP64 && precision != EXW_FP32)
     return fail(EXW_EINVAL, "precision must be EXW_FP64 or EXW_FP32");
@@ -268,10 +276,7 @@ int exw_op_apply_fused(exw_op op, const double *src, const exw_epilogue *e, int 
   if (e->o2 && !e->o2_from_o1 && !e->c)
     return fail(EXW_EINVAL, "exw_op_apply_fused: o2 needs c or o2_from_o1");
   Epilogue ep{e->b, e->o1, e->a1, e->c, e->o2, e->a2, e->o2_from_o1, e->cell_scale};
-  OpLaunch l = kernels(op->dim).op[op->degree][precision][op->cart ? 1 : 0];
-  if (!l) return fail(EXW_EUNSUP, "operator kernel not compiled");
-  EXW_HIP(l(op->view(), src, op->d_tab, op->host_tab.data(), ep, op->ctx->stream));
-  return EXW_OK;
+  return launch_epilogue(op, src, ep, precision);
 }
 
 int exw_op_apply(exw_op op, const double *src, double *dst, int precision) {
@@ -305,7 +310,31 @@ __global__ void __launch_bounds__(256)
     }
   }
 }
+// out = sum_i coef[i] * in[i] over n_in <= 16 vectors (out may alias an input)
+struct LinComb {
+  const double *in[16];
+  double coef[16];
+  int n_in;
+};
+__global__ void __launch_bounds__(256)
+    lincomb_kernel(int64_t n, const LinComb lc, double *__restrict__ out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int j = 0; j < lc.n_in; ++j) s += lc.coef[j] * __builtin_nontemporal_load(lc.in[j] + i);
+    __builtin_nontemporal_store(s, out + i);
+  }
+}
 }  // namespace
+
+static int lincomb(exw_op op, const LinComb &lc, double *out) {
+  const int64_t n = op->n_dofs();
+  const int64_t blocks = std::min<int64_t>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(lincomb_kernel, dim3((unsigned)blocks), dim3(256), 0, op->ctx->stream, n, lc,
+                     out);
+  EXW_HIP(hipGetLastError());
+  return EXW_OK;
+}
 
 extern "C" {
 
@@ -357,8 +386,88 @@ int exw_time_step_work_vectors(int integrator) {
     case EXW_LSRK33R2:
     case EXW_LSRK59R2: return 1;
     case EXW_ADER: return 2;
+    case EXW_LSRK45R3: return 2;
+    case EXW_SSPRK: return 14;
+    case EXW_ADER_ADCONFULL: return 2;
     default: return -1;
   }
+}
+
+// Shu-Osher coefficients of the optimal SSPRK with 8 stages and order 4
+// (S. J. Ruuth, Math. Comp. 75 (2006) 183-207) as the reference tabulates them
+// (SSPRK::initialize_coeffs, source/time_integrators.cc:726-858): stage s is
+// sum_{l<s} A[s-1][l] u_l + dt B[s-1][l] F(u_l).
+static const double kSspA[8][8] = {
+    {1.},
+    {0.538569155333175, 0.461430844666825},
+    {0.004485387460763, 0., 0.995514612539237},
+    {0.164495299288580, 0.016875060685979, 0., 0.818629640025440},
+    {0.426933682982668, 0.157047028197878, 0.023164224070770, 0., 0.392855064748685},
+    {0.082083400476958, 0.000000039091042, 0.033974171137350, 0.005505195713107, 0.,
+     0.878437193581543},
+    {0.006736365648625, 0.010581829625529, 0.009353386191951, 0.101886062556838,
+     0.000023428364930, 0., 0.871418927612128},
+    {0.071115287415749, 0.018677648343953, 0.007902408660034, 0.319384027162348,
+     0.007121989995845, 0.001631615692736, 0., 0.574167022729334}};
+static const double kSspB[8][8] = {
+    {0.282318339066479},
+    {0., 0.130270389660380},
+    {0.003963092203460, 0., 0.281052031928487},
+    {0.000038019518678, 0.004764139104512, 0., 0.231114160282572},
+    {0.000019921336144, 0.044337256156151, 0.006539685265423, 0., 0.110910189373703},
+    {0.000000034006679, 0.000000011036118, 0.009591531566657, 0.001554217709960, 0.,
+     0.247998929466160},
+    {0.013159891155054, 0.002987444564164, 0.002640632454359, 0.028764303955070,
+     0.000006614257074, 0., 0.246017544274548},
+    {0.000000010647874, 0.005273042658132, 0.002230994887525, 0.090167968072837,
+     0.002010668386475, 0.000460635032368, 0., 0.162097880203691}};
+
+// SSPRK(8,4): stage s first combines every known term (u_0..u_{s-1},
+// F_0..F_{s-2}) into its output vector, then the sweep over u_{s-1} stores
+// F_{s-1} and adds dt B[s-1][s-1] F_{s-1} to that output in its epilogue; the
+// last stage adds into vec_np and stores no F.  work: u_1..u_7, F_0..F_6.
+static int ssprk84(exw_op op, double dt, const double *vec_n, double *vec_np, double *work,
+                   int precision) {
+  const int64_t N = op->n_dofs();
+  const double *u[8];
+  double *F[7];
+  u[0] = vec_n;
+  for (int l = 1; l < 8; ++l) u[l] = work + (l - 1) * N;
+  for (int l = 0; l < 7; ++l) F[l] = work + (7 + l) * N;
+  for (int st = 1; st <= 8; ++st) {
+    double *out = st < 8 ? const_cast<double *>(u[st]) : vec_np;
+    const double *c = out;
+    if (st == 1) {
+      c = vec_n;  // u_1 = u_0 + dt B[0][0] F_0
+    } else {
+      LinComb lc{};
+      for (int l = 0; l < st; ++l) {
+        if (kSspA[st - 1][l] != 0.0) {
+          lc.in[lc.n_in] = u[l];
+          lc.coef[lc.n_in++] = kSspA[st - 1][l];
+        }
+        if (l < st - 1 && kSspB[st - 1][l] != 0.0) {
+          lc.in[lc.n_in] = F[l];
+          lc.coef[lc.n_in++] = dt * kSspB[st - 1][l];
+        }
+      }
+      if (int rc = lincomb(op, lc, out)) return rc;
+    }
+    Epilogue ep{};
+    if (st < 8) {
+      ep.o1 = F[st - 1];  // F_{s-1} = k
+      ep.a1 = 1.0;
+      ep.c = c;
+      ep.o2 = out;
+      ep.a2 = dt * kSspB[st - 1][st - 1];
+    } else {
+      ep.b = out;
+      ep.o1 = out;
+      ep.a1 = dt * kSspB[7][7];
+    }
+    if (int rc = launch_epilogue(op, u[st - 1], ep, precision)) return rc;
+  }
+  return EXW_OK;
 }
 
 // Low-storage RK with the RKVectorUpdater fused into each operator sweep
@@ -439,6 +548,75 @@ int exw_time_step(exw_op op, int integrator, double dt, double *vec_n, double *v
     case EXW_ADER:
       if (!work) return fail(EXW_EINVAL, "exw_time_step: work is NULL");
       return exw_ader_step(op, vec_n, vec_np, work, dt, ader_recon, precision);
+    case EXW_LSRK45R3: {
+      // LowStorageRK45Reg3 (source/time_integrators.cc:366-425).  With
+      // y_s = u + dt sum_{i<=s} b_i k_i the stage inputs are
+      // x_{s+1} = y_{s-2} + dt (a_{s+1,s-1} k_{s-1} + a_{s+1,s} k_s); each sweep
+      // (k_s in registers) writes x_{s+1} = z_{s-1} + dt a_{s+1,s} k_s, the
+      // known part of the stage after next z_s = y_{s-1} + dt a_{s+2,s} k_s,
+      // and y_s = y_{s-1} + dt b_s k_s (kept in vec_np).  Registers: vec_n
+      // (u, later a stage input), work[0] (stage inputs), work[1] (z).
+      if (!work) return fail(EXW_EINVAL, "exw_time_step: work is NULL");
+      const double a21 = 2365592473904. / 8146167614645., a32 = 4278267785271. / 6823155464066.,
+                   a43 = 2789585899612. / 8986505720531., a54 = 15310836689591. / 24358012670437.,
+                   a31 = -722262345248. / 10870640012513., a42 = 1365858020701. / 8494387045469.,
+                   a53 = 3819021186. / 2763618202291.;
+      const double b1 = 846876320697. / 6523801458457., b2 = 3032295699695. / 12397907741132.,
+                   b3 = 612618101729. / 6534652265123., b4 = 1155491934595. / 2954287928812.,
+                   b5 = 707644755468. / 5028292464395.;
+      double *xa = work, *z = work + N, *xb = vec_n;
+      int rc;
+      Epilogue e1{};  // stage 1: x2, z1 = u + dt a31 k1, y1 = u + dt b1 k1
+      e1.b = vec_n; e1.o1 = xa; e1.a1 = dt * a21;
+      e1.c = vec_n; e1.o2 = z; e1.a2 = dt * a31; e1.o3 = vec_np; e1.a3 = dt * b1;
+      if ((rc = launch_epilogue(op, vec_n, e1, precision))) return rc;
+      Epilogue e2{};  // stage 2: x3 = z1 + dt a32 k2, z2 = y1 + dt a42 k2, y2
+      e2.b = z; e2.o1 = xb; e2.a1 = dt * a32;
+      e2.c = vec_np; e2.o2 = z; e2.a2 = dt * a42; e2.o3 = vec_np; e2.a3 = dt * b2;
+      if ((rc = launch_epilogue(op, xa, e2, precision))) return rc;
+      Epilogue e3{};  // stage 3: x4 = z2 + dt a43 k3, z3 = y2 + dt a53 k3, y3
+      e3.b = z; e3.o1 = xa; e3.a1 = dt * a43;
+      e3.c = vec_np; e3.o2 = z; e3.a2 = dt * a53; e3.o3 = vec_np; e3.a3 = dt * b3;
+      if ((rc = launch_epilogue(op, xb, e3, precision))) return rc;
+      Epilogue e4{};  // stage 4: x5 = z3 + dt a54 k4, y4
+      e4.b = z; e4.o1 = xb; e4.a1 = dt * a54;
+      e4.c = vec_np; e4.o2 = vec_np; e4.a2 = dt * b4;
+      if ((rc = launch_epilogue(op, xa, e4, precision))) return rc;
+      Epilogue e5{};  // stage 5: vec_np = y4 + dt b5 k5
+      e5.b = vec_np; e5.o1 = vec_np; e5.a1 = dt * b5;
+      return launch_epilogue(op, xb, e5, precision);
+    }
+    case EXW_SSPRK:
+      if (!work) return fail(EXW_EINVAL, "exw_time_step: work is NULL");
+      return ssprk84(op, dt, vec_n, vec_np, work, precision);
+    case EXW_ADER_ADCONFULL: {
+      // WaveEquationOperationADERADCONFULL::apply_ader
+      // (source/wave_equation_operations.cc:689-735) with ArbitraryHighOrderDG
+      // (source/time_integrators.cc:37-50): vec_np = sum_{j=0}^{k+1} (dt A)^j/j! vec_n.
+      // Sweep j reads w_{j-1} (w_0 = vec_n) and writes w_j = (dt/j) A w_{j-1} and
+      // vec_np += w_j in its epilogue; w alternates between the two work vectors.
+      if (!work) return fail(EXW_EINVAL, "exw_time_step: work is NULL");
+      const int J = op->degree + 1;
+      const double *w = vec_n;
+      for (int j = 1; j <= J; ++j) {
+        double *wn = work + ((j - 1) & 1) * N;
+        Epilogue e{};
+        if (j < J) {
+          e.o1 = wn;
+          e.a1 = dt / j;
+          e.c = j == 1 ? vec_n : vec_np;
+          e.o2 = vec_np;
+          e.a2 = dt / j;
+        } else {
+          e.b = J == 1 ? vec_n : vec_np;
+          e.o1 = vec_np;
+          e.a1 = dt / j;
+        }
+        if (int rc = launch_epilogue(op, w, e, precision)) return rc;
+        w = wn;
+      }
+      return EXW_OK;
+    }
     default:
       return fail(EXW_EINVAL, "unknown time integrator " + std::to_string(integrator) +
                                   " requested");

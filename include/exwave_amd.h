@@ -46,6 +46,9 @@ extern "C" {
 #define EXW_LSRK59R2 4
 #define EXW_ADER 5
 #define EXW_ADER_LTS 6 /* exw_run / exw_lts_*: ADER with local time stepping */
+#define EXW_LSRK45R3 7
+#define EXW_SSPRK 8          /* SSPRK with 8 stages and order 4 (the reference's 'SSPRK') */
+#define EXW_ADER_ADCONFULL 9 /* ADER with the full operator's Taylor series */
 
 typedef struct exw_ctx_s *exw_ctx;
 typedef struct exw_op_s *exw_op;

@@ -144,7 +144,7 @@ def test_rk_stage_matches_updater(X, ctx):
 
 
 @pytest.mark.parametrize("integ", [O.EXPL_EULER, O.CLRK4, O.LSRK45R2, O.LSRK33R2, O.LSRK59R2,
-                                   O.ADER])
+                                   O.ADER, O.LSRK45R3, O.SSPRK, O.ADER_ADCONFULL])
 @pytest.mark.parametrize("geom", ["cart3d", "deformed2d"])
 def test_time_step_parity(X, ctx, integ, geom):
     """exw_time_step (fused stages, 3-register rotation) == the oracle's
@@ -206,7 +206,8 @@ def test_project_and_norms(X, ctx, dim, k, tr):
     assert abs(op.energy(d) - om.energy(k, want)) <= 1e-12 * om.energy(k, want)
 
 
-FIXTURES = ["lsrk45r2_2d_cartesian", "clrk4_3d", "ader_2d_norecon_ref2", "ader_2d_recon_ref2"]
+FIXTURES = ["lsrk45r2_2d_cartesian", "clrk4_3d", "ader_2d_norecon_ref2", "ader_2d_recon_ref2",
+            "aderadconfull_2d"]
 
 
 @pytest.mark.parametrize("name", FIXTURES)
