@@ -33,7 +33,12 @@ class _Mesh(C.Structure):
     _fields_ = [("dim", C.c_int), ("n_cells", C.c_int),
                 ("vertices", C.POINTER(C.c_double)), ("neighbor", C.POINTER(C.c_int)),
                 ("rho", C.POINTER(C.c_double)), ("speed", C.POINTER(C.c_double)),
-                ("min_vertex_distance_over_c", C.c_double)]
+                ("min_vertex_distance_over_c", C.c_double),
+                ("n_hanging", C.c_int), ("hanging", C.POINTER(C.c_int)),
+                ("nc_rec", C.POINTER(C.c_int)), ("nc_pos", C.POINTER(C.c_int))]
+
+
+NC_FACE = -1000  # neighbour entry of a non-conforming face
 
 
 class Exact(C.Structure):
@@ -81,6 +86,9 @@ def lib():
         L.ox_mesh_hypercube.restype = C.POINTER(_Mesh)
         L.ox_mesh_hypercube.argtypes = [C.c_int] * 5 + [C.c_double, C.c_int]
         L.ox_mesh_hypercube.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int]
+        L.ox_mesh_box_refined.restype = C.POINTER(_Mesh)
+        L.ox_mesh_box_refined.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_int,
+                                          C.POINTER(C.c_int)]
         L.ox_mesh_free.argtypes = [C.POINTER(_Mesh)]
         L.ox_mesh_set_material.argtypes = [C.POINTER(_Mesh), dp, dp]
         L.ox_apply.argtypes = [C.POINTER(_Mesh), C.c_int, dp, dp, C.c_double]
@@ -137,6 +145,28 @@ class Mesh:
         self._m = lib().ox_mesh_hypercube(dim, n_intervals, n_refinements, transform, factor,
                                           boundary_id)
         self.n_cells = self._m.contents.n_cells
+
+    @classmethod
+    def box_refined(cls, dim, n_intervals, refine, transform=0, factor=0.0, boundary_id=2):
+        """ox_mesh_box_refined: base cells flagged in refine (n_intervals^dim,
+        lexicographic) split once; faces to unrefined cells hang."""
+        r = np.ascontiguousarray(refine, dtype=np.int32).ravel()
+        assert r.size == n_intervals ** dim
+        self = cls.__new__(cls)
+        self.dim = dim
+        self._m = lib().ox_mesh_box_refined(dim, n_intervals, transform, factor, boundary_id,
+                                            r.ctypes.data_as(C.POINTER(C.c_int)))
+        self.n_cells = self._m.contents.n_cells
+        return self
+
+    @property
+    def hanging(self):
+        """(n_hanging, 2 + 2^(dim-1)) records: coarse cell, coarse face, fine cells."""
+        n = self._m.contents.n_hanging
+        w = 2 + (1 << (self.dim - 1))
+        if n == 0:
+            return np.zeros((0, w), dtype=np.int32)
+        return np.ctypeslib.as_array(self._m.contents.hanging, shape=(n, w)).copy()
 
     def __del__(self):
         if getattr(self, "_m", None) is not None and _lib is not None:

@@ -278,8 +278,119 @@ void ox_mesh_set_material(ox_mesh *m, const double *rho, const double *speed) {
   m->min_vertex_distance_over_c = mind;
 }
 
+static double min_vertex_distance(int dim, const double *V) {
+  const int nv = 1 << dim;
+  double mind = 1e300;
+  for (int a = 0; a < nv; ++a)
+    for (int b = a + 1; b < nv; ++b) {
+      double s = 0.0;
+      for (int d = 0; d < dim; ++d)
+        s += (V[a * dim + d] - V[b * dim + d]) * (V[a * dim + d] - V[b * dim + d]);
+      if (s < mind) mind = s;
+    }
+  return sqrt(mind);
+}
+
+ox_mesh *ox_mesh_box_refined(int dim, int n_intervals, int transform, double factor,
+                             int boundary_id, const int *refine) {
+  const int N = n_intervals, nv = 1 << dim, nface = 2 * dim, nbase = ipow(N, dim);
+  const int nsub = 1 << (dim - 1), rec = 2 + nsub;
+  int *first = (int *)malloc(sizeof(int) * (nbase + 1));
+  first[0] = 0;
+  for (int B = 0; B < nbase; ++B) first[B + 1] = first[B] + (refine[B] ? nv : 1);
+  ox_mesh *m = (ox_mesh *)calloc(1, sizeof(ox_mesh));
+  m->dim = dim;
+  m->n_cells = first[nbase];
+  m->vertices = (double *)malloc(sizeof(double) * m->n_cells * nv * dim);
+  m->neighbor = (int *)malloc(sizeof(int) * m->n_cells * nface);
+  m->rho = (double *)malloc(sizeof(double) * m->n_cells);
+  m->speed = (double *)malloc(sizeof(double) * m->n_cells);
+  m->nc_rec = (int *)malloc(sizeof(int) * m->n_cells * nface);
+  m->nc_pos = (int *)malloc(sizeof(int) * m->n_cells * nface);
+  m->hanging = (int *)malloc(sizeof(int) * (nbase * nface + 1) * rec);
+  m->n_hanging = 0;
+  double mind = 1e300;
+  for (int B = 0; B < nbase; ++B) {
+    int idx[3] = {0, 0, 0}, t = B;
+    for (int d = 0; d < dim; ++d) {
+      idx[d] = t % N;
+      t /= N;
+    }
+    double PV[8 * 3];
+    for (int v = 0; v < nv; ++v) {
+      double x[3] = {0.0, 0.0, 0.0};
+      for (int d = 0; d < dim; ++d) x[d] = (double)(idx[d] + ((v >> d) & 1)) / N;
+      transform_vertex(dim, transform, factor, x);
+      for (int d = 0; d < dim; ++d) PV[v * dim + d] = x[d];
+    }
+    const int nch = refine[B] ? nv : 1;
+    for (int ch = 0; ch < nch; ++ch) {
+      const int c = first[B] + ch;
+      for (int v = 0; v < nv; ++v) {
+        double xi[3];
+        for (int d = 0; d < dim; ++d)
+          xi[d] = refine[B] ? 0.5 * (((ch >> d) & 1) + ((v >> d) & 1)) : (double)((v >> d) & 1);
+        multilinear_point(dim, PV, xi, m->vertices + ((long)c * nv + v) * dim);
+      }
+      for (int d = 0; d < dim; ++d)
+        for (int s = 0; s < 2; ++s) {
+          const int f = 2 * d + s, cf = c * nface + f;
+          m->nc_rec[cf] = m->nc_pos[cf] = -1;
+          const int cd = (ch >> d) & 1;
+          if (refine[B] && cd != s) {  /* sibling inside the parent */
+            m->neighbor[cf] = first[B] + (ch ^ (1 << d));
+            continue;
+          }
+          const int ni = idx[d] + (s ? 1 : -1);
+          if (ni < 0 || ni >= N) {
+            m->neighbor[cf] = -boundary_id;
+            continue;
+          }
+          const int B2 = B + (s ? 1 : -1) * ipow(N, d);
+          if (!refine[B2]) {
+            m->neighbor[cf] = refine[B] ? OX_NC_FACE : first[B2];  /* fine side filled below */
+          } else if (refine[B]) {
+            m->neighbor[cf] = first[B2] + (ch ^ (1 << d));
+          } else {
+            /* coarse side of a hanging face: the children of B2 touching it */
+            int *r = m->hanging + (long)m->n_hanging * rec;
+            r[0] = c;
+            r[1] = f;
+            for (int b = 0; b < nsub; ++b) {
+              int chn = (s ? 0 : 1) << d, j = 0;
+              for (int e = 0; e < dim; ++e)
+                if (e != d) chn |= ((b >> j++) & 1) << e;
+              r[2 + b] = first[B2] + chn;
+            }
+            m->neighbor[cf] = OX_NC_FACE;
+            m->nc_rec[cf] = m->n_hanging++;
+          }
+        }
+      m->rho[c] = 1.0;
+      m->speed[c] = 1.0;
+      const double md = min_vertex_distance(dim, m->vertices + (long)c * nv * dim);
+      if (md < mind) mind = md;
+    }
+  }
+  /* fine sides */
+  for (int h = 0; h < m->n_hanging; ++h) {
+    const int *r = m->hanging + (long)h * rec;
+    for (int b = 0; b < nsub; ++b) {
+      const int cf = r[2 + b] * nface + (r[1] ^ 1);
+      m->nc_rec[cf] = h;
+      m->nc_pos[cf] = b;
+    }
+  }
+  m->min_vertex_distance_over_c = mind;
+  free(first);
+  return m;
+}
+
 void ox_mesh_free(ox_mesh *m) {
   if (!m) return;
+  free(m->hanging);
+  free(m->nc_rec);
+  free(m->nc_pos);
   free(m->vertices);
   free(m->neighbor);
   free(m->rho);
@@ -324,6 +435,114 @@ static void face_flux(int dim, const double *nrm, const double *up, const double
   }
   for (int d = 0; d < dim; ++d) out[d] = fac * (pp - lambda) / rho_p * nrm[d];
   out[dim] = fac * c_p * c_p * rho_p * (-nvp + tau_p * (lambda - pp));
+}
+
+/* value of field uk (n^dim nodal values) at the reference point xi */
+static double eval_at(const ox_basis *b, int dim, const double *uk, const double *xi) {
+  const int n = b->n;
+  double l[3][OX_MAX_N];
+  for (int d = 0; d < dim; ++d) ox_lagrange(n, b->x, xi[d], l[d], NULL);
+  double s = 0.0;
+  const int nd = ipow(n, dim);
+  for (int q = 0; q < nd; ++q) {
+    double w = 1.0;
+    int t = q;
+    for (int d = 0; d < dim; ++d) {
+      w *= l[d][t % n];
+      t /= n;
+    }
+    s += w * uk[q];
+  }
+  return s;
+}
+
+/* Face f of cell c on a non-conforming (2:1) face.  deal.II integrates such a
+ * face on the fine cell's face with its QGauss(k+1) points and evaluates the
+ * coarse cell there (FEFaceEvaluation on subfaces):
+ *   fine side: own trace and geometry as usual, the coarse neighbour's
+ *     polynomial at the matching points of its face quarter;
+ *   coarse side: the sum over the 2^(dim-1) subfaces of the same integrals,
+ *     with this cell's trace, geometry and test functions at the mapped
+ *     points (xi_t + b_t)/2 and the weights w_t/2 per tangential direction.
+ * Same flux (source/wave_equation_operations.cc:480-551) on both sides. */
+static void cell_face_nc(const ox_mesh *m, const ox_basis *b, int c, int f, const double *src,
+                         double *r) {
+  const int dim = m->dim, n = b->n, nf = dim + 1, nface = 2 * dim;
+  const int nd = ipow(n, dim), nv = 1 << dim, nt = ipow(n, dim - 1), nsub = 1 << (dim - 1);
+  const int rec = 2 + nsub;
+  const int d = f >> 1, s = f & 1;
+  const int h = m->nc_rec[c * nface + f], pos = m->nc_pos[c * nface + f];
+  const int *R = m->hanging + (long)h * rec;
+  const double *V = m->vertices + (long)c * nv * dim;
+  const double *u = src + (long)c * nf * nd;
+  const double rho = m->rho[c], cc = m->speed[c];
+  const int stride_d = ipow(n, d);
+  const int nsb = pos >= 0 ? 1 : nsub;
+  for (int sb = 0; sb < nsb; ++sb) {
+    const int bpos = pos >= 0 ? pos : sb;
+    const int other = pos >= 0 ? R[0] : R[2 + sb];
+    const double *uo = src + (long)other * nf * nd;
+    for (int t = 0; t < nt; ++t) {
+      int base = 0, tt = t, j = 0;
+      double xs[3], xc[3], wt = 1.0; /* fine-face point, coarse-face point */
+      xs[d] = xc[d] = (double)s;
+      for (int e = 0; e < dim; ++e) {
+        if (e == d) continue;
+        const int te = tt % n;
+        tt /= n;
+        base += te * ipow(n, e);
+        xs[e] = b->x[te];
+        xc[e] = 0.5 * (b->x[te] + ((bpos >> j) & 1));
+        wt *= b->w[te];
+        ++j;
+      }
+      /* own side: fine -> the usual face point; coarse -> the mapped point */
+      const double *xo = pos >= 0 ? xs : xc;
+      if (pos < 0) wt /= nsub;
+      double J[9], Ji[9];
+      const double det = multilinear_jacobian(dim, V, xo, J, Ji);
+      double nrm[3], na = 0.0;
+      for (int i = 0; i < dim; ++i) {
+        nrm[i] = Ji[d * dim + i];
+        na += nrm[i] * nrm[i];
+      }
+      na = sqrt(na);
+      for (int i = 0; i < dim; ++i) nrm[i] = (s ? 1.0 : -1.0) * nrm[i] / na;
+      const double jxw_f = fabs(det) * na * wt;
+      double up[4], um[4], val[4];
+      double xn[3]; /* the other side's reference point */
+      for (int e = 0; e < dim; ++e) xn[e] = pos >= 0 ? xc[e] : xs[e];
+      xn[d] = (double)(1 - s);
+      for (int k = 0; k < nf; ++k) {
+        if (pos >= 0) {
+          double a = 0.0;
+          for (int i = 0; i < n; ++i) a += b->face[s][i] * u[k * nd + base + i * stride_d];
+          up[k] = a;
+        } else {
+          up[k] = eval_at(b, dim, u + k * nd, xo);
+        }
+        um[k] = eval_at(b, dim, uo + k * nd, xn);
+      }
+      face_flux(dim, nrm, up, um, rho, cc, m->rho[other], m->speed[other], 0, 1.0, val);
+      if (pos >= 0) {
+        for (int k = 0; k < nf; ++k)
+          for (int i = 0; i < n; ++i)
+            r[k * nd + base + i * stride_d] += b->face[s][i] * val[k] * jxw_f;
+      } else {
+        double l[3][OX_MAX_N];
+        for (int e = 0; e < dim; ++e) ox_lagrange(n, b->x, xo[e], l[e], NULL);
+        for (int q = 0; q < nd; ++q) {
+          double phi = 1.0;
+          int tq = q;
+          for (int e = 0; e < dim; ++e) {
+            phi *= l[e][tq % n];
+            tq /= n;
+          }
+          for (int k = 0; k < nf; ++k) r[k * nd + q] += phi * val[k] * jxw_f;
+        }
+      }
+    }
+  }
 }
 
 typedef struct work {
@@ -436,7 +655,10 @@ static void apply_cell(const ox_mesh *m, const ox_basis *b, int c, const double 
   cell_volume(m, b, c, u, r, jxw, W);
   for (int f = 0; f < 2 * dim; ++f) {
     const int nb = m->neighbor[c * 2 * dim + f];
-    cell_face(m, b, c, f, u, nb >= 0 ? src + (long)nb * nf * nd : NULL, 1.0, r);
+    if (nb == OX_NC_FACE)
+      cell_face_nc(m, b, c, f, src, r);
+    else
+      cell_face(m, b, c, f, u, nb >= 0 ? src + (long)nb * nf * nd : NULL, 1.0, r);
   }
   /* inverse mass: collocation at Gauss points -> divide by JxW */
   double *o = dst + (long)c * nf * nd;
@@ -1503,6 +1725,7 @@ static void lts_schedule(ox_lts *L, double time0) {
 
 ox_lts *ox_lts_create(const ox_mesh *m, int degree, double cfl, double fastest_dt,
                       int max_clusters, int max_diff, double time0, int recon) {
+  if (m->n_hanging > 0) return NULL; /* non-conforming faces: not restated for LTS */
   ox_lts *L = (ox_lts *)calloc(1, sizeof(ox_lts));
   L->m = m;
   L->degree = degree;

@@ -56,7 +56,19 @@ typedef struct ox_mesh {
   double *rho;      /* n_cells */
   double *speed;    /* n_cells */
   double min_vertex_distance_over_c; /* min_K min_vertex_distance(K)/c_K */
+  /* Non-conforming (2:1) faces, as deal.II's hanging faces after one local
+   * refinement: record h is hanging[h*(2+2^(dim-1)) + ...] = {coarse cell,
+   * coarse face, fine cell of subface 0, 1, ...}.  A subface position b has
+   * bit j set when the fine face covers the upper half of the j-th tangential
+   * direction (the directions other than the face's, ascending); the fine
+   * cells see the face as their face (coarse face ^ 1).  Both sides hold
+   * OX_NC_FACE in neighbor[]; nc_rec / nc_pos give the record and the subface
+   * position (-1 on the coarse side) per cell face. */
+  int n_hanging;
+  int *hanging;
+  int *nc_rec, *nc_pos; /* n_cells * 2*dim, -1 where conforming */
 } ox_mesh;
+#define OX_NC_FACE (-1000)
 
 /* Exact solution: superposition of membrane modes in a homogeneous medium
  * (include/input_parameters.h:94-128 is the single mode m_d = m, rho=c=1). */
@@ -85,6 +97,14 @@ ox_mesh *ox_mesh_hypercube(int dim, int n_intervals, int n_refinements, int tran
                            double factor, int boundary_id);
 void ox_mesh_set_material(ox_mesh *m, const double *rho, const double *speed);
 void ox_mesh_free(ox_mesh *m);
+/* The n_intervals^dim box mesh of ox_mesh_hypercube (transform applied to the
+ * base vertices) with the base cells flagged in refine[] (lexicographic, xi_0
+ * fastest) split once into 2^dim children whose vertices are the parent's
+ * multilinear map at the dyadic points.  Cells are numbered base cell by base
+ * cell, children lexicographic inside their parent.  Faces between a refined
+ * and an unrefined base cell are non-conforming (hanging[]). */
+ox_mesh *ox_mesh_box_refined(int dim, int n_intervals, int transform, double factor,
+                             int boundary_id, const int *refine);
 
 /* dst = M^-1 A(src) * (multiplied by boundary_fac: +1 RK, -1 second ADER sweep) */
 void ox_apply(const ox_mesh *m, int degree, const double *src, double *dst, double boundary_fac);
