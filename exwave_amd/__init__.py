@@ -18,8 +18,8 @@ import ctypes as C
 from .capi import (ADER, ADER_ADCONFULL, ADER_LTS, CLRK4, EXPL_EULER, FP32, FP64,  # noqa
                    PRECISION_AUTO, INTEGRATORS, LSRK33R2, LSRK45R2, LSRK45R3, SSPRK,
                    LSRK59R2, TRANSFORM_AXIS_2PI, TRANSFORM_NONE, TRANSFORM_PRODUCT_2PI,
-                   TRANSFORM_REFERENCE_SINE, Epilogue, ExwError, Modes, Problem, check, load,
-                   mesh_hypercube)
+                   TRANSFORM_REFERENCE_SINE, NC_FACE, Epilogue, ExwError, Modes, Problem, check,
+                   load, mesh_box_refined, mesh_hypercube)
 
 
 def _ptr(t):
@@ -60,7 +60,9 @@ class Context:
 class WaveOperator:
     """WaveEquationOperation<dim,k> on the device (setup + apply + integrators)."""
 
-    def __init__(self, ctx: Context, dim: int, degree: int, vertices, neighbor, rho, speed):
+    def __init__(self, ctx: Context, dim: int, degree: int, vertices, neighbor, rho, speed,
+                 hanging=None):
+        """hanging: records of the 2:1 hanging faces (exw_op_create_nc), or None."""
         import numpy as np
         L = load()
         V = np.ascontiguousarray(vertices, dtype=np.float64)
@@ -72,9 +74,16 @@ class WaveOperator:
         assert r.shape == (n_cells,) and c.shape == (n_cells,)
         h = C.c_void_p()
         dp, ip = C.POINTER(C.c_double), C.POINTER(C.c_int)
-        check(L.exw_op_create(ctx.h, dim, degree, n_cells, V.ctypes.data_as(dp),
-                              nb.ctypes.data_as(ip), r.ctypes.data_as(dp), c.ctypes.data_as(dp),
-                              C.byref(h)))
+        if hanging is not None and len(hanging) > 0:
+            H = np.ascontiguousarray(hanging, dtype=np.int32)
+            assert H.shape[1] == 2 + (1 << (dim - 1))
+            check(L.exw_op_create_nc(ctx.h, dim, degree, n_cells, V.ctypes.data_as(dp),
+                                     nb.ctypes.data_as(ip), H.shape[0], H.ctypes.data_as(ip),
+                                     r.ctypes.data_as(dp), c.ctypes.data_as(dp), C.byref(h)))
+        else:
+            check(L.exw_op_create(ctx.h, dim, degree, n_cells, V.ctypes.data_as(dp),
+                                  nb.ctypes.data_as(ip), r.ctypes.data_as(dp),
+                                  c.ctypes.data_as(dp), C.byref(h)))
         self.h, self.ctx, self.dim, self.degree, self.n_cells = h, ctx, dim, degree, n_cells
         self.n_dofs = int(L.exw_op_n_dofs(h))
         self.is_cartesian = bool(L.exw_op_is_cartesian(h))

@@ -111,6 +111,8 @@ SIGNATURES = [
     ("exw_l2_errors", _I, [_VP, C.POINTER(Modes), _D, _VP, _DP]),
     ("exw_energy", _I, [_VP, _VP, _DP]),
     ("exw_post_n_values", C.c_int64, [_VP]),
+    ("exw_op_create_nc", _I, [_VP, _I, _I, _I, _DP, _IP, _I, _IP, _DP, _DP, C.POINTER(_VP)]),
+    ("exw_mesh_box_refined", _I, [_I, _I, _I, _D, _I, _IP, _IP, _DP, _IP, _IP, _IP]),
     ("exw_post_pressure", _I, [_VP, _VP, _VP, _VP, _VP]),
     ("exw_post_error", _I, [_VP, C.POINTER(Modes), _D, _VP, _DP]),
     ("exw_mesh_hypercube", _I, [_I, _I, _I, _I, _D, _I, _IP, _DP, _IP]),
@@ -155,6 +157,31 @@ def check(rc):
     if rc < 0:
         raise ExwError(rc, load().exw_last_error().decode())
     return rc
+
+
+NC_FACE = -1000  # EXW_NC_FACE: neighbour entry of a hanging (2:1) face
+
+
+def mesh_box_refined(dim, n_intervals, refine, transform=TRANSFORM_NONE, factor=0.0,
+                     boundary_id=2):
+    """exw_mesh_box_refined: (vertices, neighbor, hanging records) of the box
+    with the flagged base cells (lexicographic, xi_0 fastest) refined once."""
+    import numpy as np
+    L = load()
+    r = np.ascontiguousarray(refine, dtype=np.int32).ravel()
+    ip = C.POINTER(C.c_int)
+    nc, nh = C.c_int(0), C.c_int(0)
+    check(L.exw_mesh_box_refined(dim, n_intervals, transform, factor, boundary_id,
+                                 r.ctypes.data_as(ip), C.byref(nc), None, None, C.byref(nh),
+                                 None))
+    V = np.zeros((nc.value, 1 << dim, dim))
+    nb = np.zeros((nc.value, 2 * dim), dtype=np.int32)
+    H = np.zeros((max(nh.value, 1), 2 + (1 << (dim - 1))), dtype=np.int32)
+    check(L.exw_mesh_box_refined(dim, n_intervals, transform, factor, boundary_id,
+                                 r.ctypes.data_as(ip), C.byref(nc),
+                                 V.ctypes.data_as(C.POINTER(C.c_double)), nb.ctypes.data_as(ip),
+                                 C.byref(nh), H.ctypes.data_as(ip)))
+    return V, nb, H[: nh.value]
 
 
 def mesh_hypercube(dim, n_intervals, n_refinements=0, transform=TRANSFORM_NONE, factor=0.0,

@@ -27,12 +27,17 @@ struct exw_op_s {
   double *d_rho = nullptr, *d_speed = nullptr, *d_h = nullptr, *d_vertices = nullptr;
   double *d_tab = nullptr, *d_errtab = nullptr, *d_cktab = nullptr, *d_posttab = nullptr;
   double *d_partial = nullptr, *d_sum = nullptr;
+  // non-conforming faces: the cells that touch them (NcArgs) and the table
+  int n_nc = 0, n_hanging = 0;
+  int *d_nc_cells = nullptr, *d_nc_kind = nullptr, *d_nc_other = nullptr;
+  double *d_nctab = nullptr;
   int n_partial = 0;
   exw::CkTab ck{};
   std::vector<double> host_tab;
   // host copies of the connectivity and geometry (local-time-stepping set-up)
   std::vector<double> h_vertices;
   std::vector<int> h_neighbor;
+  exw::NcArgs nc_args() const { return exw::NcArgs{n_nc, d_nc_cells, d_nc_kind, d_nc_other}; }
   exw::DevMesh view() const {
     exw::DevMesh m;
     m.n_cells = n_cells;

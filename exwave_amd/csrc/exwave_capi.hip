@@ -115,6 +115,10 @@ static void free_op(exw_op op) {
   hipFree(op->d_errtab);
   hipFree(op->d_cktab);
   hipFree(op->d_posttab);
+  hipFree(op->d_nc_cells);
+  hipFree(op->d_nc_kind);
+  hipFree(op->d_nc_other);
+  hipFree(op->d_nctab);
   hipFree(op->d_partial);
   hipFree(op->d_sum);
   delete op;
@@ -153,8 +157,100 @@ static int upload(exw_op op, T **dptr, const T *host, size_t n) {
 }
 }  // extern "C++"
 
+// the hanging-face records of exw_op_create_nc against the connectivity and
+// the geometry; fills the per-cell face kinds (0 conforming, 1 fine side,
+// 2 coarse side) and partners of the NcArgs layout
+static int check_hanging(int dim, int n_cells, const double *vertices, const int *neighbor,
+                         int n_hanging, const int *hanging, std::vector<int> &kind,
+                         std::vector<int> &other) {
+  const int nv = 1 << dim, nface = 2 * dim, nsub = 1 << (dim - 1), rec = 2 + nsub;
+  kind.assign((size_t)n_cells * nface, 0);
+  other.assign((size_t)n_cells * nface * 4, -1);
+  auto corner = [&](int c, const double *xi, double *x) {
+    const double *V = vertices + (size_t)c * nv * dim;
+    for (int i = 0; i < dim; ++i) x[i] = 0.0;
+    for (int u = 0; u < nv; ++u) {
+      double w = 1.0;
+      for (int d = 0; d < dim; ++d) w *= ((u >> d) & 1) ? xi[d] : 1.0 - xi[d];
+      for (int i = 0; i < dim; ++i) x[i] += w * V[u * dim + i];
+    }
+  };
+  for (int h = 0; h < n_hanging; ++h) {
+    const int *r = hanging + (size_t)h * rec;
+    const int c = r[0], f = r[1];
+    if (c < 0 || c >= n_cells || f < 0 || f >= nface || neighbor[(size_t)c * nface + f] != EXW_NC_FACE ||
+        kind[(size_t)c * nface + f] != 0)
+      return fail(EXW_EINVAL, "hanging record " + std::to_string(h) + ": bad coarse cell/face");
+    kind[(size_t)c * nface + f] = 2;
+    const int d = f >> 1, s = f & 1;
+    // size of the coarse face, for the geometric tolerance
+    double lo[3], hi[3], x[3], y[3], xi[3];
+    for (int e = 0; e < dim; ++e) xi[e] = e == d ? s : 0.0;
+    corner(c, xi, lo);
+    for (int e = 0; e < dim; ++e) xi[e] = e == d ? s : 1.0;
+    corner(c, xi, hi);
+    double diam = 0.0;
+    for (int i = 0; i < dim; ++i) diam = std::max(diam, std::fabs(hi[i] - lo[i]));
+    for (int b = 0; b < nsub; ++b) {
+      const int fc = r[2 + b];
+      if (fc < 0 || fc >= n_cells || neighbor[(size_t)fc * nface + (f ^ 1)] != EXW_NC_FACE ||
+          kind[(size_t)fc * nface + (f ^ 1)] != 0)
+        return fail(EXW_EINVAL, "hanging record " + std::to_string(h) + ": bad fine cell");
+      kind[(size_t)fc * nface + (f ^ 1)] = 1;
+      other[((size_t)fc * nface + (f ^ 1)) * 4 + 0] = c;
+      other[((size_t)fc * nface + (f ^ 1)) * 4 + 1] = b;
+      other[((size_t)c * nface + f) * 4 + b] = fc;
+      // the fine face's corners are the coarse face's map at the subface corners
+      for (int v = 0; v < nsub; ++v) {
+        double xf[3], xc[3];
+        int j = 0;
+        for (int e = 0; e < dim; ++e) {
+          if (e == d) {
+            xf[e] = 1 - s;
+            xc[e] = s;
+          } else {
+            const int bit = (v >> j) & 1;
+            xf[e] = bit;
+            xc[e] = 0.5 * (bit + ((b >> j) & 1));
+            ++j;
+          }
+        }
+        corner(fc, xf, x);
+        corner(c, xc, y);
+        for (int i = 0; i < dim; ++i)
+          if (std::fabs(x[i] - y[i]) > 1e-12 * diam)
+            return fail(EXW_EINVAL, "hanging record " + std::to_string(h) +
+                                        ": fine face is not the subface's image");
+      }
+    }
+  }
+  for (size_t i = 0; i < kind.size(); ++i)
+    if (neighbor[i] == EXW_NC_FACE && kind[i] == 0)
+      return fail(EXW_EINVAL, "EXW_NC_FACE in neighbor[] without a hanging record");
+  return EXW_OK;
+}
+
+static int op_create(exw_ctx ctx, int dim, int degree, int n_cells, const double *vertices,
+                     const int *neighbor, int n_hanging, const int *hanging, const double *rho,
+                     const double *speed, exw_op *out);
+
 int exw_op_create(exw_ctx ctx, int dim, int degree, int n_cells, const double *vertices,
                   const int *neighbor, const double *rho, const double *speed, exw_op *out) {
+  return op_create(ctx, dim, degree, n_cells, vertices, neighbor, 0, nullptr, rho, speed, out);
+}
+
+int exw_op_create_nc(exw_ctx ctx, int dim, int degree, int n_cells, const double *vertices,
+                     const int *neighbor, int n_hanging, const int *hanging, const double *rho,
+                     const double *speed, exw_op *out) {
+  if (n_hanging < 0 || (n_hanging > 0 && !hanging))
+    return fail(EXW_EINVAL, "exw_op_create_nc: bad hanging records");
+  return op_create(ctx, dim, degree, n_cells, vertices, neighbor, n_hanging, hanging, rho, speed,
+                   out);
+}
+
+static int op_create(exw_ctx ctx, int dim, int degree, int n_cells, const double *vertices,
+                     const int *neighbor, int n_hanging, const int *hanging, const double *rho,
+                     const double *speed, exw_op *out) {
   if (!ctx || !out || !vertices || !neighbor || !rho || !speed)
     return fail(EXW_EINVAL, "exw_op_create: null argument");
   if (dim != 2 && dim != 3) return fail(EXW_EINVAL, "exw_op_create: dim must be 2 or 3");
@@ -166,9 +262,16 @@ int exw_op_create(exw_ctx ctx, int dim, int degree, int n_cells, const double *v
   // validate connectivity and materials (source/wave_equation_operations.cc:615-616)
   for (int64_t i = 0; i < (int64_t)n_cells * nface; ++i) {
     const int nb = neighbor[i];
+    if (nb == EXW_NC_FACE && n_hanging > 0) continue;
     if (nb >= n_cells || (nb < 0 && (nb < -3 || nb > -1)))
       return fail(EXW_EINVAL, "set your boundary ids correctly: 1 - soft wall, 2 - hard wall, "
                               "3 - first order ABC (or neighbour index out of range)");
+  }
+  std::vector<int> nc_kind, nc_other;
+  if (n_hanging > 0) {
+    if (int rc = check_hanging(dim, n_cells, vertices, neighbor, n_hanging, hanging, nc_kind,
+                               nc_other))
+      return rc;
   }
   for (int c = 0; c < n_cells; ++c)
     if (!(rho[c] > 0.0) || !(speed[c] > 0.0))
@@ -213,6 +316,36 @@ int exw_op_create(exw_ctx ctx, int dim, int degree, int n_cells, const double *v
       (rc = upload(op, &op->d_vertices, vertices, (size_t)n_cells * nv * dim))) {
     free_op(op);
     return rc;
+  }
+  if (n_hanging > 0) {
+    // the cells that touch a hanging face, in cell order, with their face kinds
+    std::vector<int> cells, kind, other;
+    for (int c = 0; c < n_cells; ++c) {
+      bool any = false;
+      for (int f = 0; f < nface; ++f) any |= nc_kind[(size_t)c * nface + f] != 0;
+      if (!any) continue;
+      cells.push_back(c);
+      kind.insert(kind.end(), nc_kind.begin() + (size_t)c * nface,
+                  nc_kind.begin() + (size_t)(c + 1) * nface);
+      other.insert(other.end(), nc_other.begin() + (size_t)c * nface * 4,
+                   nc_other.begin() + (size_t)(c + 1) * nface * 4);
+    }
+    std::vector<double> nt(4 * N + 2 * N * N);
+    double *X = nt.data(), *W = X + N, *L = W + N, *H = L + 2 * N;
+    cx::gauss01(N, X, W);
+    cx::lagrange(N, X, 0.0, L, nullptr);
+    cx::lagrange(N, X, 1.0, L + N, nullptr);
+    for (int b = 0; b < 2; ++b)
+      for (int t = 0; t < N; ++t) cx::lagrange(N, X, 0.5 * (X[t] + b), H + (b * N + t) * N, nullptr);
+    op->n_nc = (int)cells.size();
+    op->n_hanging = n_hanging;
+    if ((rc = upload(op, &op->d_nc_cells, cells.data(), cells.size())) ||
+        (rc = upload(op, &op->d_nc_kind, kind.data(), kind.size())) ||
+        (rc = upload(op, &op->d_nc_other, other.data(), other.size())) ||
+        (rc = upload(op, &op->d_nctab, nt.data(), nt.size()))) {
+      free_op(op);
+      return rc;
+    }
   }
   std::vector<double> tab, etab, ckt;
   CkTab ck{};
@@ -259,6 +392,11 @@ static int launch_epilogue(exw_op op, const double *src, const Epilogue &ep, int
   OpLaunch l = kernels(op->dim).op[op->degree][precision][op->cart ? 1 : 0];
   if (!l) return fail(EXW_EUNSUP, "operator kernel not compiled");
   EXW_HIP(l(op->view(), src, op->d_tab, op->host_tab.data(), ep, op->ctx->stream));
+  if (op->n_nc > 0) {  // the hanging faces' subface integrals, added to the outputs
+    NcLaunch ln = kernels(op->dim).nc[op->degree];
+    if (!ln) return fail(EXW_EUNSUP, "hanging-face kernel not compiled");
+    EXW_HIP(ln(op->view(), op->d_nctab, op->nc_args(), src, ep, op->ctx->stream));
+  }
   return EXW_OK;
 }
 

@@ -87,6 +87,104 @@ int exw_mesh_hypercube(int dim, int n_intervals, int n_refinements, int transfor
   return EXW_OK;
 }
 
+int exw_mesh_box_refined(int dim, int n_intervals, int transform, double factor, int boundary_id,
+                         const int *refine, int *n_cells, double *vertices, int *neighbor,
+                         int *n_hanging, int *hanging) {
+  if ((dim != 2 && dim != 3) || n_intervals < 1 || transform < 0 || transform > 3 ||
+      boundary_id < 1 || boundary_id > 3 || !refine)
+    return EXW_EINVAL;
+  const int N = n_intervals, nv = 1 << dim, nface = 2 * dim, nsub = 1 << (dim - 1);
+  long nbase = 1;
+  for (int d = 0; d < dim; ++d) nbase *= N;
+  if (nbase * nv > 2000000000L) return EXW_EINVAL;
+  // first output cell of every base cell; refined ones become 2^dim children
+  std::vector<long> first(nbase + 1, 0);
+  long nh = 0;
+  for (long B = 0; B < nbase; ++B) first[B + 1] = first[B] + (refine[B] ? nv : 1);
+  for (long B = 0; B < nbase; ++B) {
+    if (refine[B]) continue;
+    long t = B, stride = 1;
+    for (int d = 0; d < dim; ++d) {
+      const long i = t % N;
+      t /= N;
+      if (i > 0 && refine[B - stride]) ++nh;
+      if (i + 1 < N && refine[B + stride]) ++nh;
+      stride *= N;
+    }
+  }
+  if (n_cells) *n_cells = (int)first[nbase];
+  if (n_hanging) *n_hanging = (int)nh;
+  if (!vertices || !neighbor || !hanging) return EXW_OK;
+  long h = 0;
+  for (long B = 0; B < nbase; ++B) {
+    long idx[3] = {0, 0, 0}, t = B;
+    for (int d = 0; d < dim; ++d) {
+      idx[d] = t % N;
+      t /= N;
+    }
+    double PV[8][3];
+    for (int v = 0; v < nv; ++v) {
+      double x[3] = {0, 0, 0};
+      for (int d = 0; d < dim; ++d) x[d] = double(idx[d] + ((v >> d) & 1)) / N;
+      transform_vertex(dim, transform, factor, x);
+      for (int d = 0; d < dim; ++d) PV[v][d] = x[d];
+    }
+    const int nch = refine[B] ? nv : 1;
+    for (int ch = 0; ch < nch; ++ch) {
+      const long c = first[B] + ch;
+      // children: the parent's multilinear map at the dyadic points
+      for (int v = 0; v < nv; ++v) {
+        double xi[3];
+        for (int d = 0; d < dim; ++d)
+          xi[d] = refine[B] ? 0.5 * (((ch >> d) & 1) + ((v >> d) & 1)) : double((v >> d) & 1);
+        double *out = vertices + (c * nv + v) * dim;
+        for (int i = 0; i < dim; ++i) out[i] = 0.0;
+        for (int u = 0; u < nv; ++u) {
+          double w = 1.0;
+          for (int d = 0; d < dim; ++d) w *= ((u >> d) & 1) ? xi[d] : 1.0 - xi[d];
+          for (int i = 0; i < dim; ++i) out[i] += w * PV[u][i];
+        }
+      }
+      long stride = 1;
+      for (int d = 0; d < dim; ++d) {
+        for (int s = 0; s < 2; ++s) {
+          const int f = 2 * d + s;
+          int *nb = neighbor + c * nface + f;
+          if (refine[B] && ((ch >> d) & 1) != s) {  // sibling
+            *nb = int(first[B] + (ch ^ (1 << d)));
+            continue;
+          }
+          const long ni = idx[d] + (s ? 1 : -1);
+          if (ni < 0 || ni >= N) {
+            *nb = -boundary_id;
+            continue;
+          }
+          const long B2 = B + (s ? stride : -stride);
+          if (refine[B] == refine[B2]) {
+            *nb = int(refine[B] ? first[B2] + (ch ^ (1 << d)) : first[B2]);
+          } else {
+            *nb = EXW_NC_FACE;
+            if (!refine[B]) {  // coarse side: one record with the touching children
+              int *r = hanging + h * (2 + nsub);
+              r[0] = int(c);
+              r[1] = f;
+              for (int b = 0; b < nsub; ++b) {
+                int chn = (s ? 0 : 1) << d, j = 0;
+                for (int e = 0; e < dim; ++e)
+                  if (e != d) chn |= ((b >> j++) & 1) << e;
+                r[2 + b] = int(first[B2] + chn);
+              }
+              ++h;
+            }
+          }
+        }
+        stride *= N;
+      }
+    }
+  }
+  return EXW_OK;
+}
+
 int exw_run(exw_ctx ctx, const exw_problem *p, char *log, int log_len, double *wall_seconds) {
   if (!ctx || !p) return EXW_EINVAL;
   int n_cells = 0, rc;

@@ -272,6 +272,9 @@ int exw_lts_categorize(int dim, int n_cells, const double *vertices, const int *
       !(cfl > 0) || max_clusters < 1 || max_diff < 1)
     return lfail(EXW_EINVAL, "exw_lts_categorize: invalid argument");
   const int nface = 2 * dim, nv = 1 << dim;
+  for (int64_t i = 0; i < (int64_t)n_cells * nface; ++i)
+    if (neighbor[i] == EXW_NC_FACE)
+      return lfail(EXW_EUNSUP, "exw_lts_categorize: meshes with hanging faces are not supported");
   // cluster by cfl * minimum vertex distance (the wave speed is not used,
   // include/cluster_manager.templates.h:57-66)
   std::vector<double> ts(n_cells);
@@ -352,6 +355,8 @@ int exw_lts_destroy(exw_lts L) {
 int exw_lts_create(exw_op op, double cfl, double fastest_dt, int max_clusters, int max_diff,
                    double time0, int recon, exw_lts *out) {
   if (!op || !out || !(fastest_dt > 0)) return lfail(EXW_EINVAL, "exw_lts_create: bad argument");
+  if (op->n_hanging > 0)
+    return lfail(EXW_EUNSUP, "exw_lts_create: meshes with hanging faces are not supported");
   const int n = op->n_cells, nface = 2 * op->dim;
   exw_lts L = new exw_lts_s;
   L->op = op;

@@ -6,6 +6,7 @@
 #include "dg_misc.hpp"
 #include "dg_operator.hpp"
 #include "dg_operator_cart.hpp"
+#include "dg_nc.hpp"
 #include "dg_post.hpp"
 
 namespace exw {
@@ -28,6 +29,9 @@ using ErrLaunch = hipError_t (*)(const DevMesh &, const double *, const DevModes
                                  const double *, double *, int, hipStream_t);
 using EnergyLaunch = hipError_t (*)(const DevMesh &, const double *, const double *, double *,
                                     int, hipStream_t);
+// (mesh, table, cells with hanging faces, src, epilogue, stream)
+using NcLaunch = hipError_t (*)(const DevMesh &, const double *, const NcArgs &, const double *,
+                                const Epilogue &, hipStream_t);
 // (mesh, post table, solution, apply(solution), post, iterations, solver, stream)
 using PostLaunch = hipError_t (*)(const DevMesh &, const double *, const double *, const double *,
                                   double *, int *, const PostSolver &, hipStream_t);
@@ -43,6 +47,7 @@ struct DimKernels {
   ErrLaunch l2err[MAX_DEGREE + 1];
   EnergyLaunch energy[MAX_DEGREE + 1];
   PostLaunch post[MAX_DEGREE + 1];
+  NcLaunch nc[MAX_DEGREE + 1];
   PostErrLaunch post_error[MAX_DEGREE + 1];
   int block[MAX_DEGREE + 1];
   int cells_per_block[MAX_DEGREE + 1];
@@ -121,6 +126,15 @@ hipError_t launch_energy(const DevMesh &m, const double *tab, const double *u, d
 }
 
 template <int DIM, int K>
+hipError_t launch_nc(const DevMesh &m, const double *tab, const NcArgs &nc, const double *src,
+                     const Epilogue &e, hipStream_t s) {
+  if (nc.n == 0) return hipSuccess;
+  hipLaunchKernelGGL((nc_face_kernel<DIM, K>), dim3(nc.n), dim3(NcShape<DIM, K>::BLOCK), 0, s, m,
+                     tab, nc, src, e);
+  return hipGetLastError();
+}
+
+template <int DIM, int K>
 hipError_t launch_post(const DevMesh &m, const double *tab, const double *sol, const double *dudt,
                        double *post, int *iterations, const PostSolver &cfg, hipStream_t s) {
   using S = PostShape<DIM, K>;
@@ -155,6 +169,7 @@ void fill_dim_kernels(DimKernels &t) {
     t.l2err[K] = launch_l2err<DIM, K>;
     t.energy[K] = launch_energy<DIM, K>;
     t.post[K] = launch_post<DIM, K>;
+    t.nc[K] = launch_nc<DIM, K>;
     t.post_error[K] = launch_post_error<DIM, K>;
     t.block[K] = Shape<DIM, K>::BLOCK;
     t.cells_per_block[K] = Shape<DIM, K>::CPB;

@@ -243,6 +243,31 @@ int exw_post_pressure(exw_op op, const double *solution, double *tmp, double *po
 int exw_post_error(exw_op op, const exw_modes *modes, double t, const double *post,
                    double *err_host);
 
+/* ---- meshes with hanging faces ---------------------------------------------
+ * One level of local refinement as deal.II makes it (p4est refinement in
+ * adapt_mesh, source/explicit_wave.cc:265-330; MatrixFree integrates the 2:1
+ * faces on the fine side with the coarse cell evaluated on the subface).
+ * neighbor[] holds EXW_NC_FACE on both sides of such a face; each record of
+ * hanging[] (2 + 2^(dim-1) ints) is {coarse cell, coarse face, the fine cells
+ * by subface position}.  Subface position b has bit j set when the fine face
+ * covers the upper half of the j-th tangential direction (the directions other
+ * than the face's, ascending); the fine cells see the face as coarse face ^ 1.
+ * Child vertices must be the parent's multilinear map at the dyadic points
+ * (deal.II's refinement of straight cells). */
+#define EXW_NC_FACE (-1000)
+/* exw_op_create for a mesh with n_hanging records in hanging[] (checked
+ * against the geometry). */
+int exw_op_create_nc(exw_ctx ctx, int dim, int degree, int n_cells, const double *vertices,
+                     const int *neighbor, int n_hanging, const int *hanging, const double *rho,
+                     const double *speed, exw_op *out);
+/* The n_intervals^dim box of exw_mesh_hypercube with the base cells flagged in
+ * refine[] (lexicographic, xi_0 fastest) refined once: cells numbered base cell
+ * by base cell, children lexicographic inside their parent.  Call with
+ * vertices = neighbor = hanging = NULL for n_cells and n_hanging. */
+int exw_mesh_box_refined(int dim, int n_intervals, int transform, double factor, int boundary_id,
+                         const int *refine, int *n_cells, double *vertices, int *neighbor,
+                         int *n_hanging, int *hanging);
+
 /* Parameters of one solve (the keys of the reference's prm files that the
  * hot path uses, source/parameters.cc:36-100). */
 typedef struct exw_problem {

@@ -40,6 +40,21 @@ def test_mesh_generator_matches_oracle(dim, nint, nref, tr, f):
     assert np.max(np.abs(V - m.vertices)) < 1e-15
 
 
+@pytest.mark.parametrize("dim,n,tr,f,seed", [(2, 4, 0, 0.0, 1), (2, 5, 1, 0.1, 2), (3, 3, 0, 0.0, 3),
+                                          (3, 4, 2, 0.05, 4)])
+def test_refined_mesh_generator_matches_oracle(dim, n, tr, f, seed):
+    """exw_mesh_box_refined (library host code) and the oracle's generator give
+    the same cells, neighbours and hanging records."""
+    import exwave_amd as X
+    ref = (np.random.default_rng(seed).uniform(size=(n,) * dim) < 0.5).astype(np.int32)
+    V, nb, H = X.mesh_box_refined(dim, n, ref, tr, f, 2)
+    m = O.Mesh.box_refined(dim, n, ref, tr, f, 2)
+    assert len(H) > 0
+    assert np.array_equal(nb, m.neighbor)
+    assert np.array_equal(H, m.hanging)
+    assert np.max(np.abs(V - m.vertices)) < 1e-15
+
+
 def test_mesh_generator_rejects_bad_input():
     import exwave_amd as X
     with pytest.raises(X.ExwError):
