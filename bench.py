@@ -69,6 +69,9 @@ def parse():
     ap.add_argument("--nonaffine", action="store_true",
                     help="non-affine 32^3/48^3, k=4/6, FP32 vs FP64 operator (errors + GDoF/s)")
     ap.add_argument("--nonaffine-steps", type=int, default=40)
+    ap.add_argument("--refined", action="store_true",
+                    help="the two-material box with a locally refined half (hanging faces)")
+    ap.add_argument("--refined-time", type=float, default=0.5)
     return ap.parse_args()
 
 
@@ -178,6 +181,98 @@ def nonaffine_measure(steps):
         chk[f"{pname}_rel_max_vs_oracle"] = float(np.max(np.abs(g - u)) / np.max(np.abs(u)))
         chk[f"{pname}_rel_l2_vs_oracle"] = float(np.linalg.norm(g - u) / np.linalg.norm(u))
     print(json.dumps({"nonaffine": out, "oracle_check": chk}), flush=True)
+
+
+def refined_measure(final_time):
+    """The two-material box with a locally refined half: 32^3 base cells on
+    [0,1]^3, the x < 1/2 half refined once (2:1 hanging faces at x = 1/2),
+    rho = c = 1 for x < 1/2 and rho = 2, c = 4 beyond, k = 3, ADER of order 4
+    with the global step cfl/k^1.5 min(h/c) (cfl 0.1: the reference's ADER in
+    3D grows without bound at cfl 0.2 on this set-up, with or without the
+    refinement and the second material, as the CPU oracle shows; the
+    reference's own 3D ADER regression test at cfl 0.2 already carries a weakly
+    growing mode), first-order absorbing
+    walls, a Gaussian pressure pulse of width 0.05 at (0.25, 0.5, 0.5) at rest,
+    run to final_time with the FP64 and with the FP32 operator.  Reported:
+    GDoF/s per ADER step, E(T)/E(0), the FP32 vs FP64 relative L2 difference of
+    the final states, and the state after 2 steps against the CPU oracle."""
+    import numpy as np
+    import torch
+
+    import exwave_amd as X
+    import oracle as O
+    ctx = X.Context(0)
+    stream = torch.cuda.current_stream()
+    n, k, bid = 32, 3, 3
+    ref = np.zeros((n, n, n), dtype=np.int32)
+    ref[:, :, : n // 2] = 1
+    V, nb, H = X.mesh_box_refined(3, n, ref, X.TRANSFORM_NONE, 0.0, bid)
+    centre_x = V[:, :, 0].mean(axis=1)
+    rho = np.where(centre_x < 0.5, 1.0, 2.0)
+    c = np.where(centre_x < 0.5, 1.0, 4.0)
+    op = X.WaveOperator(ctx, 3, k, V, nb, rho, c, hanging=H)
+    # nodal initial field: p = exp(-|x - x0|^2 / 0.05^2), v = 0
+    xg, _ = O.gauss(k + 1)
+    G = np.meshgrid(xg, xg, xg, indexing="ij")
+    xi = np.stack([G[2].ravel(), G[1].ravel(), G[0].ravel()], axis=-1)
+    lo, hi = V[:, 0, :], V[:, -1, :]
+    pts = lo[:, None, :] + (hi - lo)[:, None, :] * xi[None, :, :]
+    r2 = ((pts - np.array([0.25, 0.5, 0.5])) ** 2).sum(axis=-1)
+    u0 = np.zeros((op.n_cells, 4, (k + 1) ** 3))
+    u0[:, 3, :] = np.exp(-r2 / 0.05 ** 2)
+    u0 = u0.ravel()
+    dt0 = 0.1 / k ** 1.5 * op.min_vertex_distance_over_c
+    steps = int(np.ceil(final_time / dt0))
+    dt = final_time / steps
+    rec = {"case": "32^3 base, x<1/2 refined once (147456 cells), rho,c = 1,1 | 2,4, k=3, "
+                   "ADER order 4, absorbing walls, Gaussian pulse at (0.25,0.5,0.5)",
+           "n_cells": op.n_cells, "n_hanging_faces": int(len(H)), "n_dofs": op.n_dofs,
+           "is_cartesian": op.is_cartesian, "dt": dt, "steps": steps, "final_time": final_time}
+    nw = max(op.work_vectors(X.ADER), 1)
+    finals = {}
+    e0 = op.energy(torch.from_numpy(u0).cuda())
+    for pname, prec in (("fp64", X.FP64), ("fp32", X.FP32)):
+        a = op.new_vector()
+        b = torch.from_numpy(u0.copy()).cuda()
+        w = op.new_vector(nw)
+        for _ in range(2):  # warm-up on a copy
+            op.time_step(X.ADER, dt, b.clone(), a, w, precision=prec)
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        ev0.record(stream)
+        for _ in range(steps):
+            a, b = b, a
+            op.time_step(X.ADER, dt, a, b, w, precision=prec)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        ms = ev0.elapsed_time(ev1)
+        rec[f"ms_per_step_{pname}"] = round(ms / steps, 4)
+        rec[f"gdofs_per_step_{pname}"] = round(op.n_dofs * steps / (ms / 1e3) / 1e9, 2)
+        rec[f"energy_ratio_{pname}"] = op.energy(b) / e0
+        finals[pname] = b
+        del a, w
+    d = finals["fp32"] - finals["fp64"]
+    rec["fp32_vs_fp64_rel_l2"] = float(torch.linalg.vector_norm(d) /
+                                       torch.linalg.vector_norm(finals["fp64"]))
+    del finals, d
+    # two steps against the oracle on the same mesh
+    m = O.Mesh.box_refined(3, n, ref, O.TRANSFORM_NONE, 0.0, bid)
+    m.set_material(rho, c)
+    u = u0.copy()
+    for _ in range(2):
+        vnp = np.zeros_like(u)
+        m.time_step(k, O.ADER, dt, u, vnp)
+        u = vnp
+    g = torch.from_numpy(u0.copy()).cuda()
+    h = op.new_vector()
+    w = op.new_vector(nw)
+    for _ in range(2):
+        op.time_step(X.ADER, dt, g, h, w)
+        g, h = h, g
+    gg = g.cpu().numpy()
+    rec["fp64_rel_max_vs_oracle_2_steps"] = float(np.max(np.abs(gg - u)) / np.max(np.abs(u)))
+    print(json.dumps({"locally_refined": rec}), flush=True)
 
 
 def serving_cpu_baseline(reqs, target_s):
@@ -335,6 +430,9 @@ def main():
         return
     if args.nonaffine:
         nonaffine_measure(args.nonaffine_steps)
+        return
+    if args.refined:
+        refined_measure(args.refined_time)
         return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); the "
