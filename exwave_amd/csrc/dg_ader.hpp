@@ -14,6 +14,8 @@
 // buffered in LDS; the level-0 contribution stays in registers, lower levels'
 // contributions live in LDS.
 #pragma once
+#include <utility>
+
 #include "dg_common.hpp"
 
 namespace exw {
@@ -33,6 +35,15 @@ struct CkShape {
   static constexpr int N_STEPS = K >= 2 ? K - 1 : 1;  // steps 0 .. max(0, K-2)
   static constexpr int N_LEVELS = (N_STEPS - 1) / 2 + 1;
   static constexpr int n_of(int l) { return K + 1 - 2 * l; }
+  // points per direction of level l (at least 1), nodes in DIM dimensions, and
+  // the offset of level l >= 1 among the lower levels' contributions
+  static constexpr int nl(int l) { return K + 1 - 2 * l > 1 ? K + 1 - 2 * l : 1; }
+  static constexpr int nd(int dim, int l) { return dim == 2 ? nl(l) * nl(l) : nl(l) * nl(l) * nl(l); }
+  static constexpr int low_off(int dim, int l) {
+    int s = 0;
+    for (int j = 1; j < l; ++j) s += nd(dim, j);
+    return s;
+  }
   static constexpr int TAB_SIZE = [] {
     int s = 0;
     for (int l = 0; l < N_LEVELS; ++l) {
@@ -49,24 +60,114 @@ __device__ __forceinline__ constexpr int ipow_c() {
   return DIM == 2 ? N * N : N * N * N;
 }
 
+namespace ck {
+
+// calls f(integral_constant<int, I>) for I = 0 .. sizeof...(I)-1, at compile time
+template <typename F, int... I>
+__device__ __forceinline__ void static_for_impl(F &&f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+template <int COUNT, typename F>
+__device__ __forceinline__ void static_for(F &&f) {
+  static_for_impl(f, std::make_integer_sequence<int, COUNT>{});
+}
+
+// Physical derivative at node qq of a level field of NN points per direction
+// (in[f][...] in LDS): outv_d = rho^-1 d_d p, outp = rho c^2 div v (the
+// time derivative is minus this).  Cartesian: d_d = (1/h_d) d/dxi_d.
+template <int DIM, int NN, int LD, bool CART, typename Number>
+__device__ __forceinline__ void derive(const Number (*in)[LD], const Number *Dl, const Number *Xl,
+                                       int qq, const Number *ih, Number rho_inv, Number rc2,
+                                       const double *V, Number *outv, Number &outp) {
+  const int qd[3] = {qq % NN, (qq / NN) % NN, DIM == 3 ? qq / (NN * NN) : 0};
+  constexpr int st[3] = {1, NN, NN * NN};
+  if constexpr (CART) {
+    Number div = Number(0);
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      const int q0 = qq - qd[d] * st[d];
+      const Number *Dr = Dl + qd[d] * NN;
+      Number gp = Number(0), gv = Number(0);
+#pragma unroll
+      for (int m = 0; m < NN; ++m) {
+        gp += Dr[m] * in[DIM][q0 + m * st[d]];
+        gv += Dr[m] * in[d][q0 + m * st[d]];
+      }
+      outv[d] = rho_inv * gp * ih[d];
+      div += gv * ih[d];
+    }
+    outp = rc2 * div;
+  } else {
+    Number gr[DIM + 1][DIM];
+#pragma unroll
+    for (int e = 0; e < DIM; ++e) {
+      const int q0 = qq - qd[e] * st[e];
+      const Number *Dr = Dl + qd[e] * NN;
+#pragma unroll
+      for (int f = 0; f <= DIM; ++f) {
+        Number g = Number(0);
+#pragma unroll
+        for (int m = 0; m < NN; ++m) g += Dr[m] * in[f][q0 + m * st[e]];
+        gr[f][e] = g;
+      }
+    }
+    Number xi[DIM], J[DIM * DIM], Ji[DIM * DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) xi[d] = Xl[qd[d]];
+    multilinear_jacobian<DIM>(V, xi, J, Ji);
+    Number div = Number(0);
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      Number pg = Number(0);
+#pragma unroll
+      for (int e = 0; e < DIM; ++e) {
+        pg += gr[DIM][e] * Ji[e * DIM + d];
+        div += gr[d][e] * Ji[e * DIM + d];
+      }
+      outv[d] = rho_inv * pg;
+    }
+    outp = rc2 * div;
+  }
+}
+
+// One tensor pass along direction DIR of the matrix M (NO x NI, row-major) on
+// every field: the input has sizes (A0, A1, A2) with A_DIR = NI, the output
+// the same with NO in direction DIR.  Lane o computes output entry o.
+template <int DIM, int DIR, int NI, int NO, int A0, int A1, int A2, int LD, typename Number>
+__device__ __forceinline__ void pass(const Number *M, const Number (*in)[LD], Number (*out)[LD],
+                                     int o) {
+  constexpr int O0 = DIR == 0 ? NO : A0, O1 = DIR == 1 ? NO : A1, O2 = DIR == 2 ? NO : A2;
+  constexpr int TOTAL = O0 * O1 * (DIM == 3 ? O2 : 1);
+  constexpr int SI[3] = {1, A0, A0 * A1};
+  if (o >= TOTAL) return;
+  int idx[3] = {o % O0, (o / O0) % O1, DIM == 3 ? o / (O0 * O1) : 0};
+  const int r = idx[DIR];
+  idx[DIR] = 0;
+  const int b = idx[0] * SI[0] + idx[1] * SI[1] + (DIM == 3 ? idx[2] * SI[2] : 0);
+  const Number *Mr = M + r * NI;
+#pragma unroll
+  for (int f = 0; f <= DIM; ++f) {
+    Number acc = Number(0);
+#pragma unroll
+    for (int c = 0; c < NI; ++c) acc += Mr[c] * in[f][b + c * SI[DIR]];
+    out[f][o] = acc;
+  }
+}
+
+}  // namespace ck
+
 template <int DIM, int K, typename Number, bool CART>
 __global__ void __launch_bounds__((Shape<DIM, K>::BLOCK))
     ader_ck_kernel(const DevMesh mesh, const double *__restrict__ src,
                    const double *__restrict__ recon, const double *__restrict__ cktab,
-                   const CkTab ck, double t2, double t1, double te_all, const CellSel sel,
+                   const CkTab ckt, double t2, double t1, double te_all, const CellSel sel,
                    double *__restrict__ tempsrc) {
   using S = Shape<DIM, K>;
   using CS = CkShape<K>;
   constexpr int N = S::N, ND = S::ND, NF = S::NF, CPB = S::CPB;
-  // sizes of the lower levels' contributions
-  constexpr int LOW = [] {
-    int s = 0;
-    for (int l = 1; l < CS::N_LEVELS; ++l) {
-      int n = CS::n_of(l);
-      s += DIM == 2 ? n * n : n * n * n;
-    }
-    return s > 0 ? s : 1;
-  }();
+  constexpr int NL = CS::N_LEVELS, NSTEPS = CS::N_STEPS;
+  // the lower levels' contributions, one after the other in s_c
+  constexpr int LOW = CS::low_off(DIM, NL) > 0 ? CS::low_off(DIM, NL) : 1;
 
   __shared__ Number s_a[CPB][NF][ND];
   __shared__ Number s_b[CPB][NF][ND];
@@ -75,10 +176,11 @@ __global__ void __launch_bounds__((Shape<DIM, K>::BLOCK))
   __shared__ Number s_ck[CS::TAB_SIZE];
 
   const int tid = threadIdx.x;
-  for (int i = tid; i < CS::TAB_SIZE && i < ck.size; i += blockDim.x) s_ck[i] = Number(cktab[i]);
+  for (int i = tid; i < CS::TAB_SIZE && i < ckt.size; i += blockDim.x) s_ck[i] = Number(cktab[i]);
   const int lc = tid / ND;
   const int q = tid - lc * ND;
   const bool lane_ok = lc < CPB;
+  const int lcc = lane_ok ? lc : 0;  // LDS slot used by spare lanes (they store nothing)
   // entry -> cell (identity, or through the selection list)
   auto cell_of = [&](int e) { return sel.list ? (e < sel.n ? sel.list[e] : -1) : e; };
   const int cell = cell_of(blockIdx.x * CPB + lc);
@@ -93,18 +195,19 @@ __global__ void __launch_bounds__((Shape<DIM, K>::BLOCK))
           c2 >= 0 && c2 < mesh.n_cells ? mesh.vertices[(size_t)c2 * NVD + i % NVD] : 0.0;
     }
   }
-  Number rho = Number(1), cc = Number(1), hh[DIM];
+  Number rho = Number(1), cc = Number(1), ih[DIM];
 #pragma unroll
-  for (int d = 0; d < DIM; ++d) hh[d] = Number(1);
+  for (int d = 0; d < DIM; ++d) ih[d] = Number(1);
   if (active) {
     rho = Number(mesh.rho[cell]);
     cc = Number(mesh.speed[cell]);
     if constexpr (CART) {
 #pragma unroll
-      for (int d = 0; d < DIM; ++d) hh[d] = Number(mesh.h[(size_t)cell * DIM + d]);
+      for (int d = 0; d < DIM; ++d) ih[d] = Number(1) / Number(mesh.h[(size_t)cell * DIM + d]);
     }
   }
   const Number rho_inv = Number(1) / rho, rc2 = rho * cc * cc;
+  const double *Vc = CART ? nullptr : s_V[CART ? 0 : lcc];
 
   const size_t cbase = (size_t)cell * NF * ND;
   Number u[NF];
@@ -112,84 +215,31 @@ __global__ void __launch_bounds__((Shape<DIM, K>::BLOCK))
   for (int f = 0; f < NF; ++f) u[f] = Number(0);
   if (active) {
 #pragma unroll
-    for (int f = 0; f < NF; ++f) {
-      u[f] = Number(src[cbase + f * ND + q]);
-      s_a[lc][f][q] = u[f];
-    }
+    for (int f = 0; f < NF; ++f) u[f] = Number(src[cbase + f * ND + q]);
+  }
+  if (lane_ok) {
+#pragma unroll
+    for (int f = 0; f < NF; ++f) s_a[lc][f][q] = u[f];
   }
   __syncthreads();
 
-  // Physical derivative of the level field `in` (n points per direction) at
-  // the lane's node: out_v[d] = rho^-1 d_d in_p, out_p = rho c^2 div in_v
-  auto derive = [&](Number(*in)[ND], int n, const Number *Dl, const Number *Xl, int qq,
-                    Number *outv, Number &outp) {
-    int qd[3] = {qq % n, (qq / n) % n, DIM == 3 ? qq / (n * n) : 0};
-    const int st[3] = {1, n, n * n};
-    if constexpr (CART) {
-      Number div = Number(0);
-#pragma unroll
-      for (int d = 0; d < DIM; ++d) {
-        const int q0 = qq - qd[d] * st[d];
-        Number gp = Number(0), gv = Number(0);
-        for (int m = 0; m < n; ++m) {
-          gp += Dl[qd[d] * n + m] * in[DIM][q0 + m * st[d]];
-          gv += Dl[qd[d] * n + m] * in[d][q0 + m * st[d]];
-        }
-        const Number ih = Number(1) / hh[d];
-        outv[d] = rho_inv * gp * ih;
-        div += gv * ih;
-      }
-      outp = rc2 * div;
-    } else {
-      Number gr[NF][DIM];
-#pragma unroll
-      for (int f = 0; f < NF; ++f)
-#pragma unroll
-        for (int e = 0; e < DIM; ++e) {
-          const int q0 = qq - qd[e] * st[e];
-          Number g = Number(0);
-          for (int m = 0; m < n; ++m) g += Dl[qd[e] * n + m] * in[f][q0 + m * st[e]];
-          gr[f][e] = g;
-        }
-      Number xi[DIM], J[DIM * DIM], Ji[DIM * DIM];
-#pragma unroll
-      for (int d = 0; d < DIM; ++d) xi[d] = Xl[qd[d]];
-      multilinear_jacobian<DIM>(s_V[lc], xi, J, Ji);
-      Number div = Number(0);
-#pragma unroll
-      for (int d = 0; d < DIM; ++d) {
-        Number pg = Number(0);
-#pragma unroll
-        for (int e = 0; e < DIM; ++e) {
-          pg += gr[DIM][e] * Ji[e * DIM + d];
-          div += gr[d][e] * Ji[e * DIM + d];
-        }
-        outv[d] = rho_inv * pg;
-      }
-      outp = rc2 * div;
-    }
-  };
-
-  auto tabv = [&](int off, int i) { return s_ck[off + i]; };
-
   // contrib (level 0) and the first derivative
-  const Number c1 = Number(0.5 * ((t2 - te) * (t2 - te) - (t1 - te) * (t1 - te)));
-  Number contrib[NF];
-  Number up[NF];
-  if (lane_ok) {
-    if (recon) {
+  const double a2 = t2 - te, a1 = t1 - te;
+  const Number c1 = Number(0.5 * (a2 * a2 - a1 * a1));
+  Number contrib[NF], up[NF];
+  if (recon) {
 #pragma unroll
-      for (int f = 0; f < NF; ++f) up[f] = active ? Number(recon[cbase + f * ND + q]) : Number(0);
-    } else {
-      Number ov[DIM], op;
-      derive(s_a[lc], N, s_ck + ck.off_D[0], s_ck + ck.off_X[0], q, ov, op);
+    for (int f = 0; f < NF; ++f) up[f] = active ? Number(recon[cbase + f * ND + q]) : Number(0);
+  } else {
+    Number ov[DIM], op;
+    ck::derive<DIM, N, ND, CART>(s_a[lcc], s_ck + ckt.off_D[0], s_ck + ckt.off_X[0], q, ih,
+                                 rho_inv, rc2, Vc, ov, op);
 #pragma unroll
-      for (int d = 0; d < DIM; ++d) up[d] = -ov[d];
-      up[DIM] = -op;
-    }
-#pragma unroll
-    for (int f = 0; f < NF; ++f) contrib[f] = Number(t2 - t1) * u[f] + c1 * up[f];
+    for (int d = 0; d < DIM; ++d) up[d] = -ov[d];
+    up[DIM] = -op;
   }
+#pragma unroll
+  for (int f = 0; f < NF; ++f) contrib[f] = Number(a2 - a1) * u[f] + c1 * up[f];
   __syncthreads();
   if (lane_ok) {
 #pragma unroll
@@ -197,134 +247,97 @@ __global__ void __launch_bounds__((Shape<DIM, K>::BLOCK))
   }
   __syncthreads();
 
-  // steps
-  Number(*cur)[NF][ND] = s_a;
-  Number(*nxt)[NF][ND] = s_b;
-  int low_off = 0;  // offset of the current level's contribution in s_c
-  double fac = -0.5;
-  for (int step = 0; step < CS::N_STEPS; ++step) {
-    fac /= -double(step + 3);
-    const int level = step / 2;
-    const int n = K + 1 - 2 * level;
-    const int nd = DIM == 2 ? n * n : n * n * n;
-    const Number fac_t =
-        Number(fac * (pow(t2 - te, double(step + 3)) - pow(t1 - te, double(step + 3))));
-    if (lane_ok && q < nd) {
+  // the derivative steps; cur/nxt alternate between s_a and s_b at every step
+  // and every projection pass
+  double fac = -0.5, p2 = a2 * a2, p1 = a1 * a1;
+  int parity = 0;  // 0: current field in s_a
+  ck::static_for<NSTEPS>([&](auto S_) {
+    constexpr int STEP = decltype(S_)::value;
+    constexpr int LEVEL = STEP / 2;
+    constexpr int n = CS::nl(LEVEL), nd = CS::nd(DIM, LEVEL);
+    fac /= -double(STEP + 3);
+    p2 *= a2;
+    p1 *= a1;
+    const Number fac_t = Number(fac * (p2 - p1));
+    Number(*cur)[ND] = parity ? s_b[lcc] : s_a[lcc];
+    Number(*nxt)[ND] = parity ? s_a[lcc] : s_b[lcc];
+    if (q < nd) {
       Number ov[DIM], op;
-      derive(cur[lc], n, s_ck + ck.off_D[level], s_ck + ck.off_X[level], q, ov, op);
+      ck::derive<DIM, n, ND, CART>(cur, s_ck + ckt.off_D[LEVEL], s_ck + ckt.off_X[LEVEL], q, ih,
+                                   rho_inv, rc2, Vc, ov, op);
+      if (lane_ok) {
 #pragma unroll
-      for (int d = 0; d < DIM; ++d) nxt[lc][d][q] = ov[d];
-      nxt[lc][DIM][q] = op;
-      if (level == 0) {
+        for (int d = 0; d < DIM; ++d) nxt[d][q] = ov[d];
+        nxt[DIM][q] = op;
+      }
+      if constexpr (LEVEL == 0) {
 #pragma unroll
         for (int d = 0; d < DIM; ++d) contrib[d] += fac_t * ov[d];
         contrib[DIM] += fac_t * op;
-      } else {
-        const bool first = (step & 1) == 0;  // first step of a reduced level sets
+      } else if (lane_ok) {
+        constexpr bool FIRST = (STEP & 1) == 0;  // the first step of a level sets
+        constexpr int OFF = CS::low_off(DIM, LEVEL);
 #pragma unroll
         for (int d = 0; d < DIM; ++d)
-          s_c[lc][d][low_off + q] = (first ? Number(0) : s_c[lc][d][low_off + q]) + fac_t * ov[d];
-        s_c[lc][DIM][low_off + q] =
-            (first ? Number(0) : s_c[lc][DIM][low_off + q]) + fac_t * op;
+          s_c[lc][d][OFF + q] = (FIRST ? Number(0) : s_c[lc][d][OFF + q]) + fac_t * ov[d];
+        s_c[lc][DIM][OFF + q] = (FIRST ? Number(0) : s_c[lc][DIM][OFF + q]) + fac_t * op;
       }
     }
     __syncthreads();
-    auto tmp = cur;
-    cur = nxt;
-    nxt = tmp;
+    parity ^= 1;
     // project to the next level after an odd step that is not the last one
-    if ((step & 1) == 1 && step < K - 2) {
-      const int nl = n - 2;
-      int sz[3] = {n, n, DIM == 3 ? n : 1};
-      const int P = ck.off_P[level + 1];
-      // directions 0.. DIM-1, ping-ponging cur/nxt
-      for (int dir = 0; dir < DIM; ++dir) {
-        if (lane_ok) {
-          for (int f = 0; f < NF; ++f) {
-            // A[r*a_rs + c*a_cs] = P[r*n + c]
-            int sz2[3] = {sz[0], sz[1], sz[2]};
-            const int total_out = (sz2[0] * sz2[1] * sz2[2] / sz2[dir]) * nl;
-            int szo[3] = {sz2[0], sz2[1], sz2[2]};
-            szo[dir] = nl;
-            const int sin[3] = {1, sz2[0], sz2[0] * sz2[1]};
-            for (int o = q; o < total_out; o += ND) {
-              int idx[3] = {o % szo[0], (o / szo[0]) % szo[1], o / (szo[0] * szo[1])};
-              const int r = idx[dir];
-              idx[dir] = 0;
-              const int b = idx[0] * sin[0] + idx[1] * sin[1] + idx[2] * sin[2];
-              Number acc = Number(0);
-              for (int c = 0; c < n; ++c) acc += tabv(P, r * n + c) * cur[lc][f][b + c * sin[dir]];
-              nxt[lc][f][o] = acc;
-            }
-          }
-        }
-        sz[dir] = nl;
+    if constexpr ((STEP & 1) == 1 && STEP < K - 2) {
+      constexpr int NLO = n - 2;
+      const Number *P = s_ck + ckt.off_P[LEVEL + 1];
+      ck::static_for<DIM>([&](auto D_) {
+        constexpr int DIR = decltype(D_)::value;
+        // sizes before this pass: directions < DIR already reduced
+        constexpr int A0 = DIR > 0 ? NLO : n, A1 = DIR > 1 ? NLO : n, A2 = n;
+        Number(*in)[ND] = parity ? s_b[lcc] : s_a[lcc];
+        Number(*out)[ND] = parity ? s_a[lcc] : s_b[lcc];
+        if (lane_ok) ck::pass<DIM, DIR, n, NLO, A0, A1, A2, ND>(P, in, out, q);
         __syncthreads();
-        auto t3 = cur;
-        cur = nxt;
-        nxt = t3;
-      }
-      if (level >= 1) low_off += nd;
+        parity ^= 1;
+      });
     }
-  }
+  });
 
   // interpolate the lower levels' contributions back up and add
-  {
-    // level offsets in s_c
-    int offs[8];
-    int o = 0;
-    for (int l = 1; l < CS::N_LEVELS; ++l) {
-      offs[l] = o;
-      const int n = K + 1 - 2 * l;
-      o += DIM == 2 ? n * n : n * n * n;
-    }
-    for (int l = CS::N_LEVELS - 1; l >= 1; --l) {
-      const int nlo = K + 1 - 2 * l, nhi = nlo + 2;
-      const int V = ck.off_V[l];
-      // interpolate s_c[level l] (nlo^DIM) to nhi^DIM through s_a/s_b
-      Number(*bufA)[NF][ND] = s_a;
-      Number(*bufB)[NF][ND] = s_b;
-      int sz[3] = {nlo, nlo, DIM == 3 ? nlo : 1};
-      for (int dir = 0; dir < DIM; ++dir) {
-        if (lane_ok) {
-          for (int f = 0; f < NF; ++f) {
-            int szo[3] = {sz[0], sz[1], sz[2]};
-            szo[dir] = nhi;
-            const int total_out = szo[0] * szo[1] * szo[2];
-            const int sin[3] = {1, sz[0], sz[0] * sz[1]};
-            for (int oo = q; oo < total_out; oo += ND) {
-              int idx[3] = {oo % szo[0], (oo / szo[0]) % szo[1], oo / (szo[0] * szo[1])};
-              const int r = idx[dir];
-              idx[dir] = 0;
-              const int b = idx[0] * sin[0] + idx[1] * sin[1] + idx[2] * sin[2];
-              Number acc = Number(0);
-              for (int c = 0; c < nlo; ++c) {
-                const Number v = dir == 0 ? s_c[lc][f][offs[l] + b + c * sin[dir]]
-                                          : bufA[lc][f][b + c * sin[dir]];
-                acc += tabv(V, r * nlo + c) * v;
-              }
-              bufB[lc][f][oo] = acc;
-            }
-          }
-        }
-        sz[dir] = nhi;
-        __syncthreads();
-        auto t3 = bufA;
-        bufA = bufB;
-        bufB = t3;
-      }
-      // bufA holds the interpolated contribution at level l-1
-      const int ndhi = DIM == 2 ? nhi * nhi : nhi * nhi * nhi;
-      if (lane_ok && q < ndhi) {
-        if (l - 1 == 0) {
+  if constexpr (NL > 1) {
+    ck::static_for<NL - 1>([&](auto L_) {
+      constexpr int LV = NL - 1 - decltype(L_)::value;  // NL-1 .. 1
+      constexpr int NLO = CS::nl(LV), NHI = NLO + 2;
+      constexpr int NDHI = DIM == 2 ? NHI * NHI : NHI * NHI * NHI;
+      constexpr int OFF = CS::low_off(DIM, LV);
+      const Number *Vt = s_ck + ckt.off_V[LV];
+      // copy the level's contribution to s_a, then DIM passes s_a <-> s_b
+      if (lane_ok && q < CS::nd(DIM, LV))
 #pragma unroll
-          for (int f = 0; f < NF; ++f) contrib[f] += bufA[lc][f][q];
+        for (int f = 0; f < NF; ++f) s_a[lc][f][q] = s_c[lc][f][OFF + q];
+      __syncthreads();
+      int par = 0;
+      ck::static_for<DIM>([&](auto D_) {
+        constexpr int DIR = decltype(D_)::value;
+        constexpr int A0 = DIR > 0 ? NHI : NLO, A1 = DIR > 1 ? NHI : NLO, A2 = NLO;
+        Number(*in)[ND] = par ? s_b[lcc] : s_a[lcc];
+        Number(*out)[ND] = par ? s_a[lcc] : s_b[lcc];
+        if (lane_ok) ck::pass<DIM, DIR, NLO, NHI, A0, A1, A2, ND>(Vt, in, out, q);
+        __syncthreads();
+        par ^= 1;
+      });
+      // the interpolated contribution (level LV-1) is in s_b (DIM odd) or s_a
+      Number(*res)[ND] = par ? s_b[lcc] : s_a[lcc];
+      if (lane_ok && q < NDHI) {
+        if constexpr (LV - 1 == 0) {
+#pragma unroll
+          for (int f = 0; f < NF; ++f) contrib[f] += res[f][q];
         } else {
-          for (int f = 0; f < NF; ++f) s_c[lc][f][offs[l - 1] + q] += bufA[lc][f][q];
+#pragma unroll
+          for (int f = 0; f < NF; ++f) s_c[lc][f][CS::low_off(DIM, LV - 1) + q] += res[f][q];
         }
       }
       __syncthreads();
-    }
+    });
   }
 
   if (active) {

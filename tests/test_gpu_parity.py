@@ -170,7 +170,7 @@ def test_time_step_parity(X, ctx, integ, geom):
 
 
 @pytest.mark.parametrize("dim,k,tr", [(2, 3, 1), (2, 6, 0), (3, 4, 0), (3, 5, 1), (3, 7, 1),
-                                      (2, 1, 0), (3, 2, 1)])
+                                      (2, 1, 0), (3, 2, 1), (3, 8, 0), (2, 8, 1), (3, 6, 0)])
 def test_ader_ck_parity(X, ctx, dim, k, tr):
     _say("ader ck parity", dim, k, tr)
     om, op = _pair(X, ctx, dim, 3, k, tr, 0.08, 2, seed=21)
