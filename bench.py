@@ -69,6 +69,8 @@ def parse():
     ap.add_argument("--nonaffine", action="store_true",
                     help="non-affine 32^3/48^3, k=4/6, FP32 vs FP64 operator (errors + GDoF/s)")
     ap.add_argument("--nonaffine-steps", type=int, default=40)
+    ap.add_argument("--ader", action="store_true",
+                    help="the CK kernel and the ADER step on the membrane workloads")
     ap.add_argument("--refined", action="store_true",
                     help="the two-material box with a locally refined half (hanging faces)")
     ap.add_argument("--refined-time", type=float, default=0.5)
@@ -181,6 +183,55 @@ def nonaffine_measure(steps):
         chk[f"{pname}_rel_max_vs_oracle"] = float(np.max(np.abs(g - u)) / np.max(np.abs(u)))
         chk[f"{pname}_rel_l2_vs_oracle"] = float(np.linalg.norm(g - u) / np.linalg.norm(u))
     print(json.dumps({"nonaffine": out, "oracle_check": chk}), flush=True)
+
+
+def ader_measure():
+    """The ADER step on the membrane workloads (Cartesian, rho = c = 1,
+    Dirichlet walls): the Cauchy-Kowalevski kernel alone (exw_ader_ck: reads the
+    state, writes the time-integrated field, 16 B/DoF algorithmic) and the whole
+    step (CK + the sweep with the update epilogue), FP64 and FP32 operator."""
+    import torch
+
+    import exwave_amd as X
+    ctx = X.Context(0)
+    stream = torch.cuda.current_stream()
+    out = []
+    for dim, k, n1d in ((3, 3, 64), (3, 4, 64), (3, 6, 32), (2, 5, 512)):
+        op = X.WaveOperator.hypercube(ctx, dim, k, n1d, 0, X.TRANSFORM_NONE, 0.0, 2)
+        modes = X.Modes.make(dim, [[1] * dim])
+        u, t = op.new_vector(), op.new_vector()
+        op.project(modes, 0.0, u)
+        w = op.new_vector(max(op.work_vectors(X.ADER), 1))
+        dt = 0.1 / k ** 1.5 * op.min_vertex_distance_over_c
+        rec = {"dim": dim, "degree": k, "cells_per_dir": n1d, "n_dofs": op.n_dofs}
+        for pname, prec in (("fp64", X.FP64), ("fp32", X.FP32)):
+            for what in ("ck", "step"):
+                def once():
+                    if what == "ck":
+                        op.ader_ck(u, None, dt, 0.0, 0.0, t, prec)
+                    else:
+                        op.time_step(X.ADER, dt, u, t, w, precision=prec)
+                once()
+                torch.cuda.synchronize()
+                R = 20
+                ev0 = torch.cuda.Event(enable_timing=True)
+                ev1 = torch.cuda.Event(enable_timing=True)
+                ev0.record(stream)
+                for _ in range(R):
+                    once()
+                ev1.record(stream)
+                torch.cuda.synchronize()
+                ms = ev0.elapsed_time(ev1) / R
+                rec[f"{what}_ms_{pname}"] = round(ms, 4)
+                if what == "ck":
+                    rec[f"ck_GBps_{pname}"] = round(16 * op.n_dofs / (ms / 1e3) / 1e9, 1)
+                else:
+                    rec[f"step_gdofs_{pname}"] = round(op.n_dofs / (ms / 1e3) / 1e9, 2)
+        out.append(rec)
+        print(json.dumps(rec), file=sys.stderr, flush=True)
+        del u, t, w, op
+        torch.cuda.empty_cache()
+    print(json.dumps({"ader": out}), flush=True)
 
 
 def refined_measure(final_time):
@@ -433,6 +484,9 @@ def main():
         return
     if args.refined:
         refined_measure(args.refined_time)
+        return
+    if args.ader:
+        ader_measure()
         return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); the "

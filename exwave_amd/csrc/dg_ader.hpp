@@ -90,8 +90,9 @@ __device__ __forceinline__ void derive(const Number (*in)[LD], const Number *Dl,
       Number gp = Number(0), gv = Number(0);
 #pragma unroll
       for (int m = 0; m < NN; ++m) {
-        gp += Dr[m] * in[DIM][q0 + m * st[d]];
-        gv += Dr[m] * in[d][q0 + m * st[d]];
+        const Number dm = Dr[m];
+        gp += dm * in[DIM][q0 + m * st[d]];
+        gv += dm * in[d][q0 + m * st[d]];
       }
       outv[d] = rho_inv * gp * ih[d];
       div += gv * ih[d];
@@ -102,12 +103,14 @@ __device__ __forceinline__ void derive(const Number (*in)[LD], const Number *Dl,
 #pragma unroll
     for (int e = 0; e < DIM; ++e) {
       const int q0 = qq - qd[e] * st[e];
-      const Number *Dr = Dl + qd[e] * NN;
+      Number dr[NN];
+#pragma unroll
+      for (int m = 0; m < NN; ++m) dr[m] = Dl[qd[e] * NN + m];
 #pragma unroll
       for (int f = 0; f <= DIM; ++f) {
         Number g = Number(0);
 #pragma unroll
-        for (int m = 0; m < NN; ++m) g += Dr[m] * in[f][q0 + m * st[e]];
+        for (int m = 0; m < NN; ++m) g += dr[m] * in[f][q0 + m * st[e]];
         gr[f][e] = g;
       }
     }
@@ -144,12 +147,16 @@ __device__ __forceinline__ void pass(const Number *M, const Number (*in)[LD], Nu
   const int r = idx[DIR];
   idx[DIR] = 0;
   const int b = idx[0] * SI[0] + idx[1] * SI[1] + (DIM == 3 ? idx[2] * SI[2] : 0);
-  const Number *Mr = M + r * NI;
+  // the lane's matrix row once in registers (the output may alias nothing
+  // the compiler can prove, so it would reload the row for every field)
+  Number mr[NI];
+#pragma unroll
+  for (int c = 0; c < NI; ++c) mr[c] = M[r * NI + c];
 #pragma unroll
   for (int f = 0; f <= DIM; ++f) {
     Number acc = Number(0);
 #pragma unroll
-    for (int c = 0; c < NI; ++c) acc += Mr[c] * in[f][b + c * SI[DIR]];
+    for (int c = 0; c < NI; ++c) acc += mr[c] * in[f][b + c * SI[DIR]];
     out[f][o] = acc;
   }
 }
