@@ -19,7 +19,7 @@ from .capi import (ADER, ADER_ADCONFULL, ADER_LTS, CLRK4, EXPL_EULER, FP32, FP64
                    PRECISION_AUTO, INTEGRATORS, LSRK33R2, LSRK45R2, LSRK45R3, SSPRK,
                    LSRK59R2, TRANSFORM_AXIS_2PI, TRANSFORM_NONE, TRANSFORM_PRODUCT_2PI,
                    TRANSFORM_REFERENCE_SINE, NC_FACE, Epilogue, ExwError, Modes, Problem, check,
-                   load, mesh_box_refined, mesh_hypercube)
+                   load, mesh_box_refined, mesh_hypercube, write_vtu)
 
 
 def _ptr(t):
@@ -151,6 +151,12 @@ class WaveOperator:
         out = C.c_double(0.0)
         check(load().exw_energy(self.h, _ptr(u), C.byref(out)))
         return out.value
+
+    def estimate_error(self, solution, tmp, error_per_cell):
+        """estimate_error: tmp <- apply(solution), error_per_cell (n_cells doubles on
+        the device) <- sqrt(int_K |grad p - v_t|^2)."""
+        check(load().exw_estimate_error(self.h, _ptr(solution), _ptr(tmp),
+                                        _ptr(error_per_cell)))
 
     @property
     def n_post(self):

@@ -111,6 +111,8 @@ SIGNATURES = [
     ("exw_l2_errors", _I, [_VP, C.POINTER(Modes), _D, _VP, _DP]),
     ("exw_energy", _I, [_VP, _VP, _DP]),
     ("exw_post_n_values", C.c_int64, [_VP]),
+    ("exw_estimate_error", _I, [_VP, _VP, _VP, _VP]),
+    ("exw_write_vtu", _I, [_I, _I, _I, _DP, _DP, _DP, _DP, _DP, _DP, _DP, C.c_char_p]),
     ("exw_op_create_nc", _I, [_VP, _I, _I, _I, _DP, _IP, _I, _IP, _DP, _DP, C.POINTER(_VP)]),
     ("exw_mesh_box_refined", _I, [_I, _I, _I, _D, _I, _IP, _IP, _DP, _IP, _IP, _IP]),
     ("exw_post_pressure", _I, [_VP, _VP, _VP, _VP, _VP]),
@@ -160,6 +162,23 @@ def check(rc):
 
 
 NC_FACE = -1000  # EXW_NC_FACE: neighbour entry of a hanging (2:1) face
+
+
+def write_vtu(filename, dim, degree, vertices, solution, error=None, error_estimate=None,
+              post_pressure=None, cluster_id=None, time_step=None):
+    """exw_write_vtu from host arrays (numpy)."""
+    import numpy as np
+
+    def p(a):
+        if a is None:
+            return None
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        return a.ctypes.data_as(C.POINTER(C.c_double))
+    V = np.ascontiguousarray(vertices, dtype=np.float64)
+    keep = [np.ascontiguousarray(x, dtype=np.float64) if x is not None else None
+            for x in (solution, error, error_estimate, post_pressure, cluster_id, time_step)]
+    check(load().exw_write_vtu(dim, degree, V.shape[0], p(V), *[p(x) for x in keep],
+                               str(filename).encode()))
 
 
 def mesh_box_refined(dim, n_intervals, refine, transform=TRANSFORM_NONE, factor=0.0,

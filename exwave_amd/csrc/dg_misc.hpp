@@ -154,6 +154,61 @@ __global__ void __launch_bounds__(256)
   if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
+// estimate_error (source/wave_equation_operations.cc:871-898): per cell
+// err = sqrt(int_K |grad p_h - v_t|^2) at the QGauss(k+1) points (the nodes),
+// dudt = apply(sol).  One lane per node; the cell's sum is taken by its first
+// lane in node order (deterministic).
+template <int DIM, int K>
+__global__ void __launch_bounds__((Shape<DIM, K>::BLOCK))
+    estimate_error_kernel(const DevMesh mesh, const double *__restrict__ tab,
+                          const double *__restrict__ sol, const double *__restrict__ dudt,
+                          double *__restrict__ err) {
+  using S = Shape<DIM, K>;
+  using TO = TabOff<S::N>;
+  constexpr int N = S::N, ND = S::ND, NF = S::NF, CPB = S::CPB;
+  __shared__ double s_p[CPB][ND];
+  __shared__ double s_e[CPB][ND];
+  __shared__ double s_tab[TO::SIZE];
+  const int tid = threadIdx.x, lc = tid / ND, q = tid - lc * ND;
+  const int cell = blockIdx.x * CPB + lc;
+  const bool active = lc < CPB && cell < mesh.n_cells;
+  for (int i = tid; i < TO::SIZE; i += blockDim.x) s_tab[i] = tab[i];
+  if (active) s_p[lc][q] = sol[((size_t)cell * NF + DIM) * ND + q];
+  __syncthreads();
+  if (active) {
+    const int qi[3] = {q % N, (q / N) % N, DIM == 3 ? q / (N * N) : 0};
+    double xi[DIM], J[DIM * DIM], Ji[DIM * DIM], w = 1.0, gref[DIM];
+#pragma unroll
+    for (int e = 0; e < DIM; ++e) {
+      xi[e] = s_tab[TO::X + qi[e]];
+      w *= s_tab[TO::W + qi[e]];
+      const int se = pow_n<N>(e), q0 = q - qi[e] * se;
+      double g = 0.0;
+#pragma unroll
+      for (int m = 0; m < N; ++m) g += s_tab[TO::D + qi[e] * N + m] * s_p[lc][q0 + m * se];
+      gref[e] = g;
+    }
+    const double det =
+        multilinear_jacobian<DIM>(mesh.vertices + (size_t)cell * (1 << DIM) * DIM, xi, J, Ji);
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) {
+      double gp = 0.0;
+#pragma unroll
+      for (int e = 0; e < DIM; ++e) gp += Ji[e * DIM + i] * gref[e];
+      const double dl = gp - dudt[((size_t)cell * NF + i) * ND + q];
+      s += dl * dl;
+    }
+    s_e[lc][q] = s * fabs(det) * w;
+  }
+  __syncthreads();
+  if (active && q == 0) {
+    double s = 0.0;
+    for (int i = 0; i < ND; ++i) s += s_e[lc][i];
+    err[cell] = sqrt(s);
+  }
+}
+
 // fixed-order final sum of `n` partial vectors of width `w`
 static __global__ void __launch_bounds__(256)
     sum_partials_kernel(const double *__restrict__ partial, int n, int w, double *__restrict__ out) {

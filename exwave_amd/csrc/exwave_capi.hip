@@ -798,6 +798,17 @@ int exw_l2_errors(exw_op op, const exw_modes *modes, double t, const double *u,
   return EXW_OK;
 }
 
+int exw_estimate_error(exw_op op, const double *solution, double *tmp, double *error_per_cell) {
+  if (!op || !solution || !tmp || !error_per_cell)
+    return fail(EXW_EINVAL, "exw_estimate_error: null argument");
+  if (tmp == solution) return fail(EXW_EINVAL, "exw_estimate_error: tmp aliases solution");
+  // apply(solution, tmp_vector) first (source/wave_equation_operations.cc:876)
+  if (int rc = exw_op_apply(op, solution, tmp, EXW_FP64)) return rc;
+  EXW_HIP(kernels(op->dim).estimate[op->degree](op->view(), op->d_tab, solution, tmp,
+                                                 error_per_cell, op->ctx->stream));
+  return EXW_OK;
+}
+
 int64_t exw_post_n_values(exw_op op) { return op ? op->n_post() : -1; }
 
 int exw_post_pressure(exw_op op, const double *solution, double *tmp, double *post,

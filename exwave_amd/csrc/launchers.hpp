@@ -29,6 +29,9 @@ using ErrLaunch = hipError_t (*)(const DevMesh &, const double *, const DevModes
                                  const double *, double *, int, hipStream_t);
 using EnergyLaunch = hipError_t (*)(const DevMesh &, const double *, const double *, double *,
                                     int, hipStream_t);
+// (mesh, basis table, solution, apply(solution), per-cell error, stream)
+using EstLaunch = hipError_t (*)(const DevMesh &, const double *, const double *, const double *,
+                                 double *, hipStream_t);
 // (mesh, table, cells with hanging faces, src, epilogue, stream)
 using NcLaunch = hipError_t (*)(const DevMesh &, const double *, const NcArgs &, const double *,
                                 const Epilogue &, hipStream_t);
@@ -48,6 +51,7 @@ struct DimKernels {
   EnergyLaunch energy[MAX_DEGREE + 1];
   PostLaunch post[MAX_DEGREE + 1];
   NcLaunch nc[MAX_DEGREE + 1];
+  EstLaunch estimate[MAX_DEGREE + 1];
   PostErrLaunch post_error[MAX_DEGREE + 1];
   int block[MAX_DEGREE + 1];
   int cells_per_block[MAX_DEGREE + 1];
@@ -126,6 +130,17 @@ hipError_t launch_energy(const DevMesh &m, const double *tab, const double *u, d
 }
 
 template <int DIM, int K>
+hipError_t launch_estimate(const DevMesh &m, const double *tab, const double *sol,
+                           const double *dudt, double *err, hipStream_t s) {
+  using S = Shape<DIM, K>;
+  const int grid = (m.n_cells + S::CPB - 1) / S::CPB;
+  if (grid == 0) return hipSuccess;
+  hipLaunchKernelGGL((estimate_error_kernel<DIM, K>), dim3(grid), dim3(S::BLOCK), 0, s, m, tab,
+                     sol, dudt, err);
+  return hipGetLastError();
+}
+
+template <int DIM, int K>
 hipError_t launch_nc(const DevMesh &m, const double *tab, const NcArgs &nc, const double *src,
                      const Epilogue &e, hipStream_t s) {
   if (nc.n == 0) return hipSuccess;
@@ -170,6 +185,7 @@ void fill_dim_kernels(DimKernels &t) {
     t.energy[K] = launch_energy<DIM, K>;
     t.post[K] = launch_post<DIM, K>;
     t.nc[K] = launch_nc<DIM, K>;
+    t.estimate[K] = launch_estimate<DIM, K>;
     t.post_error[K] = launch_post_error<DIM, K>;
     t.block[K] = Shape<DIM, K>::BLOCK;
     t.cells_per_block[K] = Shape<DIM, K>::CPB;

@@ -217,6 +217,26 @@ int exw_lts_step(exw_lts L, double time_now, const double *src, double *dst);
 int exw_mesh_hypercube(int dim, int n_intervals, int n_refinements, int transform, double factor,
                        int boundary_id, int *n_cells, double *vertices, int *neighbor);
 
+/* WaveEquationOperation::estimate_error (source/wave_equation_operations.cc:
+ * 871-898), the refinement indicator of adapt_mesh and the 'Error_estimate'
+ * output field: tmp <- apply(solution), then per cell
+ *   error_per_cell[c] = sqrt( int_K |grad p_h - v_t|^2 ) (QGauss(k+1)),
+ * v_t the velocity part of tmp.  error_per_cell: device, n_cells doubles;
+ * asynchronous. */
+int exw_estimate_error(exw_op op, const double *solution, double *tmp, double *error_per_cell);
+
+/* output_results (source/explicit_wave.cc:339-436) as a VTU file, host arrays:
+ * each cell subdivided degree times per direction (build_patches(mapping,
+ * fe_degree)), the fields evaluated at the equispaced points and written as
+ * linear sub-cells.  Point data: solution_velocity, solution_pressure and, if
+ * given, error_velocity/error_pressure (error: same layout as solution) and
+ * post_pressure ((k+2)^dim Gauss-Lobatto values per cell); cell data, if given:
+ * Error_estimate, cluster_id, time_step (n_cells values).  ASCII. */
+int exw_write_vtu(int dim, int degree, int n_cells, const double *vertices,
+                  const double *solution, const double *error, const double *error_estimate,
+                  const double *post_pressure, const double *cluster_id, const double *time_step,
+                  const char *filename);
+
 /* ---- post-processed pressure ----------------------------------------------
  * WaveEquationOperation::compute_post_pressure
  * (source/wave_equation_operations.cc:786-848): tmp <- apply(solution) (the

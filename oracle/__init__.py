@@ -124,6 +124,7 @@ def lib():
         L.ox_lts_cells.argtypes = [C.c_void_p, ip, ip]
         L.ox_lts_step.argtypes = [C.c_void_p, C.c_double, dp, dp]
         L.ox_gauss_lobatto.argtypes = [C.c_int, dp]
+        L.ox_estimate_error.argtypes = [C.POINTER(_Mesh), C.c_int, dp, dp]
         L.ox_post_pressure.argtypes = [C.POINTER(_Mesh), C.c_int, dp, dp]
         L.ox_post_set_solver.argtypes = [C.c_double, C.c_double, C.c_int, C.c_void_p, C.c_int]
         L.ox_post_error.restype = C.c_double
@@ -239,6 +240,12 @@ class Mesh:
 
     def energy(self, degree, u):
         return lib().ox_energy(self._m, degree, _p(u))
+
+    def estimate_error(self, degree, u):
+        """estimate_error: per cell sqrt(int |grad p - v_t|^2)."""
+        out = np.zeros(self.n_cells)
+        lib().ox_estimate_error(self._m, degree, _p(u), _p(out))
+        return out
 
     def post_pressure(self, degree, u, rel_tol=1e-11, abs_tol=1e-20, max_iter_factor=6,
                       iterations=False, mean_free=False):

@@ -1231,6 +1231,51 @@ double ox_energy(const ox_mesh *m, int degree, const double *u) {
   return E;
 }
 
+/* WaveEquationOperation::estimate_error (source/wave_equation_operations.cc:
+ * 871-898): tmp = apply(solution); per cell
+ * err_K = sqrt( int_K |grad p_h - v_t|^2 dx ) at the QGauss(k+1) points (the
+ * nodes), v_t the velocity part of tmp, restated literally. */
+void ox_estimate_error(const ox_mesh *m, int degree, const double *solution, double *err) {
+  const int dim = m->dim, n = degree + 1, nf = dim + 1, nd = ipow(n, dim), nv = 1 << dim;
+  const long N = ox_n_dofs(m, degree);
+  double *tmp = (double *)malloc(sizeof(double) * N);
+  ox_apply(m, degree, solution, tmp, 1.0);
+  ox_basis b;
+  ox_basis_init(&b, degree);
+#pragma omp parallel
+  {
+    double *g = (double *)malloc(sizeof(double) * dim * nd);
+#pragma omp for schedule(static)
+    for (int c = 0; c < m->n_cells; ++c) {
+      const double *V = m->vertices + (long)c * nv * dim;
+      const double *p = solution + ((long)c * nf + dim) * nd;
+      const double *vt = tmp + (long)c * nf * nd;
+      const int sz[3] = {n, n, n};
+      for (int e = 0; e < dim; ++e) tensor_apply(dim, sz, e, b.D, n, p, g + (long)e * nd, 0, 0);
+      double s = 0.0;
+      for (int q = 0; q < nd; ++q) {
+        double xi[3], J[9], Ji[9], w = 1.0;
+        int t = q;
+        for (int d = 0; d < dim; ++d) {
+          xi[d] = b.x[t % n];
+          w *= b.w[t % n];
+          t /= n;
+        }
+        const double det = multilinear_jacobian(dim, V, xi, J, Ji);
+        for (int i = 0; i < dim; ++i) {
+          double gp = 0.0;
+          for (int e = 0; e < dim; ++e) gp += Ji[e * dim + i] * g[(long)e * nd + q];
+          const double dlt = gp - vt[(long)i * nd + q];
+          s += dlt * dlt * fabs(det) * w;
+        }
+      }
+      err[c] = sqrt(s);
+    }
+    free(g);
+  }
+  free(tmp);
+}
+
 /* ------------------------------------------------------------------------ */
 /* post-processed pressure (compute_post_pressure,                            */
 /* source/wave_equation_operations.cc:786-848)                                */

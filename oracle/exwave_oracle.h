@@ -167,6 +167,10 @@ int ox_run(const ox_problem *p, char *log, int log_len, double *final_state_ener
 
 long ox_n_dofs(const ox_mesh *m, int degree);
 
+/* estimate_error (source/wave_equation_operations.cc:871-898): err[c] =
+ * sqrt(int_K |grad p_h - v_t|^2) with v_t the velocity of apply(solution). */
+void ox_estimate_error(const ox_mesh *m, int degree, const double *solution, double *err);
+
 /* Post-processed pressure (compute_post_pressure,
  * source/wave_equation_operations.cc:786-848): per cell the local Neumann
  * problem (grad phi, grad p*) = (grad phi, -rho dv/dt) on Q_{k+1} (Gauss-Lobatto

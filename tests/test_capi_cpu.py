@@ -106,3 +106,77 @@ def test_request_initial_error_matches_oracle(dim, k, r, modes, amps):
     assert abs(e0.value - err[0]) <= 1e-9 * err[0] + 1e-15, (e0.value, err[0])
     assert abs(n0.value - err[2]) <= 1e-12 * err[2]
     assert pr.value == (capi.FP32 if sc.value >= 10.0 else capi.FP64)
+
+
+@pytest.mark.parametrize("dim,k", [(2, 2), (3, 3)])
+def test_write_vtu(tmp_path, dim, k):
+    """exw_write_vtu: counts, point coordinates (the multilinear map at the
+    equispaced points), solution and post-pressure values (the polynomials
+    evaluated independently here), cell data, and sub-cells with positive
+    orientation in VTK's corner order."""
+    import xml.etree.ElementTree as ET
+    import exwave_amd as X
+    m = O.Mesh(dim, 2, 0, O.TRANSFORM_REFERENCE_SINE, 0.1)
+    rng = np.random.default_rng(dim)
+    u = rng.standard_normal(m.n_dofs(k))
+    post = rng.standard_normal(m.n_cells * (k + 2) ** dim)
+    est = rng.uniform(size=m.n_cells)
+    fn = tmp_path / "out.vtu"
+    X.write_vtu(fn, dim, k, m.vertices, u, error_estimate=est, post_pressure=post)
+    root = ET.parse(fn).getroot()
+    piece = root.find("UnstructuredGrid/Piece")
+    npt, ncl = int(piece.get("NumberOfPoints")), int(piece.get("NumberOfCells"))
+    assert npt == m.n_cells * (k + 1) ** dim and ncl == m.n_cells * k ** dim
+
+    def arr(path, name=None):
+        for a in piece.iter("DataArray"):
+            if name is None or a.get("Name") == name:
+                return np.array(a.text.split(), dtype=float)
+    pts = arr(None, "Points").reshape(npt, 3)
+    conn = arr(None, "connectivity").astype(int).reshape(ncl, 1 << dim)
+    types = arr(None, "types")
+    assert np.all(types == (9 if dim == 2 else 12))
+    # expected points and values
+    xe = np.arange(k + 1) / k
+    G = np.meshgrid(*([xe] * dim), indexing="ij")
+    xi = np.stack([G[dim - 1 - d].ravel() for d in range(dim)], axis=-1)  # xi_0 fastest
+
+    def lag(nodes, x):
+        return np.array([[np.prod([(xx - nodes[j]) / (nodes[i] - nodes[j])
+                                   for j in range(len(nodes)) if j != i])
+                          for i in range(len(nodes))] for xx in x])
+    xg, _ = O.gauss(k + 1)
+    Lg = lag(xg, xe)
+    Lp = lag(O.gauss_lobatto(k + 2), xe)
+
+    def eval_tensor(L, vals):  # vals: n^dim, xi_0 fastest
+        n = L.shape[1]
+        t = vals.reshape((n,) * dim)  # axes (xi_{dim-1}, ..., xi_0)
+        for ax in range(dim):
+            t = np.tensordot(L, t, axes=([1], [ax]))
+            t = np.moveaxis(t, 0, ax)
+        return t.ravel()
+    V = m.vertices
+    nd = (k + 1) ** dim
+    uu = u.reshape(m.n_cells, dim + 1, nd)
+    want_p, want_x, want_post = [], [], []
+    for c in range(m.n_cells):
+        w = np.ones((len(xi), 1 << dim))
+        for v in range(1 << dim):
+            for d in range(dim):
+                w[:, v] *= xi[:, d] if (v >> d) & 1 else 1 - xi[:, d]
+        want_x.append(w @ V[c])
+        want_p.append(eval_tensor(Lg, uu[c, dim]))
+        want_post.append(eval_tensor(Lp, post[c * (k + 2) ** dim:(c + 1) * (k + 2) ** dim]))
+    assert np.allclose(pts[:, :dim], np.concatenate(want_x), atol=1e-9)
+    assert np.allclose(arr(None, "solution_pressure"), np.concatenate(want_p), atol=1e-8)
+    assert np.allclose(arr(None, "post_pressure"), np.concatenate(want_post), atol=1e-8)
+    assert np.allclose(arr(None, "Error_estimate"), np.repeat(est, k ** dim), atol=1e-9)
+    # orientation of every sub-cell
+    P0 = pts[conn[:, 0], :dim]
+    if dim == 2:
+        a, b = pts[conn[:, 1], :2] - P0, pts[conn[:, 3], :2] - P0
+        assert np.all(a[:, 0] * b[:, 1] - a[:, 1] * b[:, 0] > 0)
+    else:
+        a, b, c3 = (pts[conn[:, i]] - pts[conn[:, 0]] for i in (1, 3, 4))
+        assert np.all(np.einsum("ij,ij->i", np.cross(a, b), c3) > 0)

@@ -119,3 +119,47 @@ def test_post_pressure_rejects_aliasing(X, ctx):
     post = torch.zeros(op.n_post, dtype=torch.float64, device="cuda")
     with pytest.raises(RuntimeError):
         op.post_pressure(src, src, post)
+
+
+@pytest.mark.parametrize("case", [(2, 5, O.TRANSFORM_REFERENCE_SINE, 0.2, 3, 1),
+                                  (3, 3, O.TRANSFORM_NONE, 0.0, 4, 2),
+                                  (3, 2, O.TRANSFORM_AXIS_2PI, 0.05, 6, None)],
+                         ids=lambda c: "d%d_k%d" % (c[0], c[4]))
+def test_estimate_error_matches_oracle(X, ctx, case):
+    """exw_estimate_error (estimate_error, source/wave_equation_operations.cc:
+    871-898) against the oracle, <= 1e-12 relative per cell."""
+    import torch
+    dim, nint, tr, fac, k, seed = case
+    _say("estimate error", case)
+    om = O.Mesh(dim, nint, 0, tr, fac)
+    if seed is not None:
+        rng = np.random.default_rng(seed)
+        om.set_material(rng.uniform(0.5, 2.0, om.n_cells), rng.uniform(0.5, 2.0, om.n_cells))
+    op = X.WaveOperator(ctx, dim, k, om.vertices, om.neighbor, om.rho, om.speed)
+    u = np.random.default_rng(3).standard_normal(om.n_dofs(k))
+    src = torch.from_numpy(u).cuda()
+    tmp = torch.zeros_like(src)
+    err = torch.zeros(om.n_cells, dtype=torch.float64, device="cuda")
+    op.estimate_error(src, tmp, err)
+    torch.cuda.synchronize()
+    want = om.estimate_error(k, u)
+    assert _rel(err.cpu().numpy(), want) <= 1e-12
+
+
+def test_estimate_error_hanging_faces(X, ctx):
+    """The indicator on a mesh with hanging faces (apply includes the subface
+    integrals) against the oracle."""
+    import torch
+    ref = np.zeros((3, 3, 3), dtype=np.int32)
+    ref[:, :, :1] = 1
+    om = O.Mesh.box_refined(3, 3, ref)
+    k = 3
+    op = X.WaveOperator(ctx, 3, k, om.vertices, om.neighbor, om.rho, om.speed,
+                        hanging=om.hanging)
+    u = np.random.default_rng(8).standard_normal(om.n_dofs(k))
+    src = torch.from_numpy(u).cuda()
+    tmp = torch.zeros_like(src)
+    err = torch.zeros(om.n_cells, dtype=torch.float64, device="cuda")
+    op.estimate_error(src, tmp, err)
+    torch.cuda.synchronize()
+    assert _rel(err.cpu().numpy(), om.estimate_error(k, u)) <= 1e-12

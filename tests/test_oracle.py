@@ -473,3 +473,30 @@ def test_post_pressure_keeps_cell_average():
                 det[i, j] = abs(dx0[0] * dx1[1] - dx0[1] * dx1[0])
         W = np.outer(wq, wq) * det
         assert abs((W * ph).sum() - (W * ps).sum()) <= 1e-13 * max(1.0, abs(W * ph).sum())
+
+
+def test_estimate_error_closed_form():
+    """p = a.x, v = 0: on every cell without a boundary face the operator
+    gives v_t = -a / rho, so the indicator is |a| (1 + 1/rho) sqrt(|K|)
+    (the reference's literal grad p - v_t), with random rho per cell on a box
+    mesh."""
+    dim, k = 3, 2
+    m = O.Mesh(dim, 4, 0, O.TRANSFORM_NONE, 0.0, 2)
+    rng = np.random.default_rng(4)
+    rho = rng.uniform(0.5, 2.0, m.n_cells)
+    m.set_material(rho, rng.uniform(0.5, 2.0, m.n_cells))
+    a = np.array([0.3, -1.2, 0.7])
+    x, _ = O.gauss(k + 1)
+    V = m.vertices
+    lo, hi = V[:, 0, :], V[:, -1, :]
+    G = np.meshgrid(x, x, x, indexing="ij")
+    xi = np.stack([G[2].ravel(), G[1].ravel(), G[0].ravel()], axis=-1)
+    pts = lo[:, None, :] + (hi - lo)[:, None, :] * xi[None]
+    u = np.zeros((m.n_cells, dim + 1, (k + 1) ** dim))
+    u[:, dim] = pts @ a
+    err = m.estimate_error(k, u.ravel())
+    vol = np.prod(hi - lo, axis=1)
+    want = np.linalg.norm(a) * (1 + 1 / rho) * np.sqrt(vol)
+    interior = (m.neighbor >= 0).all(axis=1)
+    assert interior.sum() > 0
+    assert np.allclose(err[interior], want[interior], rtol=1e-12, atol=0)
