@@ -180,6 +180,41 @@ class WaveOperator:
         return out.value
 
 
+def refinement_flags(error_per_cell, level, min_level, max_level, max_error_init):
+    """exw_refinement_flags: adapt_mesh's marking; 1 refine, -1 coarsen, 0 keep."""
+    import numpy as np
+    e = np.ascontiguousarray(error_per_cell, dtype=np.float64)
+    lv = np.ascontiguousarray(level, dtype=np.int32)
+    fl = np.zeros(e.size, dtype=np.int32)
+    ip = C.POINTER(C.c_int)
+    check(load().exw_refinement_flags(e.size, e.ctypes.data_as(C.POINTER(C.c_double)),
+                                      lv.ctypes.data_as(ip), min_level, max_level,
+                                      max_error_init, fl.ctypes.data_as(ip)))
+    return fl
+
+
+def adapt_box_refine(dim, n_intervals, refine_old, flags):
+    """exw_adapt_box_refine: the new base-cell refinement after the marking."""
+    import numpy as np
+    ro = np.ascontiguousarray(refine_old, dtype=np.int32).ravel()
+    fl = np.ascontiguousarray(flags, dtype=np.int32)
+    rn = np.zeros_like(ro)
+    ip = C.POINTER(C.c_int)
+    check(load().exw_adapt_box_refine(dim, n_intervals, ro.ctypes.data_as(ip),
+                                      fl.ctypes.data_as(ip), rn.ctypes.data_as(ip)))
+    return rn
+
+
+def adapt_box_transfer(ctx, dim, degree, n_intervals, refine_old, refine_new, old, new):
+    """exw_adapt_box_transfer between two exw_mesh_box_refined meshes (device vectors)."""
+    import numpy as np
+    ro = np.ascontiguousarray(refine_old, dtype=np.int32).ravel()
+    rn = np.ascontiguousarray(refine_new, dtype=np.int32).ravel()
+    ip = C.POINTER(C.c_int)
+    check(load().exw_adapt_box_transfer(ctx.h, dim, degree, n_intervals, ro.ctypes.data_as(ip),
+                                        rn.ctypes.data_as(ip), _ptr(old), _ptr(new)))
+
+
 def lts_categorize(dim, vertices, neighbor, cfl, max_clusters, max_diff):
     """exw_lts_categorize (host only): (cluster ids, flags, n_clusters,
     cluster_diff, iterations); flags 1 = slower neighbour, 2 = faster."""

@@ -21,7 +21,7 @@ OUT_DIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(OUT_DIR, "libexwave_amd.so")
 ARCH = os.environ.get("EXWAVE_ARCH", "gfx950")
 SOURCES = ["kernels_2d.hip", "kernels_3d.hip", "exwave_capi.hip", "exwave_driver.hip", "exwave_serving.hip",
-           "exwave_lts.hip", "exwave_output.hip"]
+           "exwave_lts.hip", "exwave_output.hip", "exwave_adapt.hip"]
 
 
 def hipcc():

@@ -112,6 +112,9 @@ SIGNATURES = [
     ("exw_energy", _I, [_VP, _VP, _DP]),
     ("exw_post_n_values", C.c_int64, [_VP]),
     ("exw_estimate_error", _I, [_VP, _VP, _VP, _VP]),
+    ("exw_refinement_flags", _I, [_I, _DP, _IP, _I, _I, _D, _IP]),
+    ("exw_adapt_box_refine", _I, [_I, _I, _IP, _IP, _IP]),
+    ("exw_adapt_box_transfer", _I, [_VP, _I, _I, _I, _IP, _IP, _VP, _VP]),
     ("exw_write_vtu", _I, [_I, _I, _I, _DP, _DP, _DP, _DP, _DP, _DP, _DP, C.c_char_p]),
     ("exw_op_create_nc", _I, [_VP, _I, _I, _I, _DP, _IP, _I, _IP, _DP, _DP, C.POINTER(_VP)]),
     ("exw_mesh_box_refined", _I, [_I, _I, _I, _D, _I, _IP, _IP, _DP, _IP, _IP, _IP]),

@@ -225,6 +225,27 @@ int exw_mesh_hypercube(int dim, int n_intervals, int n_refinements, int transfor
  * asynchronous. */
 int exw_estimate_error(exw_op op, const double *solution, double *tmp, double *error_per_cell);
 
+/* ---- one level of adaptive refinement (adapt_mesh, explicit_wave.cc:265-330) --
+ * Marking: GridRefinement::refine_and_coarsen_fixed_number (deal.II >= 8.5)
+ * with the fractions 0.1 / 0.6 of set_refinement_indicators (:255-260), then
+ * adapt_mesh's rules (:281-296): no refinement at max_level, no coarsening at
+ * min_level, no refinement below 0.1 max_error_init, coarsening below 0.05
+ * max_error_init.  flags[c] = 1 refine, -1 coarsen, 0 keep (host arrays). */
+int exw_refinement_flags(int n_cells, const double *error_per_cell, const int *level,
+                         int min_level, int max_level, double max_error_init, int *flags);
+/* The base-cell refinement of exw_mesh_box_refined after the marking: an
+ * unrefined cell flagged 1 is refined, a refined base cell is coarsened when all
+ * its children are flagged -1 (host arrays, refine_* of n_intervals^dim). */
+int exw_adapt_box_refine(int dim, int n_intervals, const int *refine_old, const int *flags,
+                         int *refine_new);
+/* SolutionTransfer::interpolate between the two meshes of exw_mesh_box_refined
+ * (device vectors): children take the parent's polynomial, a coarsened parent
+ * the L2 projection of its children on the reference cell, other cells are
+ * copied.  Synchronises. */
+int exw_adapt_box_transfer(exw_ctx ctx, int dim, int degree, int n_intervals,
+                           const int *refine_old, const int *refine_new,
+                           const double *old_solution, double *new_solution);
+
 /* output_results (source/explicit_wave.cc:339-436) as a VTU file, host arrays:
  * each cell subdivided degree times per direction (build_patches(mapping,
  * fe_degree)), the fields evaluated at the equispaced points and written as

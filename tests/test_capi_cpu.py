@@ -180,3 +180,57 @@ def test_write_vtu(tmp_path, dim, k):
     else:
         a, b, c3 = (pts[conn[:, i]] - pts[conn[:, 0]] for i in (1, 3, 4))
         assert np.all(np.einsum("ij,ij->i", np.cross(a, b), c3) > 0)
+
+
+def test_refinement_flags_fixed_number():
+    """refine_and_coarsen_fixed_number(0.1, 0.6) on distinct indicators: the
+    int(0.1 n) largest are refined, the int(0.6 n) smallest coarsened (cells of
+    an intermediate level, thresholds off)."""
+    import exwave_amd as X
+    rng = np.random.default_rng(0)
+    for n in (7, 10, 64, 333):
+        e = rng.permutation(n).astype(float) + 1.0
+        fl = X.refinement_flags(e, np.ones(n, np.int32), 0, 2, -1.0)
+        order = np.argsort(-e)
+        nr, ncs = int(0.1 * n), int(0.6 * n)
+        assert set(np.nonzero(fl == 1)[0]) == set(order[:nr])
+        assert set(np.nonzero(fl == -1)[0]) == set(order[n - ncs:])
+        assert (fl == 0).sum() == n - nr - ncs
+
+
+def test_refinement_flags_rules():
+    """adapt_mesh's rules: no refinement at the maximum level, no coarsening at
+    the minimum level, no refinement below 0.1 max_error_init, coarsening below
+    0.05 max_error_init (source/explicit_wave.cc:281-296); ties at a threshold
+    are flagged (>= / <=)."""
+    import exwave_amd as X
+    n = 20
+    e = np.arange(1.0, n + 1)
+    lv = np.zeros(n, np.int32)
+    lv[-1] = 1  # the largest indicator sits on the finest level
+    fl = X.refinement_flags(e, lv, 0, 1, -1.0)
+    assert fl[-1] == 0 and fl[-2] == 1  # int(0.1*20) = 2 largest: 20 (max level), 19
+    assert np.all(fl[:12] == 0)  # coarsening flags on the minimum level are cleared
+    fl = X.refinement_flags(e, np.ones(n, np.int32), 0, 2, 200.0)
+    assert np.all(fl[e < 10.0] == -1)  # below 0.05 * 200
+    assert fl[18] == 0 and fl[19] == 1  # 19 < 0.1 * 200 is not refined, 20 is
+    e2 = np.array([1.0, 1.0, 1.0, 5.0, 5.0, 2.0, 3.0, 4.0, 6.0, 6.0])
+    fl = X.refinement_flags(e2, np.ones(10, np.int32), 0, 2, -1.0)
+    assert fl[8] == 1 and fl[9] == 1  # tie at the refinement threshold: both
+
+
+def test_adapt_box_refine():
+    import exwave_amd as X
+    dim, n = 2, 3
+    ro = np.zeros((n, n), np.int32)
+    ro[0, 0] = ro[1, 1] = 1
+    V, nb, H = X.mesh_box_refined(dim, n, ro)
+    fl = np.zeros(len(V), np.int32)
+    # base 0 refined: all four children coarsen -> unrefined; base 4 (refined):
+    # three of four -> stays refined; base 2 unrefined flagged -> refined
+    fl[0:4] = -1
+    first4 = 4 + 1 + 1 + 1  # cells of bases 0 (4), 1, 2, 3 precede base 4
+    fl[first4:first4 + 3] = -1
+    fl[4 + 1] = 1  # base 2 is the third output cell block: bases 0 (4 cells), 1 -> index 5
+    rn = X.adapt_box_refine(dim, n, ro, fl)
+    assert rn[0] == 0 and rn[4] == 1 and rn[2] == 1 and rn.sum() == 2
