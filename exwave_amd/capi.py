@@ -77,7 +77,8 @@ class Problem(C.Structure):
                 ("output_every_time", C.c_double), ("ader_recon", C.c_int),
                 ("precision", C.c_int), ("modes", Modes),
                 ("lts_max_clusters", C.c_int), ("lts_max_diff", C.c_int),
-                ("post_pressure", C.c_int)]
+                ("post_pressure", C.c_int), ("n_adaptive_refinements", C.c_int),
+                ("adaptive_refinement_interval", C.c_int)]
 
 
 # (name, restype, argtypes) of every exported entry point

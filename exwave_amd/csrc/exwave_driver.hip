@@ -4,6 +4,7 @@
 // WaveEquationProblem::run (source/explicit_wave.cc:549-813).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -185,39 +186,98 @@ int exw_mesh_box_refined(int dim, int n_intervals, int transform, double factor,
   return EXW_OK;
 }
 
+namespace {
+
+// The mesh, operator and state vectors of a run; replaced by adapt_mesh.
+struct RunState {
+  exw_ctx ctx = nullptr;
+  exw_op op = nullptr;
+  double *buf = nullptr, *post = nullptr, *err = nullptr;
+  double *sol = nullptr, *tmp = nullptr, *work = nullptr;
+  int n_cells = 0;
+  void release() {
+    if (post) exw_free(ctx, post);
+    if (err) exw_free(ctx, err);
+    if (buf) exw_free(ctx, buf);
+    if (op) exw_op_destroy(op);
+    op = nullptr;
+    buf = post = err = nullptr;
+  }
+};
+
+// mesh (hypercube, or the box with refine[] when adaptive), operator and
+// vectors; the solution is left uninitialised
+int run_setup(exw_ctx ctx, const exw_problem *p, bool adaptive, const std::vector<int> &refine,
+              int nwork, RunState &st) {
+  st.ctx = ctx;
+  int n_cells = 0, n_hanging = 0, rc;
+  const int nv = 1 << p->dim;
+  std::vector<double> V;
+  std::vector<int> nb, H;
+  if (adaptive) {
+    if ((rc = exw_mesh_box_refined(p->dim, p->n_intervals, p->transform, p->transform_factor,
+                                   p->boundary_id, refine.data(), &n_cells, nullptr, nullptr,
+                                   &n_hanging, nullptr)))
+      return rc;
+    V.resize((size_t)n_cells * nv * p->dim);
+    nb.resize((size_t)n_cells * 2 * p->dim);
+    H.resize((size_t)std::max(n_hanging, 1) * (2 + nv / 2));
+    if ((rc = exw_mesh_box_refined(p->dim, p->n_intervals, p->transform, p->transform_factor,
+                                   p->boundary_id, refine.data(), &n_cells, V.data(), nb.data(),
+                                   &n_hanging, H.data())))
+      return rc;
+  } else {
+    if ((rc = exw_mesh_hypercube(p->dim, p->n_intervals, p->n_refinements, p->transform,
+                                 p->transform_factor, p->boundary_id, &n_cells, nullptr,
+                                 nullptr)))
+      return rc;
+    V.resize((size_t)n_cells * nv * p->dim);
+    nb.resize((size_t)n_cells * 2 * p->dim);
+    if ((rc = exw_mesh_hypercube(p->dim, p->n_intervals, p->n_refinements, p->transform,
+                                 p->transform_factor, p->boundary_id, &n_cells, V.data(),
+                                 nb.data())))
+      return rc;
+  }
+  std::vector<double> rho(n_cells, p->modes.rho), speed(n_cells, p->modes.c);
+  if ((rc = exw_op_create_nc(ctx, p->dim, p->degree, n_cells, V.data(), nb.data(), n_hanging,
+                             H.data(), rho.data(), speed.data(), &st.op)))
+    return rc;
+  st.n_cells = n_cells;
+  const int64_t N = exw_op_n_dofs(st.op);
+  if ((rc = exw_alloc(ctx, sizeof(double) * N * (2 + nwork), (void **)&st.buf))) return rc;
+  st.sol = st.buf;
+  st.tmp = st.buf + N;
+  st.work = st.buf + 2 * N;
+  if (p->post_pressure &&
+      (rc = exw_alloc(ctx, sizeof(double) * exw_post_n_values(st.op), (void **)&st.post)))
+    return rc;
+  if (adaptive && (rc = exw_alloc(ctx, sizeof(double) * n_cells, (void **)&st.err))) return rc;
+  return EXW_OK;
+}
+
+}  // namespace
+
 int exw_run(exw_ctx ctx, const exw_problem *p, char *log, int log_len, double *wall_seconds) {
   if (!ctx || !p) return EXW_EINVAL;
-  int n_cells = 0, rc;
-  if ((rc = exw_mesh_hypercube(p->dim, p->n_intervals, p->n_refinements, p->transform,
-                               p->transform_factor, p->boundary_id, &n_cells, nullptr, nullptr)))
-    return rc;
-  const int nv = 1 << p->dim;
-  std::vector<double> V((size_t)n_cells * nv * p->dim);
-  std::vector<int> nb((size_t)n_cells * 2 * p->dim);
-  if ((rc = exw_mesh_hypercube(p->dim, p->n_intervals, p->n_refinements, p->transform,
-                               p->transform_factor, p->boundary_id, &n_cells, V.data(), nb.data())))
-    return rc;
-  std::vector<double> rho(n_cells, p->modes.rho), speed(n_cells, p->modes.c);
-  exw_op op = nullptr;
-  if ((rc = exw_op_create(ctx, p->dim, p->degree, n_cells, V.data(), nb.data(), rho.data(),
-                          speed.data(), &op)))
-    return rc;
-  const int64_t N = exw_op_n_dofs(op);
+  const bool adaptive = p->n_adaptive_refinements > 0;
   const bool lts = p->integrator == EXW_ADER_LTS;
+  if (adaptive && (p->n_adaptive_refinements != 1 || p->n_refinements != 0 || lts ||
+                   p->adaptive_refinement_interval < 1))
+    return EXW_EUNSUP;  // one adaptive level on the coarse mesh, no LTS
   const int nwork = lts ? 0 : exw_time_step_work_vectors(p->integrator);
-  if (nwork < 0) {
-    exw_op_destroy(op);
-    return EXW_EINVAL;
-  }
-  double *buf = nullptr;
-  if ((rc = exw_alloc(ctx, sizeof(double) * N * (2 + nwork), (void **)&buf))) {
-    exw_op_destroy(op);
+  if (nwork < 0) return EXW_EINVAL;
+  int nbase = 1;
+  for (int d = 0; d < p->dim; ++d) nbase *= p->n_intervals;
+  std::vector<int> refine(adaptive ? nbase : 0, 0);
+  RunState st;
+  int rc;
+  if ((rc = run_setup(ctx, p, adaptive, refine, nwork, st))) {
+    st.release();
     return rc;
   }
-  double *sol = buf, *tmp = buf + N, *work = buf + 2 * N;
   // parameters.cc:195 and explicit_wave.cc:529-545
   const double cfl = p->cfl_number / std::pow((double)p->degree, 1.5);
-  const double dt_cfl = cfl * exw_op_min_vertex_distance_over_c(op);
+  const double dt_cfl = cfl * exw_op_min_vertex_distance_over_c(st.op);
   // TimeControl::setup (utilities.cc:31-46), then make_dofs resets the step to
   // the unrounded CFL value (explicit_wave.cc:222)
   const double T = p->final_time, tick = p->output_every_time;
@@ -232,32 +292,52 @@ int exw_run(exw_ctx ctx, const exw_problem *p, char *log, int log_len, double *w
     // WaveEquationOperationADERLTS::setup: clusters from the unrounded CFL step;
     // the time control then advances by the slowest cluster's step
     // (include/cluster_manager.templates.h:463)
-    if ((rc = exw_lts_create(op, cfl, dt_cfl, p->lts_max_clusters, p->lts_max_diff, 0.0,
+    if ((rc = exw_lts_create(st.op, cfl, dt_cfl, p->lts_max_clusters, p->lts_max_diff, 0.0,
                              p->ader_recon, &L))) {
-      exw_free(ctx, buf);
-      exw_op_destroy(op);
+      st.release();
       return rc;
     }
     exw_lts_info(L, nullptr, nullptr, nullptr, &dt);
   }
+  // adapt_mesh (explicit_wave.cc:265-330): indicator, marking, new mesh,
+  // solution transfer, new CFL step
+  double max_err_init = -1.0;
+  auto adapt = [&]() -> int {
+    int r;
+    if ((r = exw_estimate_error(st.op, st.sol, st.tmp, st.err))) return r;
+    std::vector<double> err(st.n_cells);
+    if ((r = exw_memcpy_d2h(ctx, err.data(), st.err, sizeof(double) * st.n_cells))) return r;
+    std::vector<int> lev, flags(st.n_cells), rnew(nbase);
+    for (int B = 0; B < nbase; ++B)
+      for (int ch = 0; ch < (refine[B] ? (1 << p->dim) : 1); ++ch) lev.push_back(refine[B]);
+    if ((r = exw_refinement_flags(st.n_cells, err.data(), lev.data(), 0, 1, max_err_init,
+                                  flags.data())) ||
+        (r = exw_adapt_box_refine(p->dim, p->n_intervals, refine.data(), flags.data(),
+                                  rnew.data())))
+      return r;
+    RunState nst;
+    if ((r = run_setup(ctx, p, true, rnew, nwork, nst)) ||
+        (r = exw_adapt_box_transfer(ctx, p->dim, p->degree, p->n_intervals, refine.data(),
+                                    rnew.data(), st.sol, nst.sol))) {
+      nst.release();
+      return r;
+    }
+    st.release();
+    st = nst;
+    refine = rnew;
+    dt = cfl * exw_op_min_vertex_distance_over_c(st.op);  // compute_time_step_size
+    return EXW_OK;
+  };
   int pos = 0;
   if (log && log_len > 0) log[0] = 0;
-  double *post = nullptr;
-  if (p->post_pressure && (rc = exw_alloc(ctx, sizeof(double) * exw_post_n_values(op),
-                                          (void **)&post))) {
-    exw_lts_destroy(L);
-    exw_free(ctx, buf);
-    exw_op_destroy(op);
-    return rc;
-  }
   auto output = [&]() -> int {
     double err[3], epp = -1.0;
-    int r = exw_l2_errors(op, &p->modes, time, sol, err);
+    int r = exw_l2_errors(st.op, &p->modes, time, st.sol, err);
     if (r) return r;
     // output_results: compute_post_pressure into tmp_solutions (explicit_wave.cc:476);
     // tmp is free between steps (the next step swaps it with sol first)
-    if (post && ((r = exw_post_pressure(op, sol, tmp, post, nullptr)) ||
-                 (r = exw_post_error(op, &p->modes, time, post, &epp))))
+    if (st.post && ((r = exw_post_pressure(st.op, st.sol, st.tmp, st.post, nullptr)) ||
+                    (r = exw_post_error(st.op, &p->modes, time, st.post, &epp))))
       return r;
     if (log && pos < log_len)
       pos += std::snprintf(log + pos, log_len - pos, "%u %.17g %.10e %.10e %.10e %.10e\n", step,
@@ -265,28 +345,42 @@ int exw_run(exw_ctx ctx, const exw_problem *p, char *log, int log_len, double *w
     return 0;
   };
   double wall = 0.0;
-  if (!(rc = exw_project(op, &p->modes, 0.0, sol)) && !(rc = output())) {
+  rc = exw_project(st.op, &p->modes, 0.0, st.sol);
+  if (!rc && adaptive) {
+    // the initial adaptation (explicit_wave.cc:704-716): adapt, project again,
+    // then the largest indicator of the projected field
+    if (!(rc = adapt()) && !(rc = exw_project(st.op, &p->modes, 0.0, st.sol)) &&
+        !(rc = exw_estimate_error(st.op, st.sol, st.tmp, st.err))) {
+      std::vector<double> err(st.n_cells);
+      if (!(rc = exw_memcpy_d2h(ctx, err.data(), st.err, sizeof(double) * st.n_cells))) {
+        max_err_init = 0.0;
+        for (double e : err) max_err_init = std::max(max_err_init, e);
+      }
+    }
+  }
+  if (!rc && !(rc = output())) {
     while (!(time >= T || step > max_step)) {
       ++step;
       time += dt;
-      std::swap(sol, tmp);
+      std::swap(st.sol, st.tmp);
       exw_sync(ctx);
       auto t0 = std::chrono::steady_clock::now();
       if (lts)
-        rc = exw_lts_step(L, time, tmp, sol);  // ArbitraryHighOrderDGLTS (time_integrators.cc:54-63)
+        rc = exw_lts_step(L, time, st.tmp, st.sol);  // ArbitraryHighOrderDGLTS (:54-63)
       else
-        rc = exw_time_step(op, p->integrator, dt, tmp, sol, work, p->ader_recon, p->precision);
+        rc = exw_time_step(st.op, p->integrator, dt, st.tmp, st.sol, st.work, p->ader_recon,
+                           p->precision);
       if (rc) break;
       if ((rc = exw_sync(ctx))) break;
       wall += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-      // TimeControl::at_tick (utilities.cc:89-95)
+      if (adaptive && step % p->adaptive_refinement_interval == 0 && (rc = adapt())) break;
+      // TimeControl::at_tick (utilities.cc:89-95), with the step size as any
+      // adaptation left it
       if ((int)(time / tick) != (int)((time - dt) / tick) && (rc = output())) break;
     }
   }
   exw_lts_destroy(L);
-  if (post) exw_free(ctx, post);
-  exw_free(ctx, buf);
-  exw_op_destroy(op);
+  st.release();
   if (wall_seconds) *wall_seconds = wall;
   return rc ? rc : (int)step;
 }

@@ -324,6 +324,10 @@ typedef struct exw_problem {
   exw_modes modes;     /* initial condition and exact solution */
   int lts_max_clusters, lts_max_diff; /* EXW_ADER_LTS: max_n_clusters, max_diff_clusters */
   int post_pressure;   /* also report 'error p post' at every tick (else -1) */
+  /* adapt_mesh every adaptive_refinement_interval steps and once before the
+   * loop (explicit_wave.cc:704-716, 789-791): 0, or 1 with n_refinements = 0
+   * (one level on the n_intervals^dim mesh; not with EXW_ADER_LTS) */
+  int n_adaptive_refinements, adaptive_refinement_interval;
 } exw_problem;
 
 /* WaveEquationProblem<dim>::run (source/explicit_wave.cc:549-813): mesh,

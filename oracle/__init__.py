@@ -71,7 +71,8 @@ class Problem(C.Structure):
                 ("max_time_steps", C.c_int), ("final_time", C.c_double),
                 ("output_every_time", C.c_double), ("ader_recon", C.c_int),
                 ("exact", Exact), ("lts_max_clusters", C.c_int), ("lts_max_diff", C.c_int),
-                ("post_pressure", C.c_int)]
+                ("post_pressure", C.c_int), ("n_adaptive_refinements", C.c_int),
+                ("adaptive_refinement_interval", C.c_int)]
 
 
 _lib = None
@@ -125,6 +126,7 @@ def lib():
         L.ox_lts_step.argtypes = [C.c_void_p, C.c_double, dp, dp]
         L.ox_gauss_lobatto.argtypes = [C.c_int, dp]
         L.ox_estimate_error.argtypes = [C.POINTER(_Mesh), C.c_int, dp, dp]
+        L.ox_adapt_transfer.argtypes = [C.c_int, C.c_int, C.c_int, ip, ip, dp, dp]
         L.ox_post_pressure.argtypes = [C.POINTER(_Mesh), C.c_int, dp, dp]
         L.ox_post_set_solver.argtypes = [C.c_double, C.c_double, C.c_int, C.c_void_p, C.c_int]
         L.ox_post_error.restype = C.c_double

@@ -1277,6 +1277,125 @@ void ox_estimate_error(const ox_mesh *m, int degree, const double *solution, dou
 }
 
 /* ------------------------------------------------------------------------ */
+/* adaptive refinement, one level (source/explicit_wave.cc:241-330)          */
+/* ------------------------------------------------------------------------ */
+
+static int cmp_desc(const void *a, const void *b) {
+  const double x = *(const double *)a, y = *(const double *)b;
+  return x < y ? 1 : (x > y ? -1 : 0);
+}
+
+void ox_refinement_flags(int n, const double *err, const int *level, int min_level, int max_level,
+                         double max_error_init, int *flags) {
+  /* refine_and_coarsen_fixed_number(tria, err, 0.1, 0.6): the int(0.1 n)-th
+   * largest value is the refinement threshold (cells >= it), the int(0.6 n)-th
+   * smallest the coarsening threshold (cells <= it, not flagged for
+   * refinement); a full sort gives the same order statistics */
+  double *s = (double *)malloc(sizeof(double) * n);
+  memcpy(s, err, sizeof(double) * n);
+  qsort(s, n, sizeof(double), cmp_desc);
+  const int nr = (int)(0.1 * n), ncs = (int)(0.6 * n);
+  for (int c = 0; c < n; ++c) flags[c] = 0;
+  if (nr)
+    for (int c = 0; c < n; ++c)
+      if (fabs(err[c]) >= s[nr - 1]) flags[c] = 1;
+  if (ncs)
+    for (int c = 0; c < n; ++c)
+      if (fabs(err[c]) <= s[n - ncs] && flags[c] != 1) flags[c] = -1;
+  free(s);
+  /* adapt_mesh (:281-296) */
+  for (int c = 0; c < n; ++c) {
+    if (flags[c] == 1 && level[c] == max_level)
+      flags[c] = 0;
+    else if (flags[c] == -1 && level[c] == min_level)
+      flags[c] = 0;
+    if (flags[c] == 1 && err[c] < 0.1 * max_error_init) flags[c] = 0;
+    if (err[c] < 0.05 * max_error_init) flags[c] = -1;
+  }
+}
+
+void ox_adapt_refine(int dim, int n_intervals, const int *refine_old, const int *flags,
+                     int *refine_new) {
+  const int nb = ipow(n_intervals, dim), nch = 1 << dim;
+  int c = 0;
+  for (int B = 0; B < nb; ++B) {
+    if (!refine_old[B]) {
+      refine_new[B] = flags[c++] == 1;
+    } else {
+      int all = 1;
+      for (int ch = 0; ch < nch; ++ch) all &= flags[c + ch] == -1;
+      refine_new[B] = !all;
+      c += nch;
+    }
+  }
+}
+
+void ox_adapt_transfer(int dim, int degree, int n_intervals, const int *refine_old,
+                       const int *refine_new, const double *old_u, double *new_u) {
+  ox_basis b;
+  ox_basis_init(&b, degree);
+  const int n = b.n, nd = ipow(n, dim), nf = dim + 1, nch = 1 << dim, nb = ipow(n_intervals, dim);
+  const int nq = n + 1, nqd = ipow(nq, dim);
+  double xq[OX_MAX_N], wq[OX_MAX_N];
+  ox_gauss(nq, xq, wq);
+  long co = 0, cn = 0;
+  for (int B = 0; B < nb; ++B) {
+    const int ro = refine_old[B] != 0, rn = refine_new[B] != 0;
+    if (ro == rn) {
+      const int m = ro ? nch : 1;
+      memcpy(new_u + cn * nf * nd, old_u + co * nf * nd, sizeof(double) * m * nf * nd);
+      cn += m;
+    } else if (rn) {
+      /* children: the parent's polynomial at the child's nodes */
+      for (int ch = 0; ch < nch; ++ch, ++cn)
+        for (int f = 0; f < nf; ++f)
+          for (int q = 0; q < nd; ++q) {
+            double xi[3];
+            int t = q;
+            for (int d = 0; d < dim; ++d) {
+              xi[d] = 0.5 * (b.x[t % n] + ((ch >> d) & 1));
+              t /= n;
+            }
+            new_u[(cn * nf + f) * nd + q] = eval_at(&b, dim, old_u + (co * nf + f) * nd, xi);
+          }
+    } else {
+      /* parent_i = (1/w_i) int_ref l_i u, u the children (QGauss(k+2) each) */
+      for (int f = 0; f < nf; ++f)
+        for (int i = 0; i < nd; ++i) {
+          double acc = 0.0, wi = 1.0, li[3][OX_MAX_N];
+          int ti = i;
+          int id[3];
+          for (int d = 0; d < dim; ++d) {
+            id[d] = ti % n;
+            ti /= n;
+            wi *= b.w[id[d]];
+          }
+          for (int ch = 0; ch < nch; ++ch)
+            for (int q = 0; q < nqd; ++q) {
+              double xc[3], xp[3], w = 1.0 / nch;
+              int t = q;
+              for (int d = 0; d < dim; ++d) {
+                xc[d] = xq[t % nq];
+                w *= wq[t % nq];
+                t /= nq;
+                xp[d] = 0.5 * (xc[d] + ((ch >> d) & 1));
+              }
+              double phi = 1.0;
+              for (int d = 0; d < dim; ++d) {
+                ox_lagrange(n, b.x, xp[d], li[d], NULL);
+                phi *= li[d][id[d]];
+              }
+              acc += w * phi * eval_at(&b, dim, old_u + ((co + ch) * nf + f) * nd, xc);
+            }
+          new_u[(cn * nf + f) * nd + i] = acc / wi;
+        }
+      ++cn;
+    }
+    co += ro ? nch : 1;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
 /* post-processed pressure (compute_post_pressure,                            */
 /* source/wave_equation_operations.cc:786-848)                                */
 /* ------------------------------------------------------------------------ */
@@ -2068,8 +2187,17 @@ void ox_lts_step(ox_lts *L, double time_now, const double *src, double *dst) {
 /* ------------------------------------------------------------------------ */
 
 int ox_run(const ox_problem *P, char *log, int log_len, double *final_energy) {
-  ox_mesh *m = ox_mesh_hypercube(P->dim, P->n_intervals, P->n_refinements, P->transform,
-                                 P->transform_factor, P->boundary_id);
+  const int adaptive = P->n_adaptive_refinements > 0;
+  if (adaptive && (P->n_adaptive_refinements != 1 || P->n_refinements != 0 ||
+                   P->integrator == OX_ADER_LTS))
+    return -1; /* one adaptive level on the coarse mesh only */
+  const int nbase = ipow(P->n_intervals, P->dim);
+  int *refine = (int *)calloc(nbase, sizeof(int));
+  ox_mesh *m = adaptive ? ox_mesh_box_refined(P->dim, P->n_intervals, P->transform,
+                                              P->transform_factor, P->boundary_id, refine)
+                        : ox_mesh_hypercube(P->dim, P->n_intervals, P->n_refinements,
+                                            P->transform, P->transform_factor,
+                                            P->boundary_id);
   const int k = P->degree;
   long N = ox_n_dofs(m, k);
   /* source/parameters.cc:195 and source/explicit_wave.cc:529-545 */
@@ -2101,6 +2229,50 @@ int ox_run(const ox_problem *P, char *log, int log_len, double *final_energy) {
                      ? (double *)malloc(sizeof(double) * m->n_cells * ipow(k + 2, P->dim))
                      : NULL;
   ox_project(m, k, &P->exact, 0.0, sol);
+  /* adapt_mesh (source/explicit_wave.cc:265-330) on the box mesh: marking,
+   * new mesh, solution transfer, new CFL step */
+  double max_err_init = -1.0;
+  int *rnew = (int *)calloc(nbase, sizeof(int));
+#define OX_ADAPT()                                                                         \
+  do {                                                                                     \
+    double *err = (double *)malloc(sizeof(double) * m->n_cells);                           \
+    int *lev = (int *)malloc(sizeof(int) * m->n_cells), *fl = (int *)malloc(sizeof(int) * m->n_cells); \
+    ox_estimate_error(m, k, sol, err);                                                     \
+    for (int B = 0, c = 0; B < nbase; ++B)                                                 \
+      for (int ch = 0; ch < (refine[B] ? (1 << P->dim) : 1); ++ch) lev[c++] = refine[B];    \
+    ox_refinement_flags(m->n_cells, err, lev, 0, 1, max_err_init, fl);                     \
+    ox_adapt_refine(P->dim, P->n_intervals, refine, fl, rnew);                             \
+    ox_mesh *m2 = ox_mesh_box_refined(P->dim, P->n_intervals, P->transform,                \
+                                      P->transform_factor, P->boundary_id, rnew);          \
+    const long N2 = ox_n_dofs(m2, k);                                                      \
+    double *s2 = (double *)malloc(sizeof(double) * N2);                                    \
+    ox_adapt_transfer(P->dim, k, P->n_intervals, refine, rnew, sol, s2);                   \
+    free(sol);                                                                             \
+    sol = s2;                                                                              \
+    tmp = (double *)realloc(tmp, sizeof(double) * N2);                                     \
+    t1 = (double *)realloc(t1, sizeof(double) * N2);                                       \
+    t2 = (double *)realloc(t2, sizeof(double) * N2);                                       \
+    if (post) post = (double *)realloc(post, sizeof(double) * m2->n_cells * ipow(k + 2, P->dim)); \
+    ox_mesh_free(m);                                                                       \
+    m = m2;                                                                                \
+    N = N2;                                                                                \
+    memcpy(refine, rnew, sizeof(int) * nbase);                                             \
+    /* compute_time_step_size (explicit_wave.cc:326-327, 529-545) */                       \
+    dt = cfl * m->min_vertex_distance_over_c;                                              \
+    free(err);                                                                             \
+    free(lev);                                                                             \
+    free(fl);                                                                              \
+  } while (0)
+  if (adaptive) {
+    /* the initial adaptation (explicit_wave.cc:704-716) */
+    OX_ADAPT();
+    ox_project(m, k, &P->exact, 0.0, sol);
+    double *err = (double *)malloc(sizeof(double) * m->n_cells);
+    ox_estimate_error(m, k, sol, err);
+    max_err_init = 0.0;
+    for (int c = 0; c < m->n_cells; ++c) max_err_init = fmax(max_err_init, err[c]);
+    free(err);
+  }
   int pos = 0;
   if (log && log_len > 0) log[0] = 0;
 #define OX_LOG()                                                                           \
@@ -2127,10 +2299,15 @@ int ox_run(const ox_problem *P, char *log, int log_len, double *final_energy) {
       ox_lts_step(lts, time, tmp, sol); /* ArbitraryHighOrderDGLTS, time_integrators.cc:54-63 */
     else
       ox_time_step(m, k, P->integrator, dt, tmp, sol, t1, t2, P->ader_recon);
-    /* TimeControl::at_tick (utilities.cc:89-95) */
+    if (adaptive && step % P->adaptive_refinement_interval == 0) OX_ADAPT();
+    /* TimeControl::at_tick (utilities.cc:89-95), with the step size as the
+     * adaptation left it */
     if ((int)(time / tick) != (int)((time - dt) / tick)) OX_LOG();
   }
 #undef OX_LOG
+#undef OX_ADAPT
+  free(refine);
+  free(rnew);
   if (final_energy) *final_energy = ox_energy(m, k, sol);
   ox_lts_free(lts);
   free(post);

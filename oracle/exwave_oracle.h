@@ -162,6 +162,8 @@ typedef struct ox_problem {
   ox_exact exact;
   int lts_max_clusters, lts_max_diff; /* max_n_clusters, max_diff_clusters */
   int post_pressure;                   /* also log 'error p post' (else -1) */
+  int n_adaptive_refinements;          /* 0 or 1 (one level on the n_intervals^dim mesh) */
+  int adaptive_refinement_interval;
 } ox_problem;
 int ox_run(const ox_problem *p, char *log, int log_len, double *final_state_energy);
 
@@ -170,6 +172,19 @@ long ox_n_dofs(const ox_mesh *m, int degree);
 /* estimate_error (source/wave_equation_operations.cc:871-898): err[c] =
  * sqrt(int_K |grad p_h - v_t|^2) with v_t the velocity of apply(solution). */
 void ox_estimate_error(const ox_mesh *m, int degree, const double *solution, double *err);
+
+/* One level of adaptive refinement (adapt_mesh, source/explicit_wave.cc:265-330):
+ * the marking (GridRefinement::refine_and_coarsen_fixed_number of deal.II >= 8.5
+ * with 0.1 / 0.6, then adapt_mesh's rules; flags 1 refine, -1 coarsen), the new
+ * base-cell refinement of ox_mesh_box_refined, and the solution transfer
+ * (children: the parent's polynomial; parent: the reference-cell L2 projection
+ * of its children, by quadrature). */
+void ox_refinement_flags(int n, const double *err, const int *level, int min_level, int max_level,
+                         double max_error_init, int *flags);
+void ox_adapt_refine(int dim, int n_intervals, const int *refine_old, const int *flags,
+                     int *refine_new);
+void ox_adapt_transfer(int dim, int degree, int n_intervals, const int *refine_old,
+                       const int *refine_new, const double *old_u, double *new_u);
 
 /* Post-processed pressure (compute_post_pressure,
  * source/wave_equation_operations.cc:786-848): per cell the local Neumann
