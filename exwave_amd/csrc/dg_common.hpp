@@ -199,17 +199,15 @@ __device__ __forceinline__ void flux_lambda(T rho_p, T c_p, T rho_m, T c_m, int 
   }  // bid 2: lambda = 0
 }
 
-// Boundary code of a non-conforming (2:1 hanging) face: the operator sweeps
-// give it no contribution; nc_face_kernel adds its subface integrals.
+// Boundary code of a non-conforming (2:1 hanging) face (neighbour entry
+// -NC_BID): the operator sweeps scale its contribution by zero (no branch in
+// the flux, whose code the hot kernel is sensitive to); nc_face_kernel adds
+// its subface integrals.
 constexpr int NC_BID = 1000;
 
 template <typename T>
 __device__ __forceinline__ void flux(T nvp, T pp, T nv_m, T pm, T rho_p, T c_p, T rho_m, T c_m,
                                      int bid, T &val_pn, T &val_p) {
-  if (bid == NC_BID) {
-    val_pn = val_p = T(0);
-    return;
-  }
   T l[4];
   flux_lambda<T>(rho_p, c_p, rho_m, c_m, bid, l);
   const T tau_p = recip(c_p * rho_p);
@@ -222,11 +220,6 @@ __device__ __forceinline__ void flux(T nvp, T pp, T nv_m, T pm, T rho_p, T c_p, 
 // p) -> (v test value, p test value): a[j], b[j] multiply input j.
 template <typename T>
 __device__ __forceinline__ void flux_matrix(T rho_p, T c_p, T rho_m, T c_m, int bid, T *a, T *b) {
-  if (bid == NC_BID) {
-#pragma unroll
-    for (int j = 0; j < 4; ++j) a[j] = b[j] = T(0);
-    return;
-  }
   T l[4];
   flux_lambda<T>(rho_p, c_p, rho_m, c_m, bid, l);
   const T irho = recip(rho_p), tau_p = recip(c_p * rho_p), z = c_p * c_p * rho_p;

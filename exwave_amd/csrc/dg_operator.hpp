@@ -234,10 +234,11 @@ __global__ void __launch_bounds__((Shape<DIM, K>::BLOCK))
           pm += Lo[i] * s_nb[lc][f][1][base + i * sd];
         }
         const Number sg = s ? Number(1) : Number(-1);
+        const Number fsc = bid == NC_BID ? Number(0) : Number(1);
         Number val_pn, val_p;
         flux<Number>(sg * vd, pp, sg * vdm, pm, rho, cc, rho_m, c_m, bid, val_pn, val_p);
-        s_F[lc][f][0][t] = sg * val_pn;
-        s_F[lc][f][1][t] = val_p;
+        s_F[lc][f][0][t] = fsc * sg * val_pn;
+        s_F[lc][f][1][t] = fsc * val_p;
       } else {
         Number up[NF], um[NF];
 #pragma unroll
@@ -281,8 +282,9 @@ __global__ void __launch_bounds__((Shape<DIM, K>::BLOCK))
         const Number sg = s ? Number(1) : Number(-1);
 #pragma unroll
         for (int i = 0; i < DIM; ++i) nrm[i] = sg * nrm[i] / na;
-        // the tangential weights cancel against the node weights later
-        const Number dS = fabs(det) * na;
+        // the tangential weights cancel against the node weights later; a
+        // hanging face contributes nothing here (nc_face_kernel adds it)
+        const Number dS = bid == NC_BID ? Number(0) : fabs(det) * na;
         Number nvp = Number(0), nvm = Number(0);
 #pragma unroll
         for (int i = 0; i < DIM; ++i) {

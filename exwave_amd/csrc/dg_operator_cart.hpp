@@ -146,14 +146,16 @@ __global__ void __launch_bounds__((CartShape<DIM, K>::BLOCK))
       const double rho_m = nb >= 0 ? mesh.rho[nb] : rho, c_m = nb >= 0 ? mesh.speed[nb] : cc;
       const double ih = recip(mesh.h[(size_t)cell[it] * DIM + d]);
       const double sg = (f & 1) ? 1.0 : -1.0;
+      // a hanging face contributes nothing here (nc_face_kernel adds it)
+      const double ihf = nb == -NC_BID ? 0.0 : ih;
       double a[4], b[4];
       flux_matrix<double>(rho, cc, rho_m, c_m, nb >= 0 ? 0 : -nb, a, b);
       // inputs (v_d, p, neighbour v_d, neighbour p): v.n = sg v_d on both sides
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const double in = (j == 0 || j == 2) ? sg : 1.0;
-        s_fc[lc][f][j] = Number(a[j] * in * sg * ih);
-        s_fc[lc][f][4 + j] = Number(b[j] * in * ih);
+        s_fc[lc][f][j] = Number(a[j] * in * sg * ihf);
+        s_fc[lc][f][4 + j] = Number(b[j] * in * ihf);
       }
       if ((f & 1) == 0) {  // the cell term's factors of direction d
         s_vc[lc][d][0] = Number(-ih * recip(rho));
