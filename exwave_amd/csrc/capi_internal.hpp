@@ -25,6 +25,7 @@ struct exw_op_s {
   double minvd;
   int *d_neighbor = nullptr;
   double *d_rho = nullptr, *d_speed = nullptr, *d_h = nullptr, *d_vertices = nullptr;
+  double *d_coef = nullptr;  // Cartesian meshes: DevMesh::coef
   double *d_tab = nullptr, *d_errtab = nullptr, *d_cktab = nullptr, *d_posttab = nullptr;
   double *d_partial = nullptr, *d_sum = nullptr;
   // non-conforming faces: the cells that touch them (NcArgs) and the table
@@ -46,6 +47,7 @@ struct exw_op_s {
     m.speed = d_speed;
     m.h = d_h;
     m.vertices = d_vertices;
+    m.coef = d_coef;
     return m;
   }
   int64_t n_post() const {  // post-processed pressure: (k+2)^dim values per cell

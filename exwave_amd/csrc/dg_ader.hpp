@@ -62,15 +62,7 @@ __device__ __forceinline__ constexpr int ipow_c() {
 
 namespace ck {
 
-// calls f(integral_constant<int, I>) for I = 0 .. sizeof...(I)-1, at compile time
-template <typename F, int... I>
-__device__ __forceinline__ void static_for_impl(F &&f, std::integer_sequence<int, I...>) {
-  (f(std::integral_constant<int, I>{}), ...);
-}
-template <int COUNT, typename F>
-__device__ __forceinline__ void static_for(F &&f) {
-  static_for_impl(f, std::make_integer_sequence<int, COUNT>{});
-}
+using exw::static_for;
 
 // Physical derivative at node qq of a level field of NN points per direction
 // (in[f][...] in LDS): outv_d = rho^-1 d_d p, outp = rho c^2 div v (the

@@ -10,6 +10,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <utility>
+
 namespace exw {
 
 constexpr int MAX_N = 14;  // points per direction (degree <= 12, +1 for the error rule)
@@ -46,6 +48,9 @@ struct DevMesh {
   const double *__restrict__ speed;      // [n_cells]
   const double *__restrict__ h;          // Cartesian: [n_cells][dim] cell extents
   const double *__restrict__ vertices;   // general: [n_cells][2^dim][dim]
+  // Cartesian: per cell the face flux maps [2*dim][8] and the cell-term
+  // factors [dim][2] (cart_coef_kernel); NULL where not set up
+  const double *__restrict__ coef = nullptr;
 };
 
 // o1 = b + a1*s*k, o2 = (o1 or c) + a2*s*k, o3 = (o1 or c) + a3*s*k (if o3 is
@@ -84,6 +89,16 @@ struct CellSel {
 // range of logical blocks (a bijection for any grid size), so that cells that
 // are neighbours in the numbering share one L2.
 constexpr int N_XCD = 8;
+
+// calls f(integral_constant<int, I>) for I = 0 .. COUNT-1, at compile time
+template <typename F, int... I>
+__device__ __forceinline__ void static_for_impl(F &&f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+template <int COUNT, typename F>
+__device__ __forceinline__ void static_for(F &&f) {
+  static_for_impl(f, std::make_integer_sequence<int, COUNT>{});
+}
 __device__ __forceinline__ int xcd_remap(int b, int nblk) {
   const int q = nblk / N_XCD, r = nblk % N_XCD;
   const int x = b % N_XCD, i = b / N_XCD;

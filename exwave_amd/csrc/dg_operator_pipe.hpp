@@ -1,0 +1,362 @@
+// dg_operator_pipe.hpp -- the Cartesian operator + RK epilogue as a persistent,
+// software-pipelined FP64 kernel for cells of >= 48 nodes (3D k >= 3, 2D k >= 6).
+//
+// The arithmetic is dg_operator_cart_kernel's, expression for expression (its
+// face-lane flux maps are precomputed per cell by cart_coef_kernel from the
+// same code, its line phase and epilogue are restated below), so both kernels
+// give bit-identical results; what differs is how the inputs reach the LDS.  Each
+// workgroup computes a few consecutive cells one after the other.  While it
+// computes cell t, the inputs of cell t+1 -- its nodes, the RK accumulator and
+// the 2*dim neighbours' (v_n, p) fields, its flux coefficients -- already
+// stream into the second of two
+// LDS buffers through gfx950's LDS-DMA loads (global_load_lds_dword: lane l's
+// dword lands at M0 + 4 l), which occupy no registers.  The memory pipe thus
+// always holds one cell ahead per group instead of idling while a cell
+// computes.
+//
+// The DMA is issued from inline assembly and waited for with explicit vmcnt
+// counts: the compiler treats every LDS-DMA as one alias class and would wait
+// for the prefetch together with the current cell.  Vector memory operations
+// of one wave complete in issue order, so "at most n outstanding" means all but
+// the last n issued are done.  For the count to be exact the loop issues no
+// vector loads the compiler knows of: neighbour rows, materials and extents
+// are wave-uniform scalar loads through the constant address space, and every
+// DMA instruction runs with all 64 lanes (a block of W >= 64 dwords is covered
+// by instructions starting at 0, 64, ..., and one starting at W - 64).
+#pragma once
+#include "dg_operator_cart.hpp"
+
+namespace exw {
+namespace pipe {
+
+__device__ __forceinline__ uint32_t lds_addr(const void *p) {
+  return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void *)p;
+}
+
+// 64 lanes x 4 bytes: global base + voff -> LDS m0 + 4 * lane.  M0 is a
+// reserved register that the compiler sets right before each instruction of
+// its own that reads it, so it holds nothing of the compiler's across this.
+__device__ __forceinline__ void dma_dword(uint32_t m0, const void *base, uint32_t voff) {
+  asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dword %1, %2" ::"s"(m0), "v"(voff),
+               "s"(base)
+               : "memory");
+}
+
+// 64 lanes x 16 bytes: global base + voff -> LDS m0 + 16 * lane (gfx950)
+__device__ __forceinline__ void dma_dwordx4(uint32_t m0, const void *base, uint32_t voff) {
+  asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, %2" ::"s"(m0), "v"(voff),
+               "s"(base)
+               : "memory");
+}
+
+// wait until at most N vector memory operations of this wave are outstanding
+template <int N>
+__device__ __forceinline__ void wait_vm() {
+  static_assert(N >= 0 && N <= 63, "vmcnt is a 6-bit counter");
+  asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory");
+}
+
+// workgroup barrier ordering LDS only (no wait on global stores or DMA)
+__device__ __forceinline__ void barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// wave-uniform load through the constant address space (a scalar load); only
+// for data the kernel does not write
+template <typename T>
+__device__ __forceinline__ T sld(const T *p, size_t i) {
+  return ((const __attribute__((address_space(4))) T *)p)[i];
+}
+
+}  // namespace pipe
+
+// Per-cell flux coefficients of a Cartesian mesh, one thread per (cell, face):
+// exactly the values dg_operator_cart_kernel's face lanes compute (the same
+// expressions; evaluate_inner_face / evaluate_boundary_face,
+// source/wave_equation_operations.cc:480-629).  coef[cell]: the face flux maps
+// [2*dim][8] (the numerical flux as a linear map of own v_n, own p, neighbour
+// v_n, neighbour p -> v and p lifting, with 1/h and the normal's sign folded
+// in; zero on hanging faces, which nc_face_kernel handles), then the cell-term
+// factors [dim][2] = (-1/(rho h_d), rho c^2/h_d).
+template <int DIM>
+__global__ void cart_coef_kernel(const DevMesh mesh, double *__restrict__ coef) {
+  constexpr int NFACE = 2 * DIM, NCOEF = NFACE * 8 + DIM * 2;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)mesh.n_cells * NFACE) return;
+  const int cell = int(i / NFACE), f = int(i % NFACE), d = f >> 1;
+  const int nb = mesh.neighbor[(size_t)cell * NFACE + f];
+  const double rho = mesh.rho[cell], cc = mesh.speed[cell];
+  const double rho_m = nb >= 0 ? mesh.rho[nb] : rho, c_m = nb >= 0 ? mesh.speed[nb] : cc;
+  const double ih = recip(mesh.h[(size_t)cell * DIM + d]);
+  const double sg = (f & 1) ? 1.0 : -1.0;
+  const double ihf = nb == -NC_BID ? 0.0 : ih;
+  double a[4], b[4];
+  flux_matrix<double>(rho, cc, rho_m, c_m, nb >= 0 ? 0 : -nb, a, b);
+  double *o = coef + (size_t)cell * NCOEF;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const double in = (j == 0 || j == 2) ? sg : 1.0;
+    o[f * 8 + j] = a[j] * in * sg * ihf;
+    o[f * 8 + 4 + j] = b[j] * in * ihf;
+  }
+  if ((f & 1) == 0) {
+    o[NFACE * 8 + 2 * d] = -ih * recip(rho);
+    o[NFACE * 8 + 2 * d + 1] = rho * cc * cc * ih;
+  }
+}
+
+template <int DIM, int K>
+struct PipeShape {
+  using C = CartShape<DIM, K>;
+  static constexpr int ND = C::ND, NF = C::NF, NFACE = 2 * DIM, NCOEF = NFACE * 8 + DIM * 2;
+  static constexpr int WV = (ND + 63) / 64;  // waves per cell
+  static constexpr int BLOCK = 64 * WV;
+  // the DMA blocks and the instructions that cover them, in units of one
+  // lane's load: the cell's fields (and the accumulator's) in 16-byte units
+  // where a cell's block is a multiple of 16 bytes (the host then requires
+  // 16-byte aligned vectors), else in dwords; one neighbour field and the
+  // coefficients in dwords
+  static constexpr bool X4 = (NF * ND * 8) % 16 == 0;
+  static constexpr int U_OWN = X4 ? 16 : 4;
+  static constexpr int W_OWN = NF * ND * 8 / U_OWN, DW_FLD = 2 * ND, DW_C = 2 * NCOEF;
+  static constexpr int NQ_OWN = (W_OWN + 63) / 64, NQ_FLD = (DW_FLD + 63) / 64,
+                       NQ_C = (DW_C + 63) / 64;
+  // instruction index ranges: own [0, I_B), accumulator [I_B, I_NB), neighbour
+  // fields (face f, component c; order 2f+c) [I_NB, I_C), coefficients [I_C, TOTAL)
+  static constexpr int I_B = NQ_OWN, I_NB = 2 * NQ_OWN, I_C = I_NB + 2 * NFACE * NQ_FLD,
+                       TOTAL = I_C + NQ_C;
+  // doubles of one LDS buffer: own fields, accumulator, neighbour fields, coefficients
+  static constexpr int OFF_B = NF * ND, OFF_NB = 2 * NF * ND, OFF_C = OFF_NB + 2 * NFACE * ND,
+                       BUF = (OFF_C + NCOEF + 1) / 2 * 2;  // 16-byte multiple
+  // wave w issues the instructions i with i % WV == w
+  static constexpr int count(int w, bool has_b) {
+    int n = 0;
+    for (int i = w; i < TOTAL; i += WV)
+      if (has_b || i < I_B || i >= I_NB) ++n;
+    return n;
+  }
+  static constexpr int min_count(bool has_b) {
+    int m = count(0, has_b);
+    for (int w = 1; w < WV; ++w) m = count(w, has_b) < m ? count(w, has_b) : m;
+    return m;
+  }
+  static constexpr int max_count(bool has_b) {
+    int m = count(0, has_b);
+    for (int w = 1; w < WV; ++w) m = count(w, has_b) > m ? count(w, has_b) : m;
+    return m;
+  }
+  static constexpr int LDS_BYTES = 2 * BUF * 8;
+  // cells that fill whole waves, and two buffers within a 64 KB workgroup allocation
+  static constexpr bool ENABLED = ND >= 48 && LDS_BYTES <= 65536;
+  static_assert(!ENABLED || (W_OWN >= 64 && DW_FLD >= 64 && DW_C >= 64),
+                "every DMA instruction covers 64 dwords of one block");
+  static_assert(!ENABLED || max_count(true) + NF <= 63, "outstanding operations exceed vmcnt");
+};
+
+// The line phase of dg_operator_cart_kernel (phase B) on one LDS buffer, for
+// the line of direction d = line / NT: u the cell's fields [NF][ND]; nb the
+// neighbours' (v_n, p) per face [2*DIM][2][ND], whose [2d][0] and [2d][1]
+// receive the line's dv_d/dt and dp/dt share at its own nodes; fc the face
+// flux maps [2*DIM][8]; vc the cell-term factors [DIM][2].
+template <int DIM, int K>
+__device__ __forceinline__ void pipe_line(int line, const double *u, double *nb,
+                                          const double *fc, const double *vc) {
+  using S = CartShape<DIM, K>;
+  constexpr int N = S::N, ND = S::ND, NT = S::NT;
+  constexpr const CartConst<N> &T = CartTable<N>::value;
+  const int d = line / NT, t = line - d * NT;
+  int base, sd;
+  if constexpr (DIM == 2) {
+    base = d == 0 ? t * N : t;
+    sd = d == 0 ? 1 : N;
+  } else {
+    const int t0 = t % N, t1 = t / N;
+    base = d == 0 ? t0 * N + t1 * N * N : (d == 1 ? t0 + t1 * N * N : t0 + t1 * N);
+    sd = d == 0 ? 1 : (d == 1 ? N : N * N);
+  }
+  const double cvv = vc[2 * d], cvp = vc[2 * d + 1];
+  double vl[N], pl[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const int n = base + i * sd;
+    vl[i] = u[d * ND + n];
+    pl[i] = u[DIM * ND + n];
+  }
+  double kv[N], kp[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    double gp = 0.0, dv = 0.0;
+#pragma unroll
+    for (int m = 0; m < N; ++m) {
+      gp += T.D[i * N + m] * pl[m];
+      dv += T.DT[i * N + m] * vl[m];
+    }
+    kv[i] = cvv * gp;
+    kp[i] = cvp * dv;
+  }
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    const int f = 2 * d + s;
+    double vo = 0.0, po = 0.0, vn = 0.0, pn = 0.0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const int n = base + i * sd;
+      const double ls = T.L[s * N + i], lo = T.L[(1 - s) * N + i];
+      vo += ls * vl[i];
+      po += ls * pl[i];
+      vn += lo * nb[(2 * f) * ND + n];
+      pn += lo * nb[(2 * f + 1) * ND + n];
+    }
+    const double *c = fc + 8 * f;
+    const double fv = c[0] * vo + c[1] * po + c[2] * vn + c[3] * pn;
+    const double fp = c[4] * vo + c[5] * po + c[6] * vn + c[7] * pn;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const double lw = T.LW[s * N + i];
+      kv[i] += lw * fv;
+      kp[i] += lw * fp;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const int n = base + i * sd;
+    nb[(4 * d) * ND + n] = kv[i];
+    nb[(4 * d + 1) * ND + n] = kp[i];
+  }
+}
+
+// TPG consecutive cells per group, one after the other, the loop unrolled at
+// compile time (a run-time loop would have its 1D-matrix constants hoisted
+// out of it and spilled).  Needs mesh.coef.  Epilogue: b optional, o2 only
+// with o2_from_o1; no c, cell list or cell scale (the host uses
+// dg_operator_cart_kernel for those).  The vectors must be below 4 GiB
+// (n_cells * NF * ND * 8 < 2^32), and 16-byte aligned where PipeShape::X4.
+//
+// At least 2 waves per SIMD: with the full 512-register budget of a single
+// wave the backend assumes inline assembly may use the accumulation registers
+// and reserves as many as the kernel has VGPRs, which halves the occupancy.
+template <int DIM, int K, int TPG>
+__global__ void __launch_bounds__((PipeShape<DIM, K>::BLOCK)) __attribute__((amdgpu_waves_per_eu(2)))
+    dg_operator_pipe_kernel(const DevMesh mesh, const double *__restrict__ src,
+                            const Epilogue epi) {
+  using P = PipeShape<DIM, K>;
+  constexpr int ND = P::ND, NF = P::NF, NFACE = P::NFACE, WV = P::WV;
+  constexpr int NLINE = CartShape<DIM, K>::NLINE;
+  __shared__ __align__(16) double s_buf[2][P::BUF];
+
+  const int tid = threadIdx.x;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const uint32_t lane4 = uint32_t(tid & 63) * 4u, lane16 = lane4 * 4u;
+  const int c0 = xcd_remap(blockIdx.x, gridDim.x) * TPG;
+  const int nt = min(TPG, mesh.n_cells - c0);  // >= 1: the grid is ceil(n_cells / TPG)
+  const bool has_b = epi.b != nullptr;
+
+  // the neighbour rows of the group's cells (cells past the end repeat c0)
+  int rows[TPG][NFACE];
+#pragma unroll
+  for (int t = 0; t < TPG; ++t)
+#pragma unroll
+    for (int f = 0; f < NFACE; ++f)
+      rows[t][f] = pipe::sld(mesh.neighbor, (size_t)(t < nt ? c0 + t : c0) * NFACE + f);
+
+  // this wave's share of cell c's DMA into buffer buf; global addresses are
+  // the array's base (scalar) + a 32-bit byte offset (vectors below 4 GiB)
+  auto issue = [&](int c, const int (&nb)[NFACE], int buf) __attribute__((always_inline)) {
+    const uint32_t lbase = pipe::lds_addr(&s_buf[buf][0]);
+    const uint32_t own = uint32_t(c) * uint32_t(NF * ND * 8);
+    static_for<WV>([&](auto W_) __attribute__((always_inline)) {
+      constexpr int W = decltype(W_)::value;
+      if (wave != W) return;
+      static_for<(P::TOTAL - W + WV - 1) / WV>([&](auto J_) __attribute__((always_inline)) {
+        constexpr int i = W + decltype(J_)::value * WV;
+        if constexpr (i < P::I_NB) {
+          // the cell's fields, then the accumulator's
+          constexpr bool acc = i >= P::I_B;
+          constexpr int q = acc ? i - P::I_B : i;
+          constexpr int u0 = 64 * q < P::W_OWN - 64 ? 64 * q : P::W_OWN - 64;
+          constexpr uint32_t lo = (acc ? 8 * P::OFF_B : 0) + P::U_OWN * u0;
+          const double *base = acc ? epi.b : src;
+          if (!acc || has_b) {
+            if constexpr (P::X4)
+              pipe::dma_dwordx4(lbase + lo, base, lane16 + (own + P::U_OWN * u0));
+            else
+              pipe::dma_dword(lbase + lo, base, lane4 + (own + P::U_OWN * u0));
+          }
+        } else if constexpr (i < P::I_C) {
+          constexpr int j = i - P::I_NB, fld = j / P::NQ_FLD, q = j % P::NQ_FLD;
+          constexpr int f = fld >> 1, comp = fld & 1, field = comp ? DIM : (f >> 1);
+          constexpr int s0 = 64 * q < P::DW_FLD - 64 ? 64 * q : P::DW_FLD - 64;
+          // boundary and hanging faces read the cell itself (zero flux coefficients)
+          const int cn = nb[f] >= 0 ? nb[f] : c;
+          const uint32_t g = (uint32_t(cn) * NF + field) * uint32_t(ND * 8);
+          pipe::dma_dword(lbase + 8 * (P::OFF_NB + fld * ND) + 4 * s0, src, lane4 + (g + 4 * s0));
+        } else {
+          constexpr int q = i - P::I_C;
+          constexpr int s0 = 64 * q < P::DW_C - 64 ? 64 * q : P::DW_C - 64;
+          const uint32_t g = uint32_t(c) * uint32_t(P::NCOEF * 8);
+          pipe::dma_dword(lbase + 8 * P::OFF_C + 4 * s0, mesh.coef, lane4 + (g + 4 * s0));
+        }
+      });
+    });
+  };
+
+  issue(c0, rows[0], 0);
+  static_for<TPG>([&](auto T_) __attribute__((always_inline)) {
+    constexpr int t = decltype(T_)::value;
+    if (t >= nt) return;  // uniform over the group
+    const int c = c0 + t;
+    // prefetch cell t+1, then wait for cell t.  Outstanding, oldest first:
+    // DMA(t), stores(t-1) (>= NF: o1 of every field), DMA(t+1).
+    if (t + 1 < nt) {
+      if (has_b)
+        pipe::wait_vm<63 - P::max_count(true)>();
+      else
+        pipe::wait_vm<63 - P::max_count(false)>();
+      issue(c + 1, rows[t + 1 < TPG ? t + 1 : t], (t + 1) & 1);
+      constexpr int after = t == 0 ? 0 : NF;
+      if (has_b)
+        pipe::wait_vm<P::min_count(true) + after>();
+      else
+        pipe::wait_vm<P::min_count(false) + after>();
+    } else {
+      pipe::wait_vm<t == 0 ? 0 : NF>();
+    }
+    pipe::barrier();
+
+    double *buf = s_buf[t & 1];
+    if (tid < NLINE)
+      pipe_line<DIM, K>(tid, buf, buf + P::OFF_NB, buf + P::OFF_C, buf + P::OFF_C + NFACE * 8);
+    pipe::barrier();
+
+    // fused epilogue, RKVectorUpdater semantics (source/time_integrators.cc:128-155)
+    if (tid < ND) {
+      const double *nbo = buf + P::OFF_NB;
+      double k[NF];
+      double kp = nbo[ND + tid];
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) {
+        k[d] = nbo[4 * d * ND + tid];
+        if (d > 0) kp += nbo[(4 * d + 1) * ND + tid];
+      }
+      k[DIM] = kp;
+      const size_t cbase = (size_t)c * NF * ND;
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        const size_t idx = cbase + f * ND + tid;
+        const double bv = has_b ? buf[P::OFF_B + f * ND + tid] : 0.0;
+        const double o1 = bv + epi.a1 * k[f];
+        __builtin_nontemporal_store(o1, epi.o1 + idx);
+        if (epi.o2) {
+          __builtin_nontemporal_store(o1 + epi.a2 * k[f], epi.o2 + idx);
+          if (epi.o3) __builtin_nontemporal_store(o1 + epi.a3 * k[f], epi.o3 + idx);
+        }
+      }
+    }
+    // the next cell's prefetch reuses this buffer
+    if (t + 1 < TPG) pipe::barrier();
+  });
+}
+
+}  // namespace exw

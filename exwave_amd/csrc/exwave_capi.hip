@@ -111,6 +111,7 @@ static void free_op(exw_op op) {
   hipFree(op->d_speed);
   hipFree(op->d_h);
   hipFree(op->d_vertices);
+  hipFree(op->d_coef);
   hipFree(op->d_tab);
   hipFree(op->d_errtab);
   hipFree(op->d_cktab);
@@ -316,6 +317,19 @@ static int op_create(exw_ctx ctx, int dim, int degree, int n_cells, const double
       (rc = upload(op, &op->d_vertices, vertices, (size_t)n_cells * nv * dim))) {
     free_op(op);
     return rc;
+  }
+  if (cart && n_cells > 0) {
+    // the per-cell flux coefficients of the pipelined Cartesian kernel
+    const size_t n = (size_t)n_cells * (2 * dim * 8 + 2 * dim);
+    if (hipMalloc((void **)&op->d_coef, n * sizeof(double)) != hipSuccess) {
+      free_op(op);
+      return fail(EXW_ENOMEM, "exw_op_create: hipMalloc failed");
+    }
+    if (kernels(dim).coef(op->view(), op->d_coef, ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess) {
+      free_op(op);
+      return fail(EXW_EDEVICE, "exw_op_create: flux coefficient set-up failed");
+    }
   }
   if (n_hanging > 0) {
     // the cells that touch a hanging face, in cell order, with their face kinds

@@ -220,6 +220,7 @@ struct Pool {
     m.speed = speed + start;
     m.h = h + (size_t)start * dim;
     m.vertices = vert + (size_t)start * nv * dim;
+    m.coef = nullptr;
     return m;
   }
 };

@@ -1,11 +1,16 @@
 // launchers.hpp -- typed launchers and per-(dim,degree,precision,geometry)
 // dispatch tables of the MI355X kernels.
 #pragma once
+#include <stdlib.h>
+
+#include <type_traits>
+
 #include "dg_ader.hpp"
 #include "dg_lts.hpp"
 #include "dg_misc.hpp"
 #include "dg_operator.hpp"
 #include "dg_operator_cart.hpp"
+#include "dg_operator_pipe.hpp"
 #include "dg_nc.hpp"
 #include "dg_post.hpp"
 
@@ -42,7 +47,11 @@ using PostLaunch = hipError_t (*)(const DevMesh &, const double *, const double 
 using PostErrLaunch = hipError_t (*)(const DevMesh &, const double *, const DevModes &, double,
                                      const double *, double *, int, hipStream_t);
 
+// (mesh, per-cell flux coefficients out, stream)
+using CoefLaunch = hipError_t (*)(const DevMesh &, double *, hipStream_t);
+
 struct DimKernels {
+  CoefLaunch coef;
   OpLaunch op[MAX_DEGREE + 1][2][2];  // [k][fp32][cart]
   CkLaunch ck[MAX_DEGREE + 1][2][2];
   LtsLaunch lts[MAX_DEGREE + 1];
@@ -57,9 +66,47 @@ struct DimKernels {
   int cells_per_block[MAX_DEGREE + 1];
 };
 
+// The software-pipelined Cartesian kernel is the default where it applies;
+// EXW_CART_PIPE=0 in the environment selects dg_operator_cart_kernel instead.
+inline bool cart_pipe_enabled() {
+  static const bool on = [] {
+    const char *v = getenv("EXW_CART_PIPE");
+    return !(v && v[0] == '0');
+  }();
+  return on;
+}
+
+#ifndef EXW_PIPE_TPG
+#define EXW_PIPE_TPG 2
+#endif
+
+template <int DIM>
+hipError_t launch_cart_coef(const DevMesh &m, double *coef, hipStream_t s) {
+  const int64_t n = (int64_t)m.n_cells * 2 * DIM;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL((cart_coef_kernel<DIM>), dim3(int((n + 255) / 256)), dim3(256), 0, s, m,
+                     coef);
+  return hipGetLastError();
+}
+
 template <int DIM, int K, typename Number, bool CART>
 hipError_t launch_op(const DevMesh &m, const double *src, const double *tab,
                      const double *host_tab, const Epilogue &e, hipStream_t s) {
+  if constexpr (CART && std::is_same_v<Number, double> && PipeShape<DIM, K>::ENABLED) {
+    constexpr uint64_t cell_bytes = uint64_t(PipeShape<DIM, K>::NF) * PipeShape<DIM, K>::ND * 8;
+    if (cart_pipe_enabled() && m.coef && !e.cell_list && !e.cell_scale &&
+        (!e.o2 || e.o2_from_o1) &&
+        uint64_t(m.n_cells) * cell_bytes < (uint64_t(1) << 32) &&
+        (!PipeShape<DIM, K>::X4 ||
+         ((uintptr_t)src % 16 == 0 && (uintptr_t)e.b % 16 == 0))) {
+      constexpr int TPG = EXW_PIPE_TPG;
+      const int grid = (m.n_cells + TPG - 1) / TPG;
+      if (grid == 0) return hipSuccess;
+      hipLaunchKernelGGL((dg_operator_pipe_kernel<DIM, K, TPG>), dim3(grid),
+                         dim3(PipeShape<DIM, K>::BLOCK), 0, s, m, src, e);
+      return hipGetLastError();
+    }
+  }
   if constexpr (CART) {
     // box cells: node/line/node kernel with compile-time 1D matrices
     using CS = CartShape<DIM, K>;
@@ -170,6 +217,7 @@ hipError_t launch_post_error(const DevMesh &m, const double *tab, const DevModes
 
 template <int DIM, int K>
 void fill_dim_kernels(DimKernels &t) {
+  t.coef = launch_cart_coef<DIM>;
   if constexpr (K >= 1) {
     t.op[K][0][0] = launch_op<DIM, K, double, false>;
     t.op[K][0][1] = launch_op<DIM, K, double, true>;
