@@ -107,34 +107,113 @@ __global__ void cart_coef_kernel(const DevMesh mesh, double *__restrict__ coef) 
   }
 }
 
+// One DMA instruction of a cell's inputs: 64 lanes of 4 or 16 bytes (the
+// first `lanes` active) from byte goff of a source block to byte loff of the
+// cell's LDS buffer.  Sources: the cell's fields, the accumulator's, the
+// neighbour field fld = 2*face + component (0: v_n, 1: p), the coefficients.
+enum DmaSrc { DMA_OWN, DMA_ACC, DMA_NB, DMA_COEF };
+struct DmaOp {
+  int src, fld, x4, lanes, goff, loff;
+};
+constexpr int MAX_DMA_OPS = 128;
+struct DmaPlan {
+  DmaOp op[MAX_DMA_OPS];
+  int n;
+  int slot[12];  // LDS byte offset of neighbour field 2*face + component
+  int coef;      // LDS byte offset of the coefficients
+  int bytes;     // LDS bytes of one buffer, a multiple of 16
+};
+
+// Appends the instructions that cover `len` bytes at global byte g / LDS
+// byte l (equal modulo 16; every instruction stays inside the piece).  x4:
+// an 8-byte head in dwords where they are 8 mod 16, 16-byte lanes, an 8-byte
+// tail in dwords; else dword lanes.  Runs of >= 64 lanes are covered by full
+// instructions at lane offsets 0, 64, ... and a last one ending exactly at
+// the run's end (overlapping the one before with the same data).
+// (Plans longer than MAX_DMA_OPS are counted but not stored; such shapes are
+// not enabled.)
+constexpr void dma_cover(DmaPlan &p, int src, int fld, bool x4, int g, int l, int len) {
+  auto emit = [&](const DmaOp &o) {
+    if (p.n < MAX_DMA_OPS) p.op[p.n] = o;
+    ++p.n;
+  };
+  auto run = [&](bool wide, int off, int units) {  // units of 4 or 16 bytes from off
+    const int u = wide ? 16 : 4;
+    if (units >= 64) {
+      for (int q = 0; 64 * q < units; ++q) {
+        const int o = off + u * (64 * q < units - 64 ? 64 * q : units - 64);
+        emit(DmaOp{src, fld, wide ? 1 : 0, 64, g + o, l + o});
+      }
+    } else if (units > 0) {
+      emit(DmaOp{src, fld, wide ? 1 : 0, units, g + off, l + off});
+    }
+  };
+  if (!x4) {
+    run(false, 0, len / 4);
+    return;
+  }
+  const int head = (g % 16) ? 16 - g % 16 : 0;
+  run(false, 0, head / 4);
+  const int q = (len - head) / 16;
+  run(true, head, q);
+  run(false, head + 16 * q, (len - head - 16 * q) / 4);
+}
+
 template <int DIM, int K>
 struct PipeShape {
   using C = CartShape<DIM, K>;
   static constexpr int ND = C::ND, NF = C::NF, NFACE = 2 * DIM, NCOEF = NFACE * 8 + DIM * 2;
   static constexpr int WV = (ND + 63) / 64;  // waves per cell
   static constexpr int BLOCK = 64 * WV;
-  // the DMA blocks and the instructions that cover them, in units of one
-  // lane's load: the cell's fields (and the accumulator's) in 16-byte units
-  // where a cell's block is a multiple of 16 bytes (the host then requires
-  // 16-byte aligned vectors), else in dwords; one neighbour field and the
-  // coefficients in dwords
-  static constexpr bool X4 = (NF * ND * 8) % 16 == 0;
-  static constexpr int U_OWN = X4 ? 16 : 4;
-  static constexpr int W_OWN = NF * ND * 8 / U_OWN, DW_FLD = 2 * ND, DW_C = 2 * NCOEF;
-  static constexpr int NQ_OWN = (W_OWN + 63) / 64, NQ_FLD = (DW_FLD + 63) / 64,
-                       NQ_C = (DW_C + 63) / 64;
-  // instruction index ranges: own [0, I_B), accumulator [I_B, I_NB), neighbour
-  // fields (face f, component c; order 2f+c) [I_NB, I_C), coefficients [I_C, TOTAL)
-  static constexpr int I_B = NQ_OWN, I_NB = 2 * NQ_OWN, I_C = I_NB + 2 * NFACE * NQ_FLD,
-                       TOTAL = I_C + NQ_C;
-  // doubles of one LDS buffer: own fields, accumulator, neighbour fields, coefficients
-  static constexpr int OFF_B = NF * ND, OFF_NB = 2 * NF * ND, OFF_C = OFF_NB + 2 * NFACE * ND,
-                       BUF = (OFF_C + NCOEF + 1) / 2 * 2;  // 16-byte multiple
+  // 16-byte lanes in 3D, where every cell block (NF*ND*8 = 32*ND bytes) and
+  // coefficient block (NCOEF*8 = 432 bytes) starts 16-byte aligned in a
+  // 16-byte aligned vector (the host checks the vectors); a neighbour field
+  // then starts at 8*(field*ND % 2) mod 16.  2D: dword lanes throughout.
+  static constexpr bool X4 = DIM == 3;
+  static constexpr int OWN_BYTES = NF * ND * 8, FLD_BYTES = ND * 8;
+  static constexpr int field_of(int j) { return (j & 1) ? DIM : (j >> 2); }
+
+  static constexpr DmaPlan make_plan() {
+    DmaPlan p{};
+    p.n = 0;
+    dma_cover(p, DMA_OWN, 0, X4, 0, 0, OWN_BYTES);
+    dma_cover(p, DMA_ACC, 0, X4, 0, OWN_BYTES, OWN_BYTES);
+    // neighbour fields: each placed in LDS at its global alignment modulo 16,
+    // the next field taken among those that need no pad where there is one
+    int o = 2 * OWN_BYTES;
+    bool placed[12] = {};
+    for (int k = 0; k < 2 * NFACE; ++k) {
+      int pick = -1;
+      for (int pass = 0; pass < 2 && pick < 0; ++pass)
+        for (int j = 0; j < 2 * NFACE && pick < 0; ++j)
+          if (!placed[j] && (pass == 1 || !X4 || (field_of(j) * FLD_BYTES) % 16 == o % 16))
+            pick = j;
+      const int g = field_of(pick) * FLD_BYTES;
+      if (X4) o += (g % 16 - o % 16 + 16) % 16;
+      p.slot[pick] = o;
+      placed[pick] = true;
+      dma_cover(p, DMA_NB, pick, X4, g, o, FLD_BYTES);
+      o += FLD_BYTES;
+    }
+    o = (o + 15) / 16 * 16;
+    p.coef = o;
+    dma_cover(p, DMA_COEF, 0, X4, 0, o, NCOEF * 8);
+    p.bytes = (o + NCOEF * 8 + 15) / 16 * 16;
+    return p;
+  }
+  static constexpr DmaPlan PLAN = make_plan();
+  static constexpr int TOTAL = PLAN.n;
+  static constexpr int BUF = PLAN.bytes / 8;  // doubles of one LDS buffer
+  static constexpr int OFF_B = NF * ND;       // the accumulator, in doubles
+  static constexpr int OFF_C = PLAN.coef / 8;
+  // the slot (in doubles) of field c of face f
+  static constexpr int slot(int f, int c) { return PLAN.slot[2 * f + c] / 8; }
+
   // wave w issues the instructions i with i % WV == w
   static constexpr int count(int w, bool has_b) {
     int n = 0;
-    for (int i = w; i < TOTAL; i += WV)
-      if (has_b || i < I_B || i >= I_NB) ++n;
+    for (int i = w; i < TOTAL && i < MAX_DMA_OPS; i += WV)
+      if (has_b || PLAN.op[i].src != DMA_ACC) ++n;
     return n;
   }
   static constexpr int min_count(bool has_b) {
@@ -147,23 +226,22 @@ struct PipeShape {
     for (int w = 1; w < WV; ++w) m = count(w, has_b) > m ? count(w, has_b) : m;
     return m;
   }
-  static constexpr int LDS_BYTES = 2 * BUF * 8;
+  static constexpr int LDS_BYTES = 2 * PLAN.bytes;
   // cells that fill whole waves, and two buffers within a 64 KB workgroup allocation
-  static constexpr bool ENABLED = ND >= 48 && LDS_BYTES <= 65536;
-  static_assert(!ENABLED || (W_OWN >= 64 && DW_FLD >= 64 && DW_C >= 64),
-                "every DMA instruction covers 64 dwords of one block");
+  static constexpr bool ENABLED = ND >= 48 && LDS_BYTES <= 65536 && TOTAL <= MAX_DMA_OPS;
   static_assert(!ENABLED || max_count(true) + NF <= 63, "outstanding operations exceed vmcnt");
 };
 
 // The line phase of dg_operator_cart_kernel (phase B) on one LDS buffer, for
-// the line of direction d = line / NT: u the cell's fields [NF][ND]; nb the
-// neighbours' (v_n, p) per face [2*DIM][2][ND], whose [2d][0] and [2d][1]
-// receive the line's dv_d/dt and dp/dt share at its own nodes; fc the face
+// the line of direction d = line / NT: buf holds the cell's fields [NF][ND]
+// and the neighbour fields at their plan slots; the line's dv_d/dt and dp/dt
+// share are written over the fields of face 2d at its own nodes; fc the face
 // flux maps [2*DIM][8]; vc the cell-term factors [DIM][2].
 template <int DIM, int K>
-__device__ __forceinline__ void pipe_line(int line, const double *u, double *nb,
-                                          const double *fc, const double *vc) {
+__device__ __forceinline__ void pipe_line(int line, double *buf, const double *fc,
+                                          const double *vc) {
   using S = CartShape<DIM, K>;
+  using P = PipeShape<DIM, K>;
   constexpr int N = S::N, ND = S::ND, NT = S::NT;
   constexpr const CartConst<N> &T = CartTable<N>::value;
   const int d = line / NT, t = line - d * NT;
@@ -176,13 +254,23 @@ __device__ __forceinline__ void pipe_line(int line, const double *u, double *nb,
     base = d == 0 ? t0 * N + t1 * N * N : (d == 1 ? t0 + t1 * N * N : t0 + t1 * N);
     sd = d == 0 ? 1 : (d == 1 ? N : N * N);
   }
+  // slot of field C of face 2d + S: a select between compile-time constants
+  // (never an index into the plan, which would become a memory load)
+  auto slot = [&](auto S_, auto C_) {
+    constexpr int S = decltype(S_)::value, C = decltype(C_)::value;
+    constexpr int a = P::slot(S, C), b = P::slot(2 + S, C), e = P::slot(DIM == 3 ? 4 + S : S, C);
+    if constexpr (DIM == 2) return d == 0 ? a : b;
+    else return d == 0 ? a : (d == 1 ? b : e);
+  };
+  using I0 = std::integral_constant<int, 0>;
+  using I1 = std::integral_constant<int, 1>;
   const double cvv = vc[2 * d], cvp = vc[2 * d + 1];
   double vl[N], pl[N];
 #pragma unroll
   for (int i = 0; i < N; ++i) {
     const int n = base + i * sd;
-    vl[i] = u[d * ND + n];
-    pl[i] = u[DIM * ND + n];
+    vl[i] = buf[d * ND + n];
+    pl[i] = buf[DIM * ND + n];
   }
   double kv[N], kp[N];
 #pragma unroll
@@ -196,9 +284,11 @@ __device__ __forceinline__ void pipe_line(int line, const double *u, double *nb,
     kv[i] = cvv * gp;
     kp[i] = cvp * dv;
   }
-#pragma unroll
-  for (int s = 0; s < 2; ++s) {
+  static_for<2>([&](auto S_) __attribute__((always_inline)) {
+    constexpr int s = decltype(S_)::value;
+    constexpr const CartConst<N> &T = CartTable<N>::value;
     const int f = 2 * d + s;
+    const double *nv = buf + slot(S_, I0{}), *np = buf + slot(S_, I1{});
     double vo = 0.0, po = 0.0, vn = 0.0, pn = 0.0;
 #pragma unroll
     for (int i = 0; i < N; ++i) {
@@ -206,8 +296,8 @@ __device__ __forceinline__ void pipe_line(int line, const double *u, double *nb,
       const double ls = T.L[s * N + i], lo = T.L[(1 - s) * N + i];
       vo += ls * vl[i];
       po += ls * pl[i];
-      vn += lo * nb[(2 * f) * ND + n];
-      pn += lo * nb[(2 * f + 1) * ND + n];
+      vn += lo * nv[n];
+      pn += lo * np[n];
     }
     const double *c = fc + 8 * f;
     const double fv = c[0] * vo + c[1] * po + c[2] * vn + c[3] * pn;
@@ -218,12 +308,13 @@ __device__ __forceinline__ void pipe_line(int line, const double *u, double *nb,
       kv[i] += lw * fv;
       kp[i] += lw * fp;
     }
-  }
+  });
+  double *ov = buf + slot(I0{}, I0{}), *op = buf + slot(I0{}, I1{});
 #pragma unroll
   for (int i = 0; i < N; ++i) {
     const int n = base + i * sd;
-    nb[(4 * d) * ND + n] = kv[i];
-    nb[(4 * d + 1) * ND + n] = kp[i];
+    ov[n] = kv[i];
+    op[n] = kp[i];
   }
 }
 
@@ -248,7 +339,8 @@ __global__ void __launch_bounds__((PipeShape<DIM, K>::BLOCK)) __attribute__((amd
 
   const int tid = threadIdx.x;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const uint32_t lane4 = uint32_t(tid & 63) * 4u, lane16 = lane4 * 4u;
+  const int lane = tid & 63;
+  const uint32_t lane4 = uint32_t(lane) * 4u, lane16 = lane4 * 4u;
   const int c0 = xcd_remap(blockIdx.x, gridDim.x) * TPG;
   const int nt = min(TPG, mesh.n_cells - c0);  // >= 1: the grid is ceil(n_cells / TPG)
   const bool has_b = epi.b != nullptr;
@@ -265,38 +357,29 @@ __global__ void __launch_bounds__((PipeShape<DIM, K>::BLOCK)) __attribute__((amd
   // the array's base (scalar) + a 32-bit byte offset (vectors below 4 GiB)
   auto issue = [&](int c, const int (&nb)[NFACE], int buf) __attribute__((always_inline)) {
     const uint32_t lbase = pipe::lds_addr(&s_buf[buf][0]);
-    const uint32_t own = uint32_t(c) * uint32_t(NF * ND * 8);
+    const uint32_t own = uint32_t(c) * uint32_t(P::OWN_BYTES);
     static_for<WV>([&](auto W_) __attribute__((always_inline)) {
       constexpr int W = decltype(W_)::value;
       if (wave != W) return;
       static_for<(P::TOTAL - W + WV - 1) / WV>([&](auto J_) __attribute__((always_inline)) {
-        constexpr int i = W + decltype(J_)::value * WV;
-        if constexpr (i < P::I_NB) {
-          // the cell's fields, then the accumulator's
-          constexpr bool acc = i >= P::I_B;
-          constexpr int q = acc ? i - P::I_B : i;
-          constexpr int u0 = 64 * q < P::W_OWN - 64 ? 64 * q : P::W_OWN - 64;
-          constexpr uint32_t lo = (acc ? 8 * P::OFF_B : 0) + P::U_OWN * u0;
-          const double *base = acc ? epi.b : src;
-          if (!acc || has_b) {
-            if constexpr (P::X4)
-              pipe::dma_dwordx4(lbase + lo, base, lane16 + (own + P::U_OWN * u0));
-            else
-              pipe::dma_dword(lbase + lo, base, lane4 + (own + P::U_OWN * u0));
-          }
-        } else if constexpr (i < P::I_C) {
-          constexpr int j = i - P::I_NB, fld = j / P::NQ_FLD, q = j % P::NQ_FLD;
-          constexpr int f = fld >> 1, comp = fld & 1, field = comp ? DIM : (f >> 1);
-          constexpr int s0 = 64 * q < P::DW_FLD - 64 ? 64 * q : P::DW_FLD - 64;
+        constexpr DmaOp op = P::PLAN.op[W + decltype(J_)::value * WV];
+        if (op.src == DMA_ACC && !has_b) return;
+        const double *base = op.src == DMA_ACC ? epi.b : (op.src == DMA_COEF ? mesh.coef : src);
+        uint32_t g;
+        if constexpr (op.src == DMA_NB) {
           // boundary and hanging faces read the cell itself (zero flux coefficients)
-          const int cn = nb[f] >= 0 ? nb[f] : c;
-          const uint32_t g = (uint32_t(cn) * NF + field) * uint32_t(ND * 8);
-          pipe::dma_dword(lbase + 8 * (P::OFF_NB + fld * ND) + 4 * s0, src, lane4 + (g + 4 * s0));
+          const int f = op.fld >> 1, cn = nb[f] >= 0 ? nb[f] : c;
+          g = uint32_t(cn) * uint32_t(P::OWN_BYTES);
+        } else if constexpr (op.src == DMA_COEF) {
+          g = uint32_t(c) * uint32_t(P::NCOEF * 8);
         } else {
-          constexpr int q = i - P::I_C;
-          constexpr int s0 = 64 * q < P::DW_C - 64 ? 64 * q : P::DW_C - 64;
-          const uint32_t g = uint32_t(c) * uint32_t(P::NCOEF * 8);
-          pipe::dma_dword(lbase + 8 * P::OFF_C + 4 * s0, mesh.coef, lane4 + (g + 4 * s0));
+          g = own;
+        }
+        if (lane < op.lanes) {
+          if constexpr (op.x4)
+            pipe::dma_dwordx4(lbase + op.loff, base, lane16 + (g + op.goff));
+          else
+            pipe::dma_dword(lbase + op.loff, base, lane4 + (g + op.goff));
         }
       });
     });
@@ -326,20 +409,19 @@ __global__ void __launch_bounds__((PipeShape<DIM, K>::BLOCK)) __attribute__((amd
     pipe::barrier();
 
     double *buf = s_buf[t & 1];
-    if (tid < NLINE)
-      pipe_line<DIM, K>(tid, buf, buf + P::OFF_NB, buf + P::OFF_C, buf + P::OFF_C + NFACE * 8);
+    if (tid < NLINE) pipe_line<DIM, K>(tid, buf, buf + P::OFF_C, buf + P::OFF_C + NFACE * 8);
     pipe::barrier();
 
     // fused epilogue, RKVectorUpdater semantics (source/time_integrators.cc:128-155)
     if (tid < ND) {
-      const double *nbo = buf + P::OFF_NB;
       double k[NF];
-      double kp = nbo[ND + tid];
-#pragma unroll
-      for (int d = 0; d < DIM; ++d) {
-        k[d] = nbo[4 * d * ND + tid];
-        if (d > 0) kp += nbo[(4 * d + 1) * ND + tid];
-      }
+      double kp = 0.0;
+      static_for<DIM>([&](auto D_) __attribute__((always_inline)) {
+        constexpr int d = decltype(D_)::value;
+        constexpr int sv = P::slot(2 * d, 0), sp = P::slot(2 * d, 1);
+        k[d] = buf[sv + tid];
+        kp = d == 0 ? buf[sp + tid] : kp + buf[sp + tid];
+      });
       k[DIM] = kp;
       const size_t cbase = (size_t)c * NF * ND;
 #pragma unroll
