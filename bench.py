@@ -77,20 +77,31 @@ def parse():
     return ap.parse_args()
 
 
-TRAFFIC_FILE = os.path.join(HERE, "profiles", "dg_operator_cart_3d_k4_64cube_counters.md")
+TRAFFIC_FILES = [os.path.join(HERE, "profiles", f"dg_operator_{kind}_3d_k4_64cube_counters.md")
+                 for kind in ("pipe", "cart")]
 
 
 def measured_traffic(kernel_key):
-    """HBM bytes per launch of `kernel_key` from the committed counter summary
-    (a line 'traffic_bytes_per_launch <kernel_key> <bytes>')."""
-    try:
-        for line in open(TRAFFIC_FILE):
-            f = line.split()
-            if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel_key:
-                return int(float(f[2]))
-    except OSError:
-        pass
-    return None
+    """HBM bytes per launch of `kernel_key` and the committed counter summary
+    that holds it (a line 'traffic_bytes_per_launch <kernel_key> <bytes>')."""
+    for path in TRAFFIC_FILES:
+        try:
+            for line in open(path):
+                f = line.split()
+                if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel_key:
+                    return int(float(f[2])), path
+        except OSError:
+            pass
+    return None, None
+
+
+def stage_kernel(k, fp64):
+    """(family, name) of the kernel that runs the fused LSRK stage on Cartesian
+    3D cells: the pipelined FP64 kernel for k = 3, 4 (unless EXW_CART_PIPE=0),
+    else dg_operator_cart_kernel."""
+    if fp64 and k in (3, 4) and os.environ.get("EXW_CART_PIPE", "1")[:1] != "0":
+        return "pipe", f"dg_operator_pipe_kernel<3,{k},2>"
+    return "cart", f"dg_operator_cart_kernel<3,{k},{'double' if fp64 else 'float'}>"
 
 
 def copy_calibration():
@@ -538,6 +549,7 @@ def main():
     value = N * launches / t_sec / 1e9
     # algorithmic bytes per launch (average over the 5 stages: 18 vectors/step)
     n_cells = op.n_cells
+    family, kernel_name = stage_kernel(k, prec == X.FP64)
     bytes_per_launch = 18 * 8 * N / STAGES + 64 * n_cells
     achieved = bytes_per_launch / (stage_ms / 1e3) / 1e9
     # accuracy of the result (outside the timed region)
@@ -565,17 +577,18 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
                      "traffic": None,
-                     "kernel": f"dg_operator_cart_kernel<3,{k},{'double' if prec == X.FP64 else 'float'}>",
+                     "kernel": kernel_name,
                      "algorithmic_bytes_per_launch": int(bytes_per_launch),
                      "avg_launch_ms": round(stage_ms, 5)},
         "wall_s": round(wall, 4),
         "l2_error_after": {"t": t_end, "p": err[0], "v": err[1], "mag_p": err[2]},
         "energy_ratio": E1 / E0,
     }
-    key = f"3d_k{k}_n{n1d}_t{args.tile}_{'fp64' if prec == X.FP64 else 'fp32'}"
-    out["roofline"]["traffic"] = measured_traffic(key)
-    if out["roofline"]["traffic"]:
-        out["roofline"]["traffic_source"] = os.path.relpath(TRAFFIC_FILE, HERE)
+    key = f"{family}_3d_k{k}_n{n1d}_t{args.tile}_{'fp64' if prec == X.FP64 else 'fp32'}"
+    traffic, source = measured_traffic(key)
+    out["roofline"]["traffic"] = traffic
+    if traffic:
+        out["roofline"]["traffic_source"] = os.path.relpath(source, HERE)
     del vec_n, vec_np, work
     torch.cuda.empty_cache()
     if not args.no_cpu_baseline:
