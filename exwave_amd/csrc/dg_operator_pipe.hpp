@@ -1,5 +1,6 @@
-// dg_operator_pipe.hpp -- the Cartesian operator + RK epilogue as a persistent,
-// software-pipelined FP64 kernel for cells of >= 48 nodes (3D k >= 3, 2D k >= 6).
+// dg_operator_pipe.hpp -- the Cartesian operator + RK epilogue as a
+// software-pipelined FP64 kernel for cells of >= 48 nodes whose two LDS
+// buffers fit the workgroup (3D k = 3..6, 2D k >= 6).
 //
 // The arithmetic is dg_operator_cart_kernel's, expression for expression (its
 // face-lane flux maps are precomputed per cell by cart_coef_kernel from the
@@ -25,6 +26,12 @@
 // by instructions starting at 0, 64, ..., and one starting at W - 64).
 #pragma once
 #include "dg_operator_cart.hpp"
+
+// LDS of one workgroup's two buffers at most: gfx950's 160 KB (3D k=5 runs
+// 2 groups per CU, k=6 one)
+#ifndef EXW_PIPE_LDS_MAX
+#define EXW_PIPE_LDS_MAX 163840
+#endif
 
 namespace exw {
 namespace pipe {
@@ -227,8 +234,8 @@ struct PipeShape {
     return m;
   }
   static constexpr int LDS_BYTES = 2 * PLAN.bytes;
-  // cells that fill whole waves, and two buffers within a 64 KB workgroup allocation
-  static constexpr bool ENABLED = ND >= 48 && LDS_BYTES <= 65536 && TOTAL <= MAX_DMA_OPS;
+  // cells that fill whole waves, and two buffers within EXW_PIPE_LDS_MAX bytes
+  static constexpr bool ENABLED = ND >= 48 && LDS_BYTES <= EXW_PIPE_LDS_MAX && TOTAL <= MAX_DMA_OPS;
   static_assert(!ENABLED || max_count(true) + NF <= 63, "outstanding operations exceed vmcnt");
 };
 

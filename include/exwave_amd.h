@@ -113,7 +113,7 @@ int exw_memcpy_d2h(exw_ctx ctx, void *dst, const void *src, size_t bytes);
  *   rho, speed [n_cells]            material per cell (Material, include/utilities.h:28-34)
  * Axis-aligned box meshes select the Cartesian kernels, anything else the
  * general multilinear kernels (Jacobians recomputed from the vertices).
- * FP64 sweeps on box meshes of 3D degree 3-4 and 2D degree 6-8 run a
+ * FP64 sweeps on box meshes of 3D degree 3-6 and 2D degree 6-8 run a
  * software-pipelined kernel when their vectors are 16-byte aligned (3D) and
  * below 4 GiB, and there is no cell scale and no c operand; otherwise the
  * one-cell-per-group Cartesian kernel, with bit-identical results.

@@ -1,6 +1,6 @@
 """The two Cartesian stage kernels agree bit for bit, through the C-ABI.
 
-An FP64 fused sweep on Cartesian cells of 3D k=3, 4 or 2D k=6-8 runs the
+An FP64 fused sweep on Cartesian cells of 3D k=3..6 or 2D k=6-8 runs the
 software-pipelined kernel (dg_operator_pipe_kernel); with a cell-scale array
 of ones, or with vectors that are not 16-byte aligned, the same sweep runs
 dg_operator_cart_kernel instead.  Both evaluate the same expressions in the
@@ -23,6 +23,9 @@ CASES = [  # dim, intervals, degree, boundary id, material seed, refine seed (No
     (3, 4, 4, 1, None, None),
     (3, 3, 3, 3, 2, None),
     (3, 3, 4, 2, 3, 11),
+    (3, 3, 5, 2, 6, None),
+    (3, 3, 5, 1, None, 13),
+    (3, 2, 6, 3, 7, None),
     (2, 5, 6, 2, 4, None),
     (2, 4, 7, 3, None, None),
     (2, 4, 8, 1, 5, 12),

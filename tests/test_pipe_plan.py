@@ -7,8 +7,8 @@ program evaluates each PipeShape's compile-time plan and checks that:
   * 16-byte lanes are 16-byte aligned on both sides and dword lanes 4-byte
     aligned (global offsets relative to a 16-byte aligned cell block);
   * the pieces cover their LDS places completely and do not overlap;
-  * two buffers fit the 64 KB workgroup limit and the per-wave instruction
-    counts fit the 6-bit vmcnt together with one cell's stores.
+  * two buffers fit gfx950's 160 KB of LDS per workgroup and the per-wave
+    instruction counts fit the 6-bit vmcnt together with one cell's stores.
 """
 import os
 import subprocess
@@ -47,7 +47,7 @@ void check() {
     place(101, ob, ob);
     for (int j = 0; j < 4 * DIM; ++j) place(j, p.slot[j], fb);
     place(102, p.coef, cb);
-    CHECK(2 * p.bytes <= 65536, "dim %d k %d: %d bytes of LDS", DIM, K, 2 * p.bytes);
+    CHECK(2 * p.bytes <= 163840, "dim %d k %d: %d bytes of LDS", DIM, K, 2 * p.bytes);
     CHECK(p.bytes % 16 == 0, "dim %d k %d: buffer not a 16-byte multiple", DIM, K);
     if (P::X4) CHECK(ob % 16 == 0 && cb % 16 == 0, "dim %d k %d: blocks not 16-byte strided", DIM, K);
     for (int i = 0; i < p.n; ++i) {
