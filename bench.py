@@ -97,9 +97,9 @@ def measured_traffic(kernel_key):
 
 def stage_kernel(k, fp64):
     """(family, name) of the kernel that runs the fused LSRK stage on Cartesian
-    3D cells: the pipelined FP64 kernel for k = 3..6 (unless EXW_CART_PIPE=0),
+    3D cells: the pipelined FP64 kernel for k = 3..5 (unless EXW_CART_PIPE=0),
     else dg_operator_cart_kernel."""
-    if fp64 and 3 <= k <= 6 and os.environ.get("EXW_CART_PIPE", "1")[:1] != "0":
+    if fp64 and 3 <= k <= 5 and os.environ.get("EXW_CART_PIPE", "1")[:1] != "0":
         return "pipe", f"dg_operator_pipe_kernel<3,{k},2>"
     return "cart", f"dg_operator_cart_kernel<3,{k},{'double' if fp64 else 'float'}>"
 

@@ -72,6 +72,9 @@ struct Epilogue {
   int n_list;
   double *o3;
   double a3;
+  // set by the low-storage RK stage entry point: selects the pipelined
+  // Cartesian kernel for its single-output last stage as for the others
+  int rk_stage = 0;
 };
 
 // Optional cell selection of the local-time-stepping sweeps: the kernel runs

@@ -28,7 +28,8 @@
 #include "dg_operator_cart.hpp"
 
 // LDS of one workgroup's two buffers at most: gfx950's 160 KB (3D k=5 runs
-// 2 groups per CU, k=6 one)
+// 2 groups per CU, k=6 one; the host launches the latter nowhere, as the
+// one-cell kernel was measured faster there)
 #ifndef EXW_PIPE_LDS_MAX
 #define EXW_PIPE_LDS_MAX 163840
 #endif

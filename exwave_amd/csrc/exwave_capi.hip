@@ -420,7 +420,8 @@ static int check_prec(int precision) {
   return EXW_OK;
 }
 
-int exw_op_apply_fused(exw_op op, const double *src, const exw_epilogue *e, int precision) {
+static int apply_fused(exw_op op, const double *src, const exw_epilogue *e, int precision,
+                       bool rk_stage) {
   if (!op || !src || !e || !e->o1) return fail(EXW_EINVAL, "exw_op_apply_fused: null argument");
   if (int rc = check_prec(precision)) return rc;
   if ((const double *)e->o1 == src || (const double *)e->o2 == src)
@@ -428,7 +429,12 @@ int exw_op_apply_fused(exw_op op, const double *src, const exw_epilogue *e, int 
   if (e->o2 && !e->o2_from_o1 && !e->c)
     return fail(EXW_EINVAL, "exw_op_apply_fused: o2 needs c or o2_from_o1");
   Epilogue ep{e->b, e->o1, e->a1, e->c, e->o2, e->a2, e->o2_from_o1, e->cell_scale};
+  ep.rk_stage = rk_stage ? 1 : 0;
   return launch_epilogue(op, src, ep, precision);
+}
+
+int exw_op_apply_fused(exw_op op, const double *src, const exw_epilogue *e, int precision) {
+  return apply_fused(op, src, e, precision, false);
 }
 
 int exw_op_apply(exw_op op, const double *src, double *dst, int precision) {
@@ -441,7 +447,7 @@ int exw_rk_stage(exw_op op, const double *src, const double *b, double *y_out, d
   if (!b) return fail(EXW_EINVAL, "exw_rk_stage: b is NULL");
   if (!is_last && !b_out) return fail(EXW_EINVAL, "exw_rk_stage: b_out is NULL");
   exw_epilogue e{b, y_out, f1, nullptr, is_last ? nullptr : b_out, f2, 1, nullptr};
-  return exw_op_apply_fused(op, src, &e, precision);
+  return apply_fused(op, src, &e, precision, true);
 }
 
 }  // extern "C"

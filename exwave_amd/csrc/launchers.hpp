@@ -92,10 +92,17 @@ hipError_t launch_cart_coef(const DevMesh &m, double *coef, hipStream_t s) {
 template <int DIM, int K, typename Number, bool CART>
 hipError_t launch_op(const DevMesh &m, const double *src, const double *tab,
                      const double *host_tab, const Epilogue &e, hipStream_t s) {
-  if constexpr (CART && std::is_same_v<Number, double> && PipeShape<DIM, K>::ENABLED) {
+  // The pipelined kernel where it was measured faster: 2D (k = 6..8) always;
+  // 3D k = 3..5 for the low-storage RK stages only (two outputs; their last,
+  // single-output stage stays on the same kernel: both measured equal there)
+  // -- on the other single-output sweeps (ADER, Euler, apply) the one-cell
+  // kernel's higher occupancy wins by 3-4 %, and at 3D k = 6 (one 111 KB group
+  // per CU) by 16 %.
+  if constexpr (CART && std::is_same_v<Number, double> && PipeShape<DIM, K>::ENABLED &&
+                (DIM == 2 || K <= 5)) {
     constexpr uint64_t cell_bytes = uint64_t(PipeShape<DIM, K>::NF) * PipeShape<DIM, K>::ND * 8;
     if (cart_pipe_enabled() && m.coef && !e.cell_list && !e.cell_scale &&
-        (!e.o2 || e.o2_from_o1) &&
+        (DIM == 2 ? (!e.o2 || e.o2_from_o1) : ((e.o2 && e.o2_from_o1) || (e.rk_stage && !e.o2))) &&
         uint64_t(m.n_cells) * cell_bytes < (uint64_t(1) << 32) &&
         (!PipeShape<DIM, K>::X4 ||
          ((uintptr_t)src % 16 == 0 && (uintptr_t)e.b % 16 == 0))) {
