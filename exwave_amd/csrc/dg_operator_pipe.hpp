@@ -1,29 +1,29 @@
 // dg_operator_pipe.hpp -- the Cartesian operator + RK epilogue as a
 // software-pipelined FP64 kernel for cells of >= 48 nodes whose two LDS
-// buffers fit the workgroup (3D k = 3..6, 2D k >= 6).
+// buffers fit the workgroup (3D k = 3..6, 2D k >= 6; the host picks it where
+// it was measured faster than dg_operator_cart_kernel).
 //
 // The arithmetic is dg_operator_cart_kernel's, expression for expression (its
 // face-lane flux maps are precomputed per cell by cart_coef_kernel from the
-// same code, its line phase and epilogue are restated below), so both kernels
-// give bit-identical results; what differs is how the inputs reach the LDS.  Each
-// workgroup computes a few consecutive cells one after the other.  While it
-// computes cell t, the inputs of cell t+1 -- its nodes, the RK accumulator and
-// the 2*dim neighbours' (v_n, p) fields, its flux coefficients -- already
-// stream into the second of two
-// LDS buffers through gfx950's LDS-DMA loads (global_load_lds_dword: lane l's
-// dword lands at M0 + 4 l), which occupy no registers.  The memory pipe thus
-// always holds one cell ahead per group instead of idling while a cell
-// computes.
+// same code; its line phase and epilogue are restated below), so both kernels
+// give bit-identical results; what differs is how the inputs reach the LDS.
+// Each workgroup computes two consecutive cells one after the other.  While
+// it computes cell t, the inputs of cell t+1 -- its nodes, the RK accumulator,
+// the 2*dim neighbours' (v_n, p) fields and its flux coefficients -- already
+// stream into the second of two LDS buffers through gfx950's LDS-DMA loads
+// (global_load_lds_dword / _dwordx4: lane l's 4 or 16 bytes land at
+// M0 + 4l or M0 + 16l), which occupy no registers.
 //
 // The DMA is issued from inline assembly and waited for with explicit vmcnt
 // counts: the compiler treats every LDS-DMA as one alias class and would wait
 // for the prefetch together with the current cell.  Vector memory operations
 // of one wave complete in issue order, so "at most n outstanding" means all but
-// the last n issued are done.  For the count to be exact the loop issues no
-// vector loads the compiler knows of: neighbour rows, materials and extents
-// are wave-uniform scalar loads through the constant address space, and every
-// DMA instruction runs with all 64 lanes (a block of W >= 64 dwords is covered
-// by instructions starting at 0, 64, ..., and one starting at W - 64).
+// the last n issued are done.  For the count to be exact the kernel issues no
+// vector loads the compiler knows of: the neighbour rows are wave-uniform
+// scalar loads through the constant address space, and the per-cell LDS
+// layout comes from a compile-time plan (PipeShape::PLAN) whose every DMA
+// instruction has at least lane 0 active, so that each wave issues a fixed
+// number of them.
 #pragma once
 #include "dg_operator_cart.hpp"
 
@@ -333,9 +333,8 @@ __device__ __forceinline__ void pipe_line(int line, double *buf, const double *f
 // dg_operator_cart_kernel for those).  The vectors must be below 4 GiB
 // (n_cells * NF * ND * 8 < 2^32), and 16-byte aligned where PipeShape::X4.
 //
-// At least 2 waves per SIMD: with the full 512-register budget of a single
-// wave the backend assumes inline assembly may use the accumulation registers
-// and reserves as many as the kernel has VGPRs, which halves the occupancy.
+// The two buffers admit about 2 waves per SIMD (3D k=4: 4 groups of 2 waves
+// per CU); the register budget is capped to match.
 template <int DIM, int K, int TPG>
 __global__ void __launch_bounds__((PipeShape<DIM, K>::BLOCK)) __attribute__((amdgpu_waves_per_eu(2)))
     dg_operator_pipe_kernel(const DevMesh mesh, const double *__restrict__ src,
