@@ -12,11 +12,15 @@
 //
 // GPU layout: one lane per top-level Gauss node; the working field is double
 // buffered in LDS; the level-0 contribution stays in registers, lower levels'
-// contributions live in LDS.
+// contributions live in LDS.  On Cartesian cells each derivative runs in line
+// form (one lane per line of the level, then one per node) where that was
+// measured faster.
 #pragma once
+#include <type_traits>
 #include <utility>
 
 #include "dg_common.hpp"
+#include "gauss_cx.hpp"
 
 namespace exw {
 
@@ -125,6 +129,78 @@ __device__ __forceinline__ void derive(const Number (*in)[LD], const Number *Dl,
   }
 }
 
+// The same Cartesian derivative in line form, for lane `line` < DIM * NN^(DIM-1)
+// of a cell: along its line (direction d) the lane reads p and v_d of `in` into
+// registers once and, at the line's nodes, writes rho^-1 (d/dxi_d p) into
+// out[d] and d/dxi_d v_d over in[d] (only this lane reads those entries of
+// in[d]); derive_nodes then applies 1/h and forms the divergence.  The 1D
+// matrix is the compile-time one of NN Gauss points (CartTable, built by the
+// same routines as the level's table), and every sum and product runs in
+// derive's order: N reads per line instead of N per node and direction.
+template <int DIM, int NN, int LD, typename Number>
+__device__ __forceinline__ void derive_line(Number (*in)[LD], Number (*out)[LD], int line,
+                                            Number rho_inv) {
+  constexpr int NT = DIM == 2 ? NN : NN * NN;
+  constexpr const CartConst<NN> &T = CartTable<NN>::value;
+  const int d = line / NT, t = line - d * NT;
+  int base, sd;
+  if constexpr (DIM == 2) {
+    base = d == 0 ? t * NN : t;
+    sd = d == 0 ? 1 : NN;
+  } else {
+    const int t0 = t % NN, t1 = t / NN;
+    base = d == 0 ? t0 * NN + t1 * NN * NN : (d == 1 ? t0 + t1 * NN * NN : t0 + t1 * NN);
+    sd = d == 0 ? 1 : (d == 1 ? NN : NN * NN);
+  }
+  Number pl[NN], vl[NN];
+#pragma unroll
+  for (int m = 0; m < NN; ++m) {
+    pl[m] = in[DIM][base + m * sd];
+    vl[m] = in[d][base + m * sd];
+  }
+#pragma unroll
+  for (int i = 0; i < NN; ++i) {
+    Number gp = Number(0), gv = Number(0);
+#pragma unroll
+    for (int m = 0; m < NN; ++m) {
+      const Number dm = Number(T.D[i * NN + m]);
+      gp += dm * pl[m];
+      gv += dm * vl[m];
+    }
+    out[d][base + i * sd] = rho_inv * gp;
+    in[d][base + i * sd] = gv;
+  }
+}
+
+// The node half of the line form (after a barrier): outv_d = (rho^-1 d/dxi_d p)
+// / h_d from out[d], and outp = rho c^2 sum_d (d/dxi_d v_d) / h_d from in[d].
+template <int DIM, int LD, typename Number>
+__device__ __forceinline__ void derive_nodes(const Number (*in)[LD], const Number (*out)[LD],
+                                             int qq, const Number *ih, Number rc2, Number *outv,
+                                             Number &outp) {
+  Number div = Number(0);
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    outv[d] = out[d][qq] * ih[d];
+    div += in[d][qq] * ih[d];
+  }
+  outp = rc2 * div;
+}
+
+// whether the lines of a level of NN points fit the cell's lanes
+template <int DIM, int NN, int ND>
+constexpr bool lines_fit() {
+  return DIM * (DIM == 2 ? NN : NN * NN) <= ND;
+}
+
+// where the line form was measured faster (CK kernel alone): FP64 everywhere
+// (3D k=4: 1.11 against 1.31 ms at 64^3, k=6: 0.66 against 0.93 ms at 32^3,
+// 3D k=3 equal); FP32 only in 3D from k=4 (at 3D k=3 and 2D it lost 8-10 %)
+template <int DIM, int K, typename Number>
+constexpr bool lines_pay() {
+  return std::is_same_v<Number, double> || (DIM == 3 && K >= 4);
+}
+
 // One tensor pass along direction DIR of the matrix M (NO x NI, row-major) on
 // every field: the input has sizes (A0, A1, A2) with A_DIR = NI, the output
 // the same with NO in direction DIR.  Lane o computes output entry o.
@@ -231,8 +307,15 @@ __global__ void __launch_bounds__((Shape<DIM, K>::BLOCK))
     for (int f = 0; f < NF; ++f) up[f] = active ? Number(recon[cbase + f * ND + q]) : Number(0);
   } else {
     Number ov[DIM], op;
-    ck::derive<DIM, N, ND, CART>(s_a[lcc], s_ck + ckt.off_D[0], s_ck + ckt.off_X[0], q, ih,
-                                 rho_inv, rc2, Vc, ov, op);
+    if constexpr (CART && ck::lines_fit<DIM, N, ND>() && ck::lines_pay<DIM, K, Number>()) {
+      if (lane_ok && q < DIM * (DIM == 2 ? N : N * N))
+        ck::derive_line<DIM, N, ND>(s_a[lc], s_b[lc], q, rho_inv);
+      __syncthreads();
+      ck::derive_nodes<DIM, ND>(s_a[lcc], s_b[lcc], q, ih, rc2, ov, op);
+    } else {
+      ck::derive<DIM, N, ND, CART>(s_a[lcc], s_ck + ckt.off_D[0], s_ck + ckt.off_X[0], q, ih,
+                                   rho_inv, rc2, Vc, ov, op);
+    }
 #pragma unroll
     for (int d = 0; d < DIM; ++d) up[d] = -ov[d];
     up[DIM] = -op;
@@ -260,11 +343,21 @@ __global__ void __launch_bounds__((Shape<DIM, K>::BLOCK))
     const Number fac_t = Number(fac * (p2 - p1));
     Number(*cur)[ND] = parity ? s_b[lcc] : s_a[lcc];
     Number(*nxt)[ND] = parity ? s_a[lcc] : s_b[lcc];
+    constexpr bool LINES = CART && ck::lines_fit<DIM, n, ND>() && ck::lines_pay<DIM, K, Number>();
+    if constexpr (LINES) {
+      if (lane_ok && q < DIM * (DIM == 2 ? n : n * n))
+        ck::derive_line<DIM, n, ND>(cur, nxt, q, rho_inv);
+      __syncthreads();
+    }
     if (q < nd) {
       Number ov[DIM], op;
-      ck::derive<DIM, n, ND, CART>(cur, s_ck + ckt.off_D[LEVEL], s_ck + ckt.off_X[LEVEL], q, ih,
-                                   rho_inv, rc2, Vc, ov, op);
+      if constexpr (LINES)
+        ck::derive_nodes<DIM, ND>(cur, nxt, q, ih, rc2, ov, op);
+      else
+        ck::derive<DIM, n, ND, CART>(cur, s_ck + ckt.off_D[LEVEL], s_ck + ckt.off_X[LEVEL], q,
+                                     ih, rho_inv, rc2, Vc, ov, op);
       if (lane_ok) {
+        // (the line phase left rho^-1 d/dxi_d p there: this lane's own entries)
 #pragma unroll
         for (int d = 0; d < DIM; ++d) nxt[d][q] = ov[d];
         nxt[DIM][q] = op;
