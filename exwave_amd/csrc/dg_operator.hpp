@@ -22,12 +22,24 @@
 // Number is the operator arithmetic (double, or float for the mixed path);
 // loads, the epilogue and all vectors are FP64.
 #pragma once
+#include <type_traits>
+
 #include "dg_common.hpp"
 
 namespace exw {
 
+// Waves per SIMD the general kernel is held to (1: no constraint).  Capping
+// the registers spills to scratch and was measured slower for every variant
+// but 3D k=6 with the FP32 operator: 28.9 against 24.6 GDoF/s per stage at 4
+// waves (48^3 and 32^3 non-affine cells).
+template <int DIM, int K, typename Number>
+struct GeneralWaves {
+  static constexpr int value = DIM == 3 && K == 6 && std::is_same_v<Number, float> ? 4 : 1;
+};
+
 template <int DIM, int K, typename Number, bool CART>
 __global__ void __launch_bounds__((Shape<DIM, K>::BLOCK))
+    __attribute__((amdgpu_waves_per_eu(GeneralWaves<DIM, K, Number>::value)))
     dg_operator_kernel(const DevMesh mesh, const double *__restrict__ src,
                        const double *__restrict__ tab, const Epilogue epi) {
   using S = Shape<DIM, K>;
