@@ -500,8 +500,8 @@ def main():
         ader_measure()
         return
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); the "
-              "multi-GPU sharding is a design note in DESIGN.md", file=sys.stderr)
+        print(f"bench.py: one GPU only (got --gpus {args.gpus}); the path runs on a single "
+              "MI355X and has no multi-GPU sharding", file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch
