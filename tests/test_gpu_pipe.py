@@ -29,6 +29,8 @@ CASES = [  # dim, intervals, degree, boundary id, material seed, refine seed (No
     (2, 5, 6, 2, 4, None),
     (2, 4, 7, 3, None, None),
     (2, 4, 8, 1, 5, 12),
+    (3, 1, 4, 3, 9, None),  # one cell: one group with one cell, every face a boundary
+    (2, 1, 7, 1, None, None),
 ]
 
 
