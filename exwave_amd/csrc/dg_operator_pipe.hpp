@@ -80,7 +80,8 @@ __device__ __forceinline__ T sld(const T *p, size_t i) {
 
 }  // namespace pipe
 
-// Per-cell flux coefficients of a Cartesian mesh, one thread per (cell, face):
+// Per-cell flux coefficients of a Cartesian mesh for the cells first ..
+// first+count-1, one thread per (cell, face):
 // exactly the values dg_operator_cart_kernel's face lanes compute (the same
 // expressions; evaluate_inner_face / evaluate_boundary_face,
 // source/wave_equation_operations.cc:480-629).  coef[cell]: the face flux maps
@@ -89,11 +90,12 @@ __device__ __forceinline__ T sld(const T *p, size_t i) {
 // in; zero on hanging faces, which nc_face_kernel handles), then the cell-term
 // factors [dim][2] = (-1/(rho h_d), rho c^2/h_d).
 template <int DIM>
-__global__ void cart_coef_kernel(const DevMesh mesh, double *__restrict__ coef) {
+__global__ void cart_coef_kernel(const DevMesh mesh, int first, int count,
+                                 double *__restrict__ coef) {
   constexpr int NFACE = 2 * DIM, NCOEF = NFACE * 8 + DIM * 2;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (int64_t)mesh.n_cells * NFACE) return;
-  const int cell = int(i / NFACE), f = int(i % NFACE), d = f >> 1;
+  if (i >= (int64_t)count * NFACE) return;
+  const int cell = first + int(i / NFACE), f = int(i % NFACE), d = f >> 1;
   const int nb = mesh.neighbor[(size_t)cell * NFACE + f];
   const double rho = mesh.rho[cell], cc = mesh.speed[cell];
   const double rho_m = nb >= 0 ? mesh.rho[nb] : rho, c_m = nb >= 0 ? mesh.speed[nb] : cc;
@@ -326,16 +328,17 @@ __device__ __forceinline__ void pipe_line(int line, double *buf, const double *f
   }
 }
 
-// TPG consecutive cells per group, one after the other, the loop unrolled at
-// compile time (a run-time loop would have its 1D-matrix constants hoisted
-// out of it and spilled).  Needs mesh.coef.  Epilogue: b optional, o2 only
-// with o2_from_o1; no c, cell list or cell scale (the host uses
-// dg_operator_cart_kernel for those).  The vectors must be below 4 GiB
+// TPG consecutive cells (or cell-list entries) per group, one after the
+// other, the loop unrolled at compile time (a run-time loop would have its
+// 1D-matrix constants hoisted out of it and spilled).  Needs mesh.coef.
+// Epilogue: b optional, o2 only with o2_from_o1, cell list and cell scales
+// with SEL (a separate instantiation, so that the plain sweep keeps its
+// code); no c (the host uses dg_operator_cart_kernel for that).  The vectors must be below 4 GiB
 // (n_cells * NF * ND * 8 < 2^32), and 16-byte aligned where PipeShape::X4.
 //
 // The two buffers admit about 2 waves per SIMD (3D k=4: 4 groups of 2 waves
 // per CU); the register budget is capped to match.
-template <int DIM, int K, int TPG>
+template <int DIM, int K, int TPG, bool SEL>
 __global__ void __launch_bounds__((PipeShape<DIM, K>::BLOCK)) __attribute__((amdgpu_waves_per_eu(2)))
     dg_operator_pipe_kernel(const DevMesh mesh, const double *__restrict__ src,
                             const Epilogue epi) {
@@ -348,17 +351,26 @@ __global__ void __launch_bounds__((PipeShape<DIM, K>::BLOCK)) __attribute__((amd
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lane = tid & 63;
   const uint32_t lane4 = uint32_t(lane) * 4u, lane16 = lane4 * 4u;
+  // entries e = c0 .. c0+nt-1 of the cell sequence (all cells, or the list)
+  const int n_entries = SEL && epi.cell_list ? epi.n_list : mesh.n_cells;
   const int c0 = xcd_remap(blockIdx.x, gridDim.x) * TPG;
-  const int nt = min(TPG, mesh.n_cells - c0);  // >= 1: the grid is ceil(n_cells / TPG)
+  const int nt = min(TPG, n_entries - c0);  // >= 1: the grid is ceil(n_entries / TPG)
   const bool has_b = epi.b != nullptr;
 
-  // the neighbour rows of the group's cells (cells past the end repeat c0)
-  int rows[TPG][NFACE];
+  // the group's cells (entries past the end repeat the first), their scale
+  // (a cell of scale 0 is inactive: loaded, not computed, not stored) and
+  // neighbour rows
+  int cells[TPG], rows[TPG][NFACE];
+  double cs[TPG];
 #pragma unroll
-  for (int t = 0; t < TPG; ++t)
+  for (int t = 0; t < TPG; ++t) {
+    const int e = t < nt ? c0 + t : c0;
+    cells[t] = SEL && epi.cell_list ? pipe::sld(epi.cell_list, e) : e;
+    cs[t] = SEL && epi.cell_scale ? pipe::sld(epi.cell_scale, cells[t]) : 1.0;
 #pragma unroll
     for (int f = 0; f < NFACE; ++f)
-      rows[t][f] = pipe::sld(mesh.neighbor, (size_t)(t < nt ? c0 + t : c0) * NFACE + f);
+      rows[t][f] = pipe::sld(mesh.neighbor, (size_t)cells[t] * NFACE + f);
+  }
 
   // this wave's share of cell c's DMA into buffer buf; global addresses are
   // the array's base (scalar) + a 32-bit byte offset (vectors below 4 GiB)
@@ -392,35 +404,52 @@ __global__ void __launch_bounds__((PipeShape<DIM, K>::BLOCK)) __attribute__((amd
     });
   };
 
-  issue(c0, rows[0], 0);
+  issue(cells[0], rows[0], 0);
   static_for<TPG>([&](auto T_) __attribute__((always_inline)) {
     constexpr int t = decltype(T_)::value;
     if (t >= nt) return;  // uniform over the group
-    const int c = c0 + t;
+    const int c = cells[t];
+    constexpr int tn = t + 1 < TPG ? t + 1 : t;
     // prefetch cell t+1, then wait for cell t.  Outstanding, oldest first:
-    // DMA(t), stores(t-1) (>= NF: o1 of every field), DMA(t+1).
+    // DMA(t), stores(t-1) (>= NF: o1 of every field, if cell t-1 was
+    // active; none otherwise), DMA(t+1).
+    const bool stored = t > 0 && (!SEL || cs[t > 0 ? t - 1 : 0] != 0.0);
     if (t + 1 < nt) {
       if (has_b)
         pipe::wait_vm<63 - P::max_count(true)>();
       else
         pipe::wait_vm<63 - P::max_count(false)>();
-      issue(c + 1, rows[t + 1 < TPG ? t + 1 : t], (t + 1) & 1);
-      constexpr int after = t == 0 ? 0 : NF;
-      if (has_b)
-        pipe::wait_vm<P::min_count(true) + after>();
-      else
-        pipe::wait_vm<P::min_count(false) + after>();
+      issue(cells[tn], rows[tn], (t + 1) & 1);
+      if (stored) {
+        if (has_b)
+          pipe::wait_vm<P::min_count(true) + NF>();
+        else
+          pipe::wait_vm<P::min_count(false) + NF>();
+      } else {
+        if (has_b)
+          pipe::wait_vm<P::min_count(true)>();
+        else
+          pipe::wait_vm<P::min_count(false)>();
+      }
     } else {
-      pipe::wait_vm<t == 0 ? 0 : NF>();
+      if (stored)
+        pipe::wait_vm<NF>();
+      else
+        pipe::wait_vm<0>();
     }
     pipe::barrier();
+    const bool active = !SEL || cs[t] != 0.0;  // uniform over the group
 
     double *buf = s_buf[t & 1];
-    if (tid < NLINE) pipe_line<DIM, K>(tid, buf, buf + P::OFF_C, buf + P::OFF_C + NFACE * 8);
+    if (active && tid < NLINE)
+      pipe_line<DIM, K>(tid, buf, buf + P::OFF_C, buf + P::OFF_C + NFACE * 8);
     pipe::barrier();
 
     // fused epilogue, RKVectorUpdater semantics (source/time_integrators.cc:128-155)
-    if (tid < ND) {
+    if (active && tid < ND) {
+      // (without SEL the scale is 1 and the products are exact)
+      const double ea1 = SEL ? epi.a1 * cs[t] : epi.a1, ea2 = SEL ? epi.a2 * cs[t] : epi.a2,
+                   ea3 = SEL ? epi.a3 * cs[t] : epi.a3;
       double k[NF];
       double kp = 0.0;
       static_for<DIM>([&](auto D_) __attribute__((always_inline)) {
@@ -435,11 +464,11 @@ __global__ void __launch_bounds__((PipeShape<DIM, K>::BLOCK)) __attribute__((amd
       for (int f = 0; f < NF; ++f) {
         const size_t idx = cbase + f * ND + tid;
         const double bv = has_b ? buf[P::OFF_B + f * ND + tid] : 0.0;
-        const double o1 = bv + epi.a1 * k[f];
+        const double o1 = bv + ea1 * k[f];
         __builtin_nontemporal_store(o1, epi.o1 + idx);
         if (epi.o2) {
-          __builtin_nontemporal_store(o1 + epi.a2 * k[f], epi.o2 + idx);
-          if (epi.o3) __builtin_nontemporal_store(o1 + epi.a3 * k[f], epi.o3 + idx);
+          __builtin_nontemporal_store(o1 + ea2 * k[f], epi.o2 + idx);
+          if (epi.o3) __builtin_nontemporal_store(o1 + ea3 * k[f], epi.o3 + idx);
         }
       }
     }

@@ -325,7 +325,7 @@ static int op_create(exw_ctx ctx, int dim, int degree, int n_cells, const double
       free_op(op);
       return fail(EXW_ENOMEM, "exw_op_create: hipMalloc failed");
     }
-    if (kernels(dim).coef(op->view(), op->d_coef, ctx->stream) != hipSuccess ||
+    if (kernels(dim).coef(op->view(), 0, n_cells, op->d_coef, ctx->stream) != hipSuccess ||
         hipStreamSynchronize(ctx->stream) != hipSuccess) {
       free_op(op);
       return fail(EXW_EDEVICE, "exw_op_create: flux coefficient set-up failed");

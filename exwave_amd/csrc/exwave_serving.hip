@@ -186,6 +186,7 @@ struct Pool {
   hipStream_t stream = nullptr;
   int *nbr = nullptr;
   double *rho = nullptr, *speed = nullptr, *h = nullptr, *vert = nullptr, *scale = nullptr;
+  double *coef = nullptr;  // per-cell flux coefficients (DevMesh::coef), set at admission
   double *reg[3] = {nullptr, nullptr, nullptr};
   std::vector<double> tab, etab, ckt;
   CkTab ck{};
@@ -208,8 +209,9 @@ struct Pool {
   std::deque<hipEvent_t> inflight;
   std::vector<hipEvent_t> event_pool;
 
+  size_t ncoef() const { return (size_t)nface * 8 + 2 * (size_t)dim; }
   size_t bytes_per_cell() const {
-    return sizeof(double) * (3 * (size_t)dpc + 3 + dim + (size_t)nv * dim + 1) +
+    return sizeof(double) * (3 * (size_t)dpc + 3 + dim + (size_t)nv * dim + 1 + ncoef()) +
            sizeof(int) * nface;
   }
   DevMesh view(int start, int n) const {
@@ -220,7 +222,7 @@ struct Pool {
     m.speed = speed + start;
     m.h = h + (size_t)start * dim;
     m.vertices = vert + (size_t)start * nv * dim;
-    m.coef = nullptr;
+    m.coef = coef ? coef + (size_t)start * ncoef() : nullptr;
     return m;
   }
 };
@@ -252,9 +254,10 @@ void free_pool_arrays(Pool *p) {
   hipFree(p->h);
   hipFree(p->vert);
   hipFree(p->scale);
+  hipFree(p->coef);
   for (int i = 0; i < 3; ++i) hipFree(p->reg[i]);
   p->nbr = nullptr;
-  p->rho = p->speed = p->h = p->vert = p->scale = nullptr;
+  p->rho = p->speed = p->h = p->vert = p->scale = p->coef = nullptr;
   for (int i = 0; i < 3; ++i) p->reg[i] = nullptr;
 }
 
@@ -320,14 +323,15 @@ int grow_pool(exw_server s, Pool *p, int new_cap) {
   if (s->used + extra > s->budget) return sfail(EXW_ENOMEM, "serving HBM budget exhausted");
   int *nbr = nullptr;
   double *rho = nullptr, *speed = nullptr, *h = nullptr, *vert = nullptr, *scale = nullptr,
-         *reg[3] = {nullptr, nullptr, nullptr};
+         *coef = nullptr, *reg[3] = {nullptr, nullptr, nullptr};
   const size_t nc = (size_t)new_cap;
   bool ok = hipMalloc((void **)&nbr, sizeof(int) * nc * p->nface) == hipSuccess &&
             hipMalloc((void **)&rho, sizeof(double) * nc) == hipSuccess &&
             hipMalloc((void **)&speed, sizeof(double) * nc) == hipSuccess &&
             hipMalloc((void **)&h, sizeof(double) * nc * p->dim) == hipSuccess &&
             hipMalloc((void **)&vert, sizeof(double) * nc * p->nv * p->dim) == hipSuccess &&
-            hipMalloc((void **)&scale, sizeof(double) * nc) == hipSuccess;
+            hipMalloc((void **)&scale, sizeof(double) * nc) == hipSuccess &&
+            hipMalloc((void **)&coef, sizeof(double) * nc * p->ncoef()) == hipSuccess;
   for (int i = 0; i < 3 && ok; ++i)
     ok = hipMalloc((void **)&reg[i], sizeof(double) * nc * p->dpc) == hipSuccess;
   if (!ok) {
@@ -337,6 +341,7 @@ int grow_pool(exw_server s, Pool *p, int new_cap) {
     hipFree(h);
     hipFree(vert);
     hipFree(scale);
+    hipFree(coef);
     for (int i = 0; i < 3; ++i) hipFree(reg[i]);
     return sfail(EXW_ENOMEM, "serving pool allocation failed");
   }
@@ -350,6 +355,8 @@ int grow_pool(exw_server s, Pool *p, int new_cap) {
     SHIP(hipMemcpyAsync(vert, p->vert, sizeof(double) * oc * p->nv * p->dim,
                         hipMemcpyDeviceToDevice, st));
     SHIP(hipMemcpyAsync(scale, p->scale, sizeof(double) * oc, hipMemcpyDeviceToDevice, st));
+    SHIP(hipMemcpyAsync(coef, p->coef, sizeof(double) * oc * p->ncoef(),
+                        hipMemcpyDeviceToDevice, st));
     for (int i = 0; i < 3; ++i)
       SHIP(hipMemcpyAsync(reg[i], p->reg[i], sizeof(double) * oc * p->dpc,
                           hipMemcpyDeviceToDevice, st));
@@ -368,6 +375,7 @@ int grow_pool(exw_server s, Pool *p, int new_cap) {
   p->h = h;
   p->vert = vert;
   p->scale = scale;
+  p->coef = coef;
   for (int i = 0; i < 3; ++i) p->reg[i] = reg[i];
   // the new tail joins the free list (merged with a free range ending at cap)
   int start = (int)oc, len = (int)nn;
@@ -459,6 +467,7 @@ int enqueue_step_launches(Pool *p, int from) {
           double f2, bool last) {
         Epilogue e{acc,     y_out,   f1, nullptr, last ? nullptr : b_out, f2, 1, p->scale,
                    p->d_list, p->n_list};
+        e.rk_stage = 1;
         return l(m, src, p->d_tab, host_tab, e, p->stream) == hipSuccess ? 0 : EXW_EDEVICE;
       });
   if (rc) return sfail(EXW_EDEVICE, "serving step launch failed");
@@ -531,6 +540,8 @@ int admit(exw_server s, Instance *in) {
                       sizeof(double) * in->vert.size(), hipMemcpyHostToDevice, st));
   SHIP(hipMemcpyAsync(p->scale + start, in->scale.data(), sizeof(double) * n,
                       hipMemcpyHostToDevice, st));
+  // the flux coefficients of the new cells (neighbour indices are pool-wide)
+  SHIP(dim_kernels(dim).coef(p->view(0, p->cap), start, n, p->coef, st));
   // initial field into the pool's current register (project_initial_field)
   SHIP(dim_kernels(dim).project[r.degree](p->view(start, n), p->d_tab, modes_of(r), 0.0,
                                            p->reg[p->parity] + (size_t)start * p->dpc, st));

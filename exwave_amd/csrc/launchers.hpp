@@ -47,8 +47,8 @@ using PostLaunch = hipError_t (*)(const DevMesh &, const double *, const double 
 using PostErrLaunch = hipError_t (*)(const DevMesh &, const double *, const DevModes &, double,
                                      const double *, double *, int, hipStream_t);
 
-// (mesh, per-cell flux coefficients out, stream)
-using CoefLaunch = hipError_t (*)(const DevMesh &, double *, hipStream_t);
+// (mesh, first cell, cell count, per-cell flux coefficients out, stream)
+using CoefLaunch = hipError_t (*)(const DevMesh &, int, int, double *, hipStream_t);
 
 struct DimKernels {
   CoefLaunch coef;
@@ -81,11 +81,13 @@ inline bool cart_pipe_enabled() {
 #endif
 
 template <int DIM>
-hipError_t launch_cart_coef(const DevMesh &m, double *coef, hipStream_t s) {
-  const int64_t n = (int64_t)m.n_cells * 2 * DIM;
+hipError_t launch_cart_coef(const DevMesh &m, int first, int count, double *coef,
+                            hipStream_t s) {
+  if (first < 0 || count < 0 || first + count > m.n_cells) return hipErrorInvalidValue;
+  const int64_t n = (int64_t)count * 2 * DIM;
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL((cart_coef_kernel<DIM>), dim3(int((n + 255) / 256)), dim3(256), 0, s, m,
-                     coef);
+                     first, count, coef);
   return hipGetLastError();
 }
 
@@ -101,16 +103,21 @@ hipError_t launch_op(const DevMesh &m, const double *src, const double *tab,
   if constexpr (CART && std::is_same_v<Number, double> && PipeShape<DIM, K>::ENABLED &&
                 (DIM == 2 || K <= 5)) {
     constexpr uint64_t cell_bytes = uint64_t(PipeShape<DIM, K>::NF) * PipeShape<DIM, K>::ND * 8;
-    if (cart_pipe_enabled() && m.coef && !e.cell_list && !e.cell_scale &&
+    if (cart_pipe_enabled() && m.coef &&
         (DIM == 2 ? (!e.o2 || e.o2_from_o1) : ((e.o2 && e.o2_from_o1) || (e.rk_stage && !e.o2))) &&
         uint64_t(m.n_cells) * cell_bytes < (uint64_t(1) << 32) &&
         (!PipeShape<DIM, K>::X4 ||
          ((uintptr_t)src % 16 == 0 && (uintptr_t)e.b % 16 == 0))) {
       constexpr int TPG = EXW_PIPE_TPG;
-      const int grid = (m.n_cells + TPG - 1) / TPG;
+      const int n = e.cell_list ? e.n_list : m.n_cells;
+      const int grid = (n + TPG - 1) / TPG;
       if (grid == 0) return hipSuccess;
-      hipLaunchKernelGGL((dg_operator_pipe_kernel<DIM, K, TPG>), dim3(grid),
-                         dim3(PipeShape<DIM, K>::BLOCK), 0, s, m, src, e);
+      if (e.cell_list || e.cell_scale)
+        hipLaunchKernelGGL((dg_operator_pipe_kernel<DIM, K, TPG, true>), dim3(grid),
+                           dim3(PipeShape<DIM, K>::BLOCK), 0, s, m, src, e);
+      else
+        hipLaunchKernelGGL((dg_operator_pipe_kernel<DIM, K, TPG, false>), dim3(grid),
+                           dim3(PipeShape<DIM, K>::BLOCK), 0, s, m, src, e);
       return hipGetLastError();
     }
   }
