@@ -115,9 +115,9 @@ int exw_memcpy_d2h(exw_ctx ctx, void *dst, const void *src, size_t bytes);
  * general multilinear kernels (Jacobians recomputed from the vertices).
  * FP64 RK stages with two outputs on box meshes of 3D degree 3-5, and FP64
  * sweeps on 2D degree 6-8, run a software-pipelined kernel when their vectors
- * are 16-byte aligned (3D) and below 4 GiB, and there is no cell scale and no
- * c operand; otherwise the one-cell-per-group Cartesian kernel, with
- * bit-identical results.
+ * are 16-byte aligned (3D) and below 4 GiB and there is no c operand;
+ * otherwise the one-cell-per-group Cartesian kernel, with bit-identical
+ * results.
  * EXW_CART_PIPE=0 in the environment selects the latter throughout. */
 int exw_op_create(exw_ctx ctx, int dim, int degree, int n_cells, const double *vertices,
                   const int *neighbor, const double *rho, const double *speed, exw_op *out);
