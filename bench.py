@@ -419,27 +419,6 @@ def serving_measure(ctx, cpu_seconds, want_cpu):
     return out
 
 
-def tile_order(V, nb, n1d, tile):
-    """Renumber the cells of the lexicographic n1d^3 mesh tile by tile (tiles
-    of tile^3 cells in lexicographic order, cells lexicographic inside a tile)
-    so that the z neighbours of a cell are tile^2 cells away instead of
-    n1d^2."""
-    import numpy as np
-    nt = (n1d + tile - 1) // tile
-    idx = np.arange(n1d ** 3)
-    x, y, z = idx % n1d, (idx // n1d) % n1d, idx // (n1d * n1d)
-    tx, ty, tz = x // tile, y // tile, z // tile
-    key = (((tz * nt + ty) * nt + tx) * tile + z % tile) * tile * tile + (y % tile) * tile + x % tile
-    order = np.argsort(key, kind="stable")  # new -> old
-    new_of_old = np.empty_like(order)
-    new_of_old[order] = np.arange(order.size)
-    V2 = V[order]
-    nb2 = nb[order].copy()
-    m = nb2 >= 0
-    nb2[m] = new_of_old[nb2[m]]
-    return V2, nb2
-
-
 def cpu_baseline(n1d, k, target_s):
     """Time the CPU oracle's apply + RKVectorUpdater on the same mesh and data
     (stage by stage, as the reference executes it) until ~target_s elapsed."""
@@ -513,7 +492,7 @@ def main():
     n1d, k = args.n, args.degree
     if args.tile:
         V, nb = X.mesh_hypercube(DIM, n1d, 0, X.TRANSFORM_NONE, 0.0, 2)
-        V, nb = tile_order(V, nb, n1d, args.tile)
+        V, nb = X.tile_order(V, nb, n1d, args.tile)
         ones = np.ones(V.shape[0])
         op = X.WaveOperator(ctx, DIM, k, V, nb, ones, ones)
     else:

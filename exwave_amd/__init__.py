@@ -19,7 +19,7 @@ from .capi import (ADER, ADER_ADCONFULL, ADER_LTS, CLRK4, EXPL_EULER, FP32, FP64
                    PRECISION_AUTO, INTEGRATORS, LSRK33R2, LSRK45R2, LSRK45R3, SSPRK,
                    LSRK59R2, TRANSFORM_AXIS_2PI, TRANSFORM_NONE, TRANSFORM_PRODUCT_2PI,
                    TRANSFORM_REFERENCE_SINE, NC_FACE, Epilogue, ExwError, Modes, Problem, check,
-                   load, mesh_box_refined, mesh_hypercube, write_vtu)
+                   load, mesh_box_refined, mesh_hypercube, tile_order, write_vtu)
 
 
 def _ptr(t):

@@ -220,3 +220,25 @@ def mesh_hypercube(dim, n_intervals, n_refinements=0, transform=TRANSFORM_NONE, 
     check(L.exw_mesh_hypercube(dim, n_intervals, n_refinements, transform, factor, boundary_id,
                                C.byref(n), V.ctypes.data_as(_DP), nb.ctypes.data_as(_IP)))
     return V, nb
+
+
+def tile_order(V, nb, n1d, tile):
+    """Renumber the cells of the lexicographic n1d^3 mesh of mesh_hypercube tile
+    by tile (tiles of tile^3 cells in lexicographic order, cells lexicographic
+    inside a tile), as MatrixFree renumbers cells for locality: the z
+    neighbours of a cell are then tile^2 cells away instead of n1d^2.  Returns
+    (vertices, neighbor) in the new order."""
+    import numpy as np
+    nt = (n1d + tile - 1) // tile
+    idx = np.arange(n1d ** 3)
+    x, y, z = idx % n1d, (idx // n1d) % n1d, idx // (n1d * n1d)
+    tx, ty, tz = x // tile, y // tile, z // tile
+    key = (((tz * nt + ty) * nt + tx) * tile + z % tile) * tile * tile + (y % tile) * tile + x % tile
+    order = np.argsort(key, kind="stable")  # new -> old
+    new_of_old = np.empty_like(order)
+    new_of_old[order] = np.arange(order.size)
+    V2 = V[order]
+    nb2 = nb[order].copy()
+    m = nb2 >= 0
+    nb2[m] = new_of_old[nb2[m]]
+    return V2, nb2

@@ -166,6 +166,26 @@ DevModes modes_of(const exw_request &r) {
   return d;
 }
 
+// Per-cell flux coefficients of a pool: only the FP64 sweeps that may run the
+// pipelined kernel read them (DimKernels::pipe); other pools hold none.
+size_t pool_ncoef(int dim, int k, int prec) {
+  return (prec == EXW_FP64 && dim_kernels(dim).pipe[k]) ? (size_t)2 * dim * 8 + 2 * (size_t)dim
+                                                         : 0;
+}
+
+// Device bytes per pool cell: three RK registers, rho, speed, scale, h[dim],
+// the vertices, the flux coefficients and the neighbour indices.  Admission
+// (Pool growth) and the submit-time size check both use this figure; the
+// int share covers the neighbour indices and the cell's active-list entry.
+size_t pool_cell_bytes(int dim, int k, int prec) {
+  const size_t nv = (size_t)1 << dim;
+  size_t nd = 1;
+  for (int d = 0; d < dim; ++d) nd *= (size_t)(k + 1);
+  const size_t dpc = (size_t)(dim + 1) * nd;
+  return sizeof(double) * (3 * dpc + 3 + dim + nv * dim + pool_ncoef(dim, k, prec)) +
+         sizeof(int) * (2 * (size_t)dim + 1);
+}
+
 struct Instance {
   int64_t id;
   exw_request req;
@@ -209,11 +229,8 @@ struct Pool {
   std::deque<hipEvent_t> inflight;
   std::vector<hipEvent_t> event_pool;
 
-  size_t ncoef() const { return (size_t)nface * 8 + 2 * (size_t)dim; }
-  size_t bytes_per_cell() const {
-    return sizeof(double) * (3 * (size_t)dpc + 3 + dim + (size_t)nv * dim + 1 + ncoef()) +
-           sizeof(int) * nface;
-  }
+  size_t ncoef() const { return pool_ncoef(dim, k, prec); }
+  size_t bytes_per_cell() const { return pool_cell_bytes(dim, k, prec); }
   DevMesh view(int start, int n) const {
     DevMesh m;
     m.n_cells = n;
@@ -238,6 +255,9 @@ struct exw_server_s {
   std::vector<Instance *> finishing;
   int64_t next_id = 0;
   int lookahead = 32;
+  // error-evaluation buffers of polled results (device partials, pinned sums),
+  // reused by later finalisations: hipFree would wait for every pool's stream
+  std::vector<std::pair<double *, double *>> spare_bufs;
 };
 
 // exw_ctx is opaque here; its device is set by the caller before any call
@@ -331,7 +351,8 @@ int grow_pool(exw_server s, Pool *p, int new_cap) {
             hipMalloc((void **)&h, sizeof(double) * nc * p->dim) == hipSuccess &&
             hipMalloc((void **)&vert, sizeof(double) * nc * p->nv * p->dim) == hipSuccess &&
             hipMalloc((void **)&scale, sizeof(double) * nc) == hipSuccess &&
-            hipMalloc((void **)&coef, sizeof(double) * nc * p->ncoef()) == hipSuccess;
+            (p->ncoef() == 0 ||
+             hipMalloc((void **)&coef, sizeof(double) * nc * p->ncoef()) == hipSuccess);
   for (int i = 0; i < 3 && ok; ++i)
     ok = hipMalloc((void **)&reg[i], sizeof(double) * nc * p->dpc) == hipSuccess;
   if (!ok) {
@@ -355,8 +376,9 @@ int grow_pool(exw_server s, Pool *p, int new_cap) {
     SHIP(hipMemcpyAsync(vert, p->vert, sizeof(double) * oc * p->nv * p->dim,
                         hipMemcpyDeviceToDevice, st));
     SHIP(hipMemcpyAsync(scale, p->scale, sizeof(double) * oc, hipMemcpyDeviceToDevice, st));
-    SHIP(hipMemcpyAsync(coef, p->coef, sizeof(double) * oc * p->ncoef(),
-                        hipMemcpyDeviceToDevice, st));
+    if (coef)
+      SHIP(hipMemcpyAsync(coef, p->coef, sizeof(double) * oc * p->ncoef(),
+                          hipMemcpyDeviceToDevice, st));
     for (int i = 0; i < 3; ++i)
       SHIP(hipMemcpyAsync(reg[i], p->reg[i], sizeof(double) * oc * p->dpc,
                           hipMemcpyDeviceToDevice, st));
@@ -541,7 +563,7 @@ int admit(exw_server s, Instance *in) {
   SHIP(hipMemcpyAsync(p->scale + start, in->scale.data(), sizeof(double) * n,
                       hipMemcpyHostToDevice, st));
   // the flux coefficients of the new cells (neighbour indices are pool-wide)
-  SHIP(dim_kernels(dim).coef(p->view(0, p->cap), start, n, p->coef, st));
+  if (p->coef) SHIP(dim_kernels(dim).coef(p->view(0, p->cap), start, n, p->coef, st));
   // initial field into the pool's current register (project_initial_field)
   SHIP(dim_kernels(dim).project[r.degree](p->view(start, n), p->d_tab, modes_of(r), 0.0,
                                            p->reg[p->parity] + (size_t)start * p->dpc, st));
@@ -557,8 +579,14 @@ int finalize(exw_server s, Pool *p, Instance *in) {
                      p->scale + in->start, (int64_t)in->n_cells, 0.0);
   SHIP(hipGetLastError());
   const int nb = 1024;  // as exw_l2_errors: same summation order, same result
-  SHIP(hipMalloc((void **)&in->d_partial, sizeof(double) * (3 * nb + 4)));
-  SHIP(hipHostMalloc((void **)&in->h_sum, sizeof(double) * 4, hipHostMallocDefault));
+  if (!s->spare_bufs.empty()) {
+    in->d_partial = s->spare_bufs.back().first;
+    in->h_sum = s->spare_bufs.back().second;
+    s->spare_bufs.pop_back();
+  } else {
+    SHIP(hipMalloc((void **)&in->d_partial, sizeof(double) * (3 * nb + 4)));
+    SHIP(hipHostMalloc((void **)&in->h_sum, sizeof(double) * 4, hipHostMallocDefault));
+  }
   SHIP(dim_kernels(p->dim).l2err[p->k](p->view(in->start, in->n_cells), p->d_etab,
                                         modes_of(in->req), in->t_end,
                                         p->reg[p->parity] + (size_t)in->start * p->dpc,
@@ -602,13 +630,9 @@ int release_idle_pools(exw_server s) {
   return EXW_OK;
 }
 
+// the pool bytes a request occupies once admitted (its precision resolved)
 size_t footprint(const exw_request &r, int n_cells) {
-  const int nv = 1 << r.dim;
-  int nd = 1;
-  for (int d = 0; d < r.dim; ++d) nd *= r.degree + 1;
-  const size_t dpc = (size_t)(r.dim + 1) * nd;
-  return (size_t)n_cells * (sizeof(double) * (3 * dpc + 3 + r.dim + (size_t)nv * r.dim + 1) +
-                            sizeof(int) * 2 * r.dim);
+  return (size_t)n_cells * pool_cell_bytes(r.dim, r.degree, r.precision);
 }
 
 }  // namespace
@@ -651,6 +675,10 @@ int exw_server_destroy(exw_server s) {
     delete in;
   }
   for (auto *in : s->queue) delete in;
+  for (auto &b : s->spare_bufs) {
+    hipFree(b.first);
+    hipHostFree(b.second);
+  }
   delete s;
   return EXW_OK;
 }
@@ -668,7 +696,7 @@ int exw_server_submit(exw_server s, const exw_request *r, int64_t *id) {
   exw_mesh_hypercube(r->dim, 1 << r->n_refinements, 0, EXW_TRANSFORM_NONE, 0.0, 2, &nc, nullptr,
                      nullptr);
   in->n_cells = nc;
-  if (footprint(*r, nc) > s->budget) {
+  if (footprint(in->req, nc) > s->budget) {
     delete in;
     return sfail(EXW_ENOMEM, "request larger than the serving HBM budget");
   }
@@ -770,8 +798,8 @@ int exw_server_poll(exw_server s, exw_result *out, int max_results) {
       r.err_v = std::sqrt(in->h_sum[1]);
       r.mag_p = std::sqrt(in->h_sum[2]);
       r.precision = in->req.precision;
-      hipFree(in->d_partial);
-      hipHostFree(in->h_sum);
+      // the copy into h_sum has completed (in->done): both buffers are free
+      s->spare_bufs.emplace_back(in->d_partial, in->h_sum);
       hipEventDestroy(in->done);
       if (in->pool) --in->pool->finishing;
       delete in;

@@ -64,6 +64,9 @@ struct DimKernels {
   PostErrLaunch post_error[MAX_DEGREE + 1];
   int block[MAX_DEGREE + 1];
   int cells_per_block[MAX_DEGREE + 1];
+  // the FP64 Cartesian sweeps of degree k may run dg_operator_pipe_kernel,
+  // which reads the per-cell flux coefficients (DevMesh::coef)
+  bool pipe[MAX_DEGREE + 1];
 };
 
 // The software-pipelined Cartesian kernel is the default where it applies;
@@ -251,6 +254,7 @@ void fill_dim_kernels(DimKernels &t) {
     t.post_error[K] = launch_post_error<DIM, K>;
     t.block[K] = Shape<DIM, K>::BLOCK;
     t.cells_per_block[K] = Shape<DIM, K>::CPB;
+    t.pipe[K] = PipeShape<DIM, K>::ENABLED && (DIM == 2 || K <= 5);
     fill_dim_kernels<DIM, K - 1>(t);
   }
 }

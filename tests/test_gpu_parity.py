@@ -207,7 +207,7 @@ def test_project_and_norms(X, ctx, dim, k, tr):
 
 
 FIXTURES = ["lsrk45r2_2d_cartesian", "clrk4_3d", "ader_2d_norecon_ref2", "ader_2d_recon_ref2",
-            "aderadconfull_2d"]
+            "ader_2d_recon_ref3", "ader_3d_norecon", "aderadconfull_2d"]
 
 
 @pytest.mark.parametrize("name", FIXTURES)
@@ -236,7 +236,62 @@ def test_reference_fixture_on_gpu(X, ctx, fixtures, name):
     for (st, t, ep, ev, mp, epp), ref in zip(rows, fx["error_lines"]):
         assert "%.2f" % t == ref[0]
         got = ["%.4e" % ep, "%.4e" % epp, "%.4e" % ev, "%.4e" % mp]
-        assert got == ref[1:5], (t, got, ref)
+        if name == "ader_3d_norecon" and ref[0] == "1.00":
+            # k=7 ADER at CFL 0.2 in 3D carries a weakly unstable mode seeded by
+            # round-off after t~0.7 (two CPU builds differing only in FMA
+            # contraction differ by 3e-5 here): the last line to 1e-4 relative
+            # plus half a unit of the printed digit
+            for g, w in zip([ep, epp, ev, mp], ref[1:5]):
+                assert abs(g - float(w)) <= 1e-4 * float(w) + 0.5e-4 * 10 ** np.floor(
+                    np.log10(float(w))), (t, got, ref)
+        else:
+            assert got == ref[1:5], (t, got, ref)
+
+
+def test_config1_membrane_2d_k3(X, ctx):
+    """The 2D membrane workload as stated: unit square, 16x16 Cartesian cells,
+    k=3, rho=c=1, boundary id 2, p0 = sin(pi x)sin(pi y) + sin(2 pi x)sin(3 pi y),
+    u0 = 0, LSRK45R2 at CFL 0.2, T=1, FP64: every output line of exw_run equals
+    the oracle's run (1e-9 relative), and the final errors against the
+    analytical solution are those of a k=3 discretisation at this resolution."""
+    _say("config 1 through exw_run")
+    p = X.Problem()
+    p.dim, p.degree, p.n_intervals, p.n_refinements = 2, 3, 16, 0
+    p.transform, p.transform_factor, p.boundary_id = X.TRANSFORM_NONE, 0.0, 2
+    p.integrator, p.cfl_number, p.max_time_steps = X.LSRK45R2, 0.2, 0
+    p.final_time, p.output_every_time = 1.0, 0.1
+    p.ader_recon, p.precision, p.post_pressure = 0, X.FP64, 0
+    p.modes = X.Modes.make(2, [[1, 1], [2, 3]])
+    steps, rows, _ = X.run(ctx, p)
+    q = O.Problem()
+    q.dim, q.degree, q.n_intervals, q.n_refinements = 2, 3, 16, 0
+    q.transform, q.transform_factor, q.boundary_id = O.TRANSFORM_NONE, 0.0, 2
+    q.integrator, q.cfl_number, q.max_time_steps = O.LSRK45R2, 0.2, 0
+    q.final_time, q.output_every_time = 1.0, 0.1
+    q.ader_recon, q.post_pressure = 0, 0
+    q.exact = O.Exact.make(2, [[1, 1], [2, 3]])
+    osteps, orows, _ = O.run(q)
+    assert steps == osteps and len(rows) == len(orows) == 11
+    dt = 0.2 / 3 ** 1.5 / 16
+    assert steps == int(np.ceil(1.0 / dt - 1e-9))
+    for g, w in zip(rows, orows):
+        assert g[0] == w[0] and g[1] == w[1]
+        for a, b in zip(g[2:5], w[2:5]):
+            assert abs(a - b) <= 1e-9 * abs(b), (g, w)
+    # ||p0||: two orthogonal modes of L2 norm 1/2 each
+    assert abs(rows[0][4] - np.sqrt(0.25 + 0.25)) <= 1e-9
+    # the t=0 error is that of the Gauss-point interpolant, in closed form
+    import ctypes as C
+    from exwave_amd import serving
+    r = serving.make_request(dict(dim=2, degree=3, n_refinements=4, rho=1.0, c=1.0,
+                                  modes=[1, 1], final_time=1.0, cfl=0.2), X.FP64)
+    r.modes = X.Modes.make(2, [[1, 1], [2, 3]])
+    e0, n0, sc, pr = C.c_double(), C.c_double(), C.c_double(), C.c_int()
+    X.check(X.load().exw_request_initial_error(C.byref(r), C.byref(e0), C.byref(n0), C.byref(sc),
+                                               C.byref(pr)))
+    assert abs(rows[0][2] - e0.value) <= 1e-9 * e0.value and abs(rows[0][4] - n0.value) <= 1e-12
+    # k=3 on 16x16 cells (order h^4): the error stays at the interpolation level
+    assert max(r_[2] for r_ in rows) < 1.1 * rows[0][2] and rows[-1][3] < 2e-5
 
 
 def test_error_paths(X, ctx):

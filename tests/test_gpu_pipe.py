@@ -153,3 +153,74 @@ def test_pipelined_and_cartesian_kernels_agree(X, ctx, case):
     op.apply_fused(src, o1=o1c, a1=a1, c=o1, o2=o2c, a2=0.0)
     torch.cuda.synchronize()
     assert np.array_equal(z1, o1c.cpu().numpy()), "no-accumulator paths differ"
+
+
+def test_headline_size_64cube(X, ctx):
+    """The headline workload at its full size: 64^3 Cartesian cells numbered in
+    8^3 tiles, k=4 (131,072,000 DoFs, 131,072 groups of the pipelined kernel
+    remapped over the XCDs, vector offsets up to 1 GB).
+      * one fused LSRK45 stage of the pipelined kernel is bit-identical to the
+        same stage on the one-cell Cartesian kernel (8-byte offset vectors);
+      * five LSRK45 steps at CFL 0.2 from p0 = sin(pi x)sin(pi y)sin(pi z) +
+        sin(2 pi x)sin(3 pi y)sin(pi z): the discrete energy never increases
+        beyond the round-off of its sum (upwind flux, p = 0 walls) and stays
+        within 1e-9 relative of E0;
+      * the L2 errors against the analytical solution: at t = 0 the closed
+        form of the Gauss-point interpolant, afterwards at the same level."""
+    import ctypes as C
+
+    import torch
+    from exwave_amd import serving
+    _say("headline size: 64^3 k=4, tiles of 8^3")
+    n1d, k = 64, 4
+    V, nb = X.mesh_hypercube(3, n1d, 0, X.TRANSFORM_NONE, 0.0, 2)
+    V, nb = X.tile_order(V, nb, n1d, 8)
+    ones = np.ones(V.shape[0])
+    op = X.WaveOperator(ctx, 3, k, V, nb, ones, ones)
+    assert op.is_cartesian and op.n_dofs == 131_072_000
+    N = op.n_dofs
+    modes = X.Modes.make(3, [[1, 1, 1], [2, 3, 1]])
+    u0 = op.new_vector()
+    op.project(modes, 0.0, u0)
+    # one stage (LSRK45 stage 2 coefficients) on both kernels
+    f1, f2 = 0.41, -0.23
+    b = u0 * 0.5
+    y1, b1 = op.new_vector(), op.new_vector()
+    op.rk_stage(u0, b, y1, b1, f1, f2, False)
+    sb = torch.empty(N + 1, dtype=torch.float64, device="cuda")
+    bb = torch.empty(N + 1, dtype=torch.float64, device="cuda")
+    sb[1:].copy_(u0)
+    bb[1:].copy_(b)
+    y2, b2 = op.new_vector(), op.new_vector()
+    op.rk_stage(sb[1:], bb[1:], y2, b2, f1, f2, False)
+    torch.cuda.synchronize()
+    assert torch.equal(y1, y2) and torch.equal(b1, b2), "pipelined and Cartesian stages differ"
+    del sb, bb, y2, b2, b
+    _say("  stage kernels bit-identical")
+    # closed-form t = 0 error of the interpolant
+    r = serving.make_request(dict(dim=3, degree=k, n_refinements=6, rho=1.0, c=1.0,
+                                  modes=[1, 1, 1], final_time=1.0, cfl=0.2), X.FP64)
+    r.modes = modes
+    e0, n0, sc, pr = C.c_double(), C.c_double(), C.c_double(), C.c_int()
+    X.check(X.load().exw_request_initial_error(C.byref(r), C.byref(e0), C.byref(n0), C.byref(sc),
+                                               C.byref(pr)))
+    err0 = op.l2_errors(modes, 0.0, u0)
+    assert abs(err0[0] - e0.value) <= 1e-8 * e0.value and abs(err0[2] - n0.value) <= 1e-12
+    dt = 0.2 / k ** 1.5 * op.min_vertex_distance_over_c
+    del b1
+    vec_n, vec_np = y1, u0
+    work = torch.empty(3 * N, dtype=torch.float64, device="cuda")
+    E = [op.energy(vec_np)]
+    t = 0.0
+    for s in range(5):
+        vec_n, vec_np = vec_np, vec_n
+        op.time_step(X.LSRK45R2, dt, vec_n, vec_np, work)
+        t += dt
+        E.append(op.energy(vec_np))
+    _say("  energies", E)
+    # non-increasing up to the round-off of the energy sum itself
+    assert all(E[i + 1] <= E[i] * (1 + 1e-13) for i in range(5)), E
+    assert E[0] - E[-1] <= 1e-9 * E[0]
+    err = op.l2_errors(modes, t, vec_np)
+    _say("  errors t=0", err0, "t=%g" % t, err)
+    assert err[0] <= 1.05 * err0[0] and err[1] <= 1.05 * err0[0]

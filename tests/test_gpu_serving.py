@@ -28,6 +28,17 @@ REQS = [
          final_time=0.1, cfl=0.2),
     dict(id=5, arrival=0.0, dim=2, degree=6, n_refinements=3, rho=1.1, c=0.9, modes=[3, 2],
          final_time=0.2, cfl=0.2),
+    # second instances in the pools that run the pipelined kernel (3D k=3/4,
+    # 2D k=6): they start at a cell > 0 of their pool and their flux
+    # coefficients are filled at admission with first > 0
+    dict(id=6, arrival=0.0, dim=3, degree=4, n_refinements=2, rho=1.7, c=0.6, modes=[2, 1, 1],
+         final_time=0.15, cfl=0.2),
+    dict(id=7, arrival=0.0, dim=2, degree=6, n_refinements=2, rho=0.9, c=1.6, modes=[1, 2],
+         final_time=0.1, cfl=0.2),
+    dict(id=8, arrival=0.0, dim=3, degree=3, n_refinements=2, rho=1.2, c=1.1, modes=[1, 2, 1],
+         final_time=0.12, cfl=0.2),
+    dict(id=9, arrival=0.0, dim=3, degree=3, n_refinements=1, rho=0.7, c=0.8, modes=[1, 1, 1],
+         final_time=0.2, cfl=0.2),
 ]
 
 
@@ -140,3 +151,50 @@ def test_many_instances_with_turnover(S, ctx):
     solo, _ = srv.run_one_at_a_time(ctx, reqs, X.FP64)
     for q in reqs:
         assert _key(mix[q["id"]]) == _key(solo[q["id"]]), q
+
+
+def test_turnover_pipelined_pools(S, ctx):
+    """Turnover through the pools swept by the pipelined kernel (cell lists,
+    per-cell scales, flux coefficients filled at admission for solves that
+    start anywhere in the pool): 2D k=6, 3D k=3 and 3D k=4 solves of mixed
+    size and length under a budget that forces reuse of freed cells; every
+    result is bit-identical to the solve alone."""
+    X, srv = S
+    rng = np.random.default_rng(11)
+    reqs = []
+    for i in range(24):
+        kind = i % 3
+        dim, k = [(2, 6), (3, 3), (3, 4)][kind]
+        nref = int(rng.integers(2, 4)) if dim == 2 else int(rng.integers(1, 3))
+        modes = [int(rng.integers(1, 4)) for _ in range(dim)]
+        reqs.append(dict(id=200 + i, arrival=0.0, dim=dim, degree=k, n_refinements=nref,
+                         rho=float(rng.uniform(0.5, 2)), c=float(rng.uniform(0.5, 2)),
+                         modes=modes, final_time=float(rng.uniform(0.03, 0.12)), cfl=0.2))
+    mix, _ = srv.run_server(ctx, reqs, lambda _q: X.FP64, hbm_budget_bytes=3 << 20,
+                            realtime=False)
+    solo, _ = srv.run_one_at_a_time(ctx, reqs, X.FP64)
+    for q in reqs:
+        assert _key(mix[q["id"]]) == _key(solo[q["id"]]), q
+
+
+def test_footprint_matches_pool_size(S, ctx):
+    """The size check at submission uses the pool's own per-cell bytes: a
+    request whose pool cells (with the pipelined kernel's flux coefficients)
+    exceed the budget is rejected at submit instead of waiting forever; one
+    that fits is served."""
+    X, srv = S
+    # 2D k=6, 16x16 cells: 3940 B per pool cell with coefficients (1,008,640 B),
+    # 3652 B without (934,912 B)
+    q = dict(id=0, arrival=0.0, dim=2, degree=6, n_refinements=4, rho=1.0, c=1.0, modes=[1, 1],
+             final_time=0.02, cfl=0.2)
+    s = srv.Server(ctx, hbm_budget_bytes=1_000_000)
+    with pytest.raises(X.ExwError):
+        s.submit(srv.make_request(q, X.FP64))
+    s.close()
+    # 2D k=1 (no coefficients: the one-cell kernel) with 64x64 cells fits 2 MiB
+    q2 = dict(id=1, arrival=0.0, dim=2, degree=1, n_refinements=6, rho=1.0, c=1.0, modes=[1, 1],
+              final_time=0.02, cfl=0.2)
+    res, _ = srv.run_server(ctx, [q2], lambda _q: X.FP64, hbm_budget_bytes=2 << 20,
+                            realtime=False, stall_timeout=60.0)
+    solo, _ = srv.run_one_at_a_time(ctx, [q2], X.FP64)
+    assert _key(res[1]) == _key(solo[1])
