@@ -46,6 +46,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 PEAK_HBM_GBS = 8000.0
+PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 vector (the FP64 matrix peak is the same)
 N_CELLS_1D, DEGREE, DIM = 64, 4, 3
 STAGES = 5
 
@@ -64,6 +65,8 @@ def parse():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--tile", type=int, default=8,
                     help="number the cells in tiles of tile^3 (0: lexicographic)")
+    ap.add_argument("--blocks", type=int, default=1,
+                    help="with --tile: blocks^3 blocks numbered one after the other")
     ap.add_argument("--serving-cpu-seconds", type=float, default=10.0)
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--nonaffine", action="store_true",
@@ -71,6 +74,8 @@ def parse():
     ap.add_argument("--nonaffine-steps", type=int, default=40)
     ap.add_argument("--ader", action="store_true",
                     help="the CK kernel and the ADER step on the membrane workloads")
+    ap.add_argument("--ader-case", action="append", default=None,
+                    help="dim,degree,cells_per_dir: only this --ader case (repeatable)")
     ap.add_argument("--refined", action="store_true",
                     help="the two-material box with a locally refined half (hanging faces)")
     ap.add_argument("--refined-time", type=float, default=0.5)
@@ -102,6 +107,53 @@ def stage_kernel(k, fp64):
     if fp64 and 3 <= k <= 5 and os.environ.get("EXW_CART_PIPE", "1")[:1] != "0":
         return "pipe", f"dg_operator_pipe_kernel<3,{k},2>"
     return "cart", f"dg_operator_cart_kernel<3,{k},{'double' if fp64 else 'float'}>"
+
+
+def stream_ceiling(reps=10):
+    """Measured HBM stream rates on this GPU (torch's elementwise kernels on
+    1 GiB FP64 vectors, HIP events on the launch stream, best of `reps`):
+    copy (1 read + 1 write), add (2 reads + 1 write), and the stage's own
+    pattern done in two passes (b += a; c = b + s*a: 2+1 and 2+1).  The best
+    rate is the practical ceiling a bandwidth-bound kernel can reach here."""
+    import torch
+    n = 1 << 27
+    stream = torch.cuda.current_stream()
+    a = torch.rand(n, dtype=torch.float64, device="cuda")
+    b = torch.rand(n, dtype=torch.float64, device="cuda")
+    c = torch.empty_like(a)
+
+    def best(fn, nbytes):
+        fn()
+        rates = []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            fn()
+            e1.record(stream)
+            torch.cuda.synchronize()
+            rates.append(nbytes / (e0.elapsed_time(e1) / 1e3) / 1e9)
+        return round(max(rates), 1)
+    out = {"copy_gbs": best(lambda: c.copy_(a), 16 * n),
+           "add_gbs": best(lambda: torch.add(a, b, alpha=0.5, out=c), 24 * n)}
+    out["stream_ceiling_gbs"] = max(out["copy_gbs"], out["add_gbs"])
+    out["method"] = "torch copy_ / add on 1 GiB FP64 vectors, best of %d, HIP events" % reps
+    del a, b, c
+    torch.cuda.empty_cache()
+    return out
+
+
+def stage_flops_per_cell(dim, k, two_outputs=True):
+    """Floating-point operations of one fused Cartesian stage per cell, counted
+    from the kernel's arithmetic (FMA = 2): per line of the dim*N^(dim-1) lines,
+    the collocation and weak derivatives (2 N^2 FMA), both face traces of own
+    and neighbour (4 N FMA per side), the flux maps (2 x 7 per side), the
+    lifting (2 N FMA per side) and the 1/h scalings (2 N); per node the
+    pressure-share sum (dim - 1) and the epilogue (an FMA per output)."""
+    N = k + 1
+    per_line = 4 * N * N + 2 * (8 * N + 14 + 4 * N) + 2 * N
+    nodes = N ** dim
+    outputs = 2 if two_outputs else 1
+    return dim * N ** (dim - 1) * per_line + nodes * ((dim - 1) + (dim + 1) * 2 * outputs)
 
 
 def copy_calibration():
@@ -196,7 +248,40 @@ def nonaffine_measure(steps):
     print(json.dumps({"nonaffine": out, "oracle_check": chk}), flush=True)
 
 
-def ader_measure():
+def ck_flops_per_cell(dim, k):
+    """Floating-point operations of the CK time integral per Cartesian cell,
+    counted from the algorithm (FMA = 2): every derivative (the first and the
+    max(k-1, 1) steps, on level l with n = k+1-2l points) is two line
+    derivatives of N^2 FMA per line plus 3 dim + 1 node operations and the
+    contribution update (an FMA per field and node); every projection and
+    interpolation pass is one FMA per field, output and input point."""
+    nsteps = max(k - 1, 1)
+
+    def n_of(lev):
+        return max(k + 1 - 2 * lev, 1)
+    total = 0
+    for lev in [0] + [s // 2 for s in range(nsteps)]:
+        n = n_of(lev)
+        total += 2 * 2 * dim * n ** (dim + 1) + n ** dim * (3 * dim + 1) + 2 * (dim + 1) * n ** dim
+
+    def passes(n_in, n_out):  # dim tensor passes n_in -> n_out, outputs per pass
+        f, sizes = 0, [n_in] * dim
+        for d in range(dim):
+            sizes[d] = n_out
+            outs = 1
+            for v in sizes:
+                outs *= v
+            f += 2 * (dim + 1) * outs * n_in
+        return f
+    for s in range(nsteps):
+        if s % 2 == 1 and s < k - 2:
+            n = n_of(s // 2)
+            total += passes(n, n - 2)  # projection
+            total += passes(n - 2, n)  # interpolation back
+    return total
+
+
+def ader_measure(cases=None):
     """The ADER step on the membrane workloads (Cartesian, rho = c = 1,
     Dirichlet walls): the Cauchy-Kowalevski kernel alone (exw_ader_ck: reads the
     state, writes the time-integrated field, 16 B/DoF algorithmic) and the whole
@@ -207,7 +292,7 @@ def ader_measure():
     ctx = X.Context(0)
     stream = torch.cuda.current_stream()
     out = []
-    for dim, k, n1d in ((3, 3, 64), (3, 4, 64), (3, 6, 32), (2, 5, 512)):
+    for dim, k, n1d in cases or ((3, 3, 64), (3, 4, 64), (3, 6, 32), (2, 5, 512)):
         op = X.WaveOperator.hypercube(ctx, dim, k, n1d, 0, X.TRANSFORM_NONE, 0.0, 2)
         modes = X.Modes.make(dim, [[1] * dim])
         u, t = op.new_vector(), op.new_vector()
@@ -236,6 +321,9 @@ def ader_measure():
                 rec[f"{what}_ms_{pname}"] = round(ms, 4)
                 if what == "ck":
                     rec[f"ck_GBps_{pname}"] = round(16 * op.n_dofs / (ms / 1e3) / 1e9, 1)
+                    n_cells = op.n_dofs // ((dim + 1) * (k + 1) ** dim)
+                    rec[f"ck_TFLOPs_{pname}"] = round(
+                        n_cells * ck_flops_per_cell(dim, k) / (ms / 1e3) / 1e12, 2)
                 else:
                     rec[f"step_gdofs_{pname}"] = round(op.n_dofs / (ms / 1e3) / 1e9, 2)
         out.append(rec)
@@ -245,19 +333,20 @@ def ader_measure():
     print(json.dumps({"ader": out}), flush=True)
 
 
-def refined_measure(final_time):
-    """The two-material box with a locally refined half: 32^3 base cells on
-    [0,1]^3, the x < 1/2 half refined once (2:1 hanging faces at x = 1/2),
-    rho = c = 1 for x < 1/2 and rho = 2, c = 4 beyond, k = 3, ADER of order 4
-    with the global step cfl/k^1.5 min(h/c) (cfl 0.1: the reference's ADER in
-    3D grows without bound at cfl 0.2 on this set-up, with or without the
-    refinement and the second material, as the CPU oracle shows; the
-    reference's own 3D ADER regression test at cfl 0.2 already carries a weakly
-    growing mode), first-order absorbing
-    walls, a Gaussian pressure pulse of width 0.05 at (0.25, 0.5, 0.5) at rest,
-    run to final_time with the FP64 and with the FP32 operator.  Reported:
-    GDoF/s per ADER step, E(T)/E(0), the FP32 vs FP64 relative L2 difference of
-    the final states, and the state after 2 steps against the CPU oracle."""
+def refined_measure(final_time, cfl_number=0.15):
+    """The two-material box with a locally refined half as the workload states
+    it: 32^3 base cells on [0,1]^3, the x < 1/2 half refined once (2:1 hanging
+    faces at x = 1/2), rho = c = 1 for x < 1/2 and rho = 2, c = 4 beyond, k = 3,
+    ADER of order 4 with local time stepping: cells clustered by h/c, so the
+    coarse c = 4 half is the fast cluster and the refined half steps twice as
+    long (2 clusters, step ratio 2); p = 0 walls; a Ricker pressure source at
+    (0.25, 0.5, 0.5), f0 = 10, t0 = 1/f0, time-integrated into every update of
+    its cells; from rest to final_time, FP64.  CFL 0.15 instead of 0.2: at 0.2
+    this scheme's energy grows by orders of magnitude within 40 steps (shown on
+    the CPU oracle by the test suite), at 0.15 it decays.  Reported: ms and
+    GDoF/s per global LTS step, the same time span with global ADER at the
+    fastest step (the gain of LTS), energy and pressure norm at T, and two LTS
+    steps on the 8^3-base version of the mesh against the CPU oracle."""
     import numpy as np
     import torch
 
@@ -265,75 +354,102 @@ def refined_measure(final_time):
     import oracle as O
     ctx = X.Context(0)
     stream = torch.cuda.current_stream()
-    n, k, bid = 32, 3, 3
-    ref = np.zeros((n, n, n), dtype=np.int32)
-    ref[:, :, : n // 2] = 1
-    V, nb, H = X.mesh_box_refined(3, n, ref, X.TRANSFORM_NONE, 0.0, bid)
-    centre_x = V[:, :, 0].mean(axis=1)
-    rho = np.where(centre_x < 0.5, 1.0, 2.0)
-    c = np.where(centre_x < 0.5, 1.0, 4.0)
-    op = X.WaveOperator(ctx, 3, k, V, nb, rho, c, hanging=H)
-    # nodal initial field: p = exp(-|x - x0|^2 / 0.05^2), v = 0
-    xg, _ = O.gauss(k + 1)
-    G = np.meshgrid(xg, xg, xg, indexing="ij")
-    xi = np.stack([G[2].ravel(), G[1].ravel(), G[0].ravel()], axis=-1)
-    lo, hi = V[:, 0, :], V[:, -1, :]
-    pts = lo[:, None, :] + (hi - lo)[:, None, :] * xi[None, :, :]
-    r2 = ((pts - np.array([0.25, 0.5, 0.5])) ** 2).sum(axis=-1)
-    u0 = np.zeros((op.n_cells, 4, (k + 1) ** 3))
-    u0[:, 3, :] = np.exp(-r2 / 0.05 ** 2)
-    u0 = u0.ravel()
-    dt0 = 0.1 / k ** 1.5 * op.min_vertex_distance_over_c
-    steps = int(np.ceil(final_time / dt0))
-    dt = final_time / steps
-    rec = {"case": "32^3 base, x<1/2 refined once (147456 cells), rho,c = 1,1 | 2,4, k=3, "
-                   "ADER order 4, absorbing walls, Gaussian pulse at (0.25,0.5,0.5)",
+    k, bid, x0 = 3, 2, [0.25, 0.5, 0.5]
+
+    def build(n):
+        ref = np.zeros((n, n, n), dtype=np.int32)
+        ref[:, :, : n // 2] = 1
+        V, nb, H = X.mesh_box_refined(3, n, ref, X.TRANSFORM_NONE, 0.0, bid)
+        cx_ = V[:, :, 0].mean(axis=1)
+        rho = np.where(cx_ < 0.5, 1.0, 2.0)
+        c = np.where(cx_ < 0.5, 1.0, 4.0)
+        op = X.WaveOperator(ctx, 3, k, V, nb, rho, c, hanging=H)
+        op.set_source(X.Source.make(x0, 1.0, 10.0))
+        return op, ref, rho, c, H
+
+    op, ref, rho, c, H = build(32)
+    cfl = cfl_number / k ** 1.5
+    fastest = cfl * op.min_vertex_distance_over_c
+    lts = X.LocalTimeStepping(op, cfl, fastest, 2, 1, 0.0, False, speed_aware=True)
+    cl, _ = lts.cells()
+    steps = int(np.ceil(final_time / lts.global_dt - 1e-9))
+    rec = {"case": "32^3 base, x<1/2 refined once, rho,c = 1,1 | 2,4, k=3, ADER order 4 with "
+                   "LTS (clusters by h/c), p=0 walls, Ricker source f0=10 at (0.25,0.5,0.5)",
            "n_cells": op.n_cells, "n_hanging_faces": int(len(H)), "n_dofs": op.n_dofs,
-           "is_cartesian": op.is_cartesian, "dt": dt, "steps": steps, "final_time": final_time}
+           "cfl_number": cfl_number, "n_clusters": lts.n_clusters,
+           "cluster_step_ratio": lts.global_dt / fastest,
+           "cells_per_cluster": [int((cl == i).sum()) for i in range(lts.n_clusters)],
+           "fastest_dt": fastest, "global_dt": lts.global_dt, "steps": steps,
+           "final_time": steps * lts.global_dt}
+    u0 = op.new_vector()
+    u0.zero_()
+    a, b = op.new_vector(), op.new_vector()
+    b.copy_(u0)
+    t = 0.0
+    for _ in range(2):  # warm-up (from rest, discarded)
+        t += lts.global_dt
+        lts.step(t, b, a)
+    del lts
+    lts = X.LocalTimeStepping(op, cfl, fastest, 2, 1, 0.0, False, speed_aware=True)
+    b.copy_(u0)
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    ev0.record(stream)
+    t = 0.0
+    for _ in range(steps):
+        t += lts.global_dt
+        lts.step(t, b, a)
+        a, b = b, a
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1)
+    rec["ms_per_global_step"] = round(ms / steps, 4)
+    rec["gdofs_per_global_step"] = round(op.n_dofs * steps / (ms / 1e3) / 1e9, 2)
+    rec["energy_T"] = op.energy(b)
+    rec["p_l2_T"] = float(torch.linalg.vector_norm(b.view(op.n_cells, 4, -1)[:, 3, :]))
+    # the same time span with global ADER at the fastest step (what LTS saves)
     nw = max(op.work_vectors(X.ADER), 1)
-    finals = {}
-    e0 = op.energy(torch.from_numpy(u0).cuda())
-    for pname, prec in (("fp64", X.FP64), ("fp32", X.FP32)):
-        a = op.new_vector()
-        b = torch.from_numpy(u0.copy()).cuda()
-        w = op.new_vector(nw)
-        for _ in range(2):  # warm-up on a copy
-            op.time_step(X.ADER, dt, b.clone(), a, w, precision=prec)
-        ev0 = torch.cuda.Event(enable_timing=True)
-        ev1 = torch.cuda.Event(enable_timing=True)
-        torch.cuda.synchronize()
-        ev0.record(stream)
-        for _ in range(steps):
-            a, b = b, a
-            op.time_step(X.ADER, dt, a, b, w, precision=prec)
-        ev1.record(stream)
-        torch.cuda.synchronize()
-        ms = ev0.elapsed_time(ev1)
-        rec[f"ms_per_step_{pname}"] = round(ms / steps, 4)
-        rec[f"gdofs_per_step_{pname}"] = round(op.n_dofs * steps / (ms / 1e3) / 1e9, 2)
-        rec[f"energy_ratio_{pname}"] = op.energy(b) / e0
-        finals[pname] = b
-        del a, w
-    d = finals["fp32"] - finals["fp64"]
-    rec["fp32_vs_fp64_rel_l2"] = float(torch.linalg.vector_norm(d) /
-                                       torch.linalg.vector_norm(finals["fp64"]))
-    del finals, d
-    # two steps against the oracle on the same mesh
-    m = O.Mesh.box_refined(3, n, ref, O.TRANSFORM_NONE, 0.0, bid)
-    m.set_material(rho, c)
-    u = u0.copy()
-    for _ in range(2):
-        vnp = np.zeros_like(u)
-        m.time_step(k, O.ADER, dt, u, vnp)
-        u = vnp
-    g = torch.from_numpy(u0.copy()).cuda()
-    h = op.new_vector()
     w = op.new_vector(nw)
-    for _ in range(2):
-        op.time_step(X.ADER, dt, g, h, w)
+    g, h = op.new_vector(), op.new_vector()
+    g.zero_()
+    nglob = int(round(steps * lts.global_dt / fastest))
+    torch.cuda.synchronize()
+    ev0.record(stream)
+    tg = 0.0
+    for _ in range(nglob):
+        op.time_step(X.ADER, fastest, g, h, w)
+        op.source_add(tg, tg + fastest, h)
+        tg += fastest
         g, h = h, g
-    gg = g.cpu().numpy()
-    rec["fp64_rel_max_vs_oracle_2_steps"] = float(np.max(np.abs(gg - u)) / np.max(np.abs(u)))
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    msg = ev0.elapsed_time(ev1)
+    rec["global_ader_ms_same_span"] = round(msg, 2)
+    rec["lts_speedup_vs_global_ader"] = round(msg / ms, 3)
+    d = b - g
+    rec["lts_vs_global_ader_rel_l2"] = float(torch.linalg.vector_norm(d) /
+                                             torch.linalg.vector_norm(g))
+    del a, b, g, h, w, d, lts, op
+    # two LTS steps on the 8^3-base mesh against the CPU oracle
+    ops, refs, rhos, cs, _ = build(8)
+    fs = cfl * ops.min_vertex_distance_over_c
+    lts = X.LocalTimeStepping(ops, cfl, fs, 2, 1, 0.0, False, speed_aware=True)
+    m = O.Mesh.box_refined(3, 8, refs, O.TRANSFORM_NONE, 0.0, bid)
+    m.set_material(rhos, cs)
+    ol = O.Lts(m, k, cfl, fs, 2, 1, 0.0, 0, speed_aware=True, source=O.Source.make(x0, 1.0, 10.0))
+    e = O.Exact.make(3, [[1, 1, 1]])
+    u = m.project(k, e, 0.0)
+    s_ = torch.from_numpy(u.copy()).cuda()
+    d_ = torch.zeros_like(s_)
+    t = 0.0
+    for _ in range(2):
+        t += lts.global_dt
+        lts.step(t, s_, d_)
+        u = ol.step(t, u)
+        s_, d_ = d_, s_
+    rec["fp64_rel_max_vs_oracle_2_lts_steps_8cube_base"] = float(
+        np.max(np.abs(s_.cpu().numpy() - u)) / np.max(np.abs(u)))
     print(json.dumps({"locally_refined": rec}), flush=True)
 
 
@@ -419,6 +535,23 @@ def serving_measure(ctx, cpu_seconds, want_cpu):
     return out
 
 
+def host_cpu():
+    """CPU model (/proc/cpuinfo) and the logical CPUs this process may use."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count()
+    return {"cpu_model": model, "host_logical_cpus": os.cpu_count(), "usable_cpus": usable}
+
+
 def cpu_baseline(n1d, k, target_s):
     """Time the CPU oracle's apply + RKVectorUpdater on the same mesh and data
     (stage by stage, as the reference executes it) until ~target_s elapsed."""
@@ -458,7 +591,7 @@ def cpu_baseline(n1d, k, target_s):
         if el >= target_s and stages >= 1:
             break
     return {"value": round(N * stages / el / 1e9, 4), "unit": "GDoF/s per RK stage",
-            "cores": threads, "kind": "port",
+            "cores": threads, "kind": "port", **host_cpu(),
             "sample": f"{stages} LSRK45 stages (oracle apply + RKVectorUpdater, FP64) of the "
                       f"{n1d}^3 k={k} workload, {el:.1f} s, OpenMP {threads} threads; CPU "
                       "restatement, not deal.II"}
@@ -476,7 +609,8 @@ def main():
         refined_measure(args.refined_time)
         return
     if args.ader:
-        ader_measure()
+        ader_measure([tuple(int(v) for v in c.split(",")) for c in args.ader_case]
+                     if args.ader_case else None)
         return
     if args.gpus != 1:
         print(f"bench.py: one GPU only (got --gpus {args.gpus}); the path runs on a single "
@@ -492,7 +626,7 @@ def main():
     n1d, k = args.n, args.degree
     if args.tile:
         V, nb = X.mesh_hypercube(DIM, n1d, 0, X.TRANSFORM_NONE, 0.0, 2)
-        V, nb = X.tile_order(V, nb, n1d, args.tile)
+        V, nb = X.tile_order(V, nb, n1d, args.tile, args.blocks)
         ones = np.ones(V.shape[0])
         op = X.WaveOperator(ctx, DIM, k, V, nb, ones, ones)
     else:
@@ -550,7 +684,9 @@ def main():
         "data": "synthetic (analytic membrane modes (1,1,1)+(2,3,1), projected)",
         "config": {"workload": f"3D membrane, unit cube, {n1d}^3 Cartesian hexes, k={k}, "
                                f"{N} DoFs, LSRK45R2 at CFL 0.2, rho=c=1, p=0 boundary",
-                   "cell_order": f"tiles of {args.tile}^3" if args.tile else "lexicographic",
+                   "cell_order": (f"tiles of {args.tile}^3" +
+                                  (f" in {args.blocks}^3 blocks" if args.blocks > 1 else "")
+                                  if args.tile else "lexicographic"),
                    "n_dofs": N, "n_cells": n_cells, "degree": k, "integrator": "LSRK45R2",
                    "dt": dt, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
@@ -558,11 +694,26 @@ def main():
                      "traffic": None,
                      "kernel": kernel_name,
                      "algorithmic_bytes_per_launch": int(bytes_per_launch),
-                     "avg_launch_ms": round(stage_ms, 5)},
+                     "avg_launch_ms": round(stage_ms, 5),
+                     # 4 of 5 LSRK45 stages have two outputs
+                     "flops_per_launch": int(n_cells * (4 * stage_flops_per_cell(DIM, k, True) +
+                                                        stage_flops_per_cell(DIM, k, False)) / 5),
+                     "fp64_vector_peak_tflops": PEAK_FP64_TFLOPS},
         "wall_s": round(wall, 4),
         "l2_error_after": {"t": t_end, "p": err[0], "v": err[1], "mag_p": err[2]},
         "energy_ratio": E1 / E0,
     }
+    rl = out["roofline"]
+    rl["achieved_tflops"] = round(rl["flops_per_launch"] / (stage_ms / 1e3) / 1e12, 2)
+    rl["flop_frac"] = round(rl["achieved_tflops"] / PEAK_FP64_TFLOPS, 4)
+    try:
+        sc = stream_ceiling()
+        rl["stream_ceiling_gbs"] = sc["stream_ceiling_gbs"]
+        rl["stream"] = sc
+        rl["frac_of_stream_ceiling"] = round(achieved / sc["stream_ceiling_gbs"], 4)
+    except RuntimeError as ex:  # out of memory next to the workload: report none
+        rl["stream_ceiling_gbs"] = None
+        rl["stream_error"] = str(ex)[:200]
     key = f"{family}_3d_k{k}_n{n1d}_t{args.tile}_{'fp64' if prec == X.FP64 else 'fp32'}"
     traffic, source = measured_traffic(key)
     out["roofline"]["traffic"] = traffic

@@ -19,7 +19,8 @@ from .capi import (ADER, ADER_ADCONFULL, ADER_LTS, CLRK4, EXPL_EULER, FP32, FP64
                    PRECISION_AUTO, INTEGRATORS, LSRK33R2, LSRK45R2, LSRK45R3, SSPRK,
                    LSRK59R2, TRANSFORM_AXIS_2PI, TRANSFORM_NONE, TRANSFORM_PRODUCT_2PI,
                    TRANSFORM_REFERENCE_SINE, NC_FACE, Epilogue, ExwError, Modes, Problem, check,
-                   load, mesh_box_refined, mesh_hypercube, tile_order, write_vtu)
+                   load, mesh_box_refined, mesh_hypercube, tile_order, write_vtu, Source,
+                   LTS_SPEED_AWARE)
 
 
 def _ptr(t):
@@ -139,6 +140,14 @@ class WaveOperator:
     def work_vectors(integrator):
         return int(load().exw_time_step_work_vectors(integrator))
 
+    def set_source(self, src):
+        """exw_op_set_source: a Source, or None to remove it."""
+        self._src = src
+        check(load().exw_op_set_source(self.h, C.byref(src) if src is not None else None))
+
+    def source_add(self, t1, t2, u):
+        check(load().exw_source_add(self.h, t1, t2, _ptr(u)))
+
     def project(self, modes: Modes, t, dst):
         check(load().exw_project(self.h, C.byref(modes), t, _ptr(dst)))
 
@@ -237,10 +246,11 @@ class LocalTimeStepping:
     """ADER with local time stepping on a WaveOperator (exw_lts_*)."""
 
     def __init__(self, op: WaveOperator, cfl, fastest_dt, max_clusters, max_diff, time0=0.0,
-                 recon=True):
+                 recon=True, speed_aware=False):
         h = C.c_void_p()
-        check(load().exw_lts_create(op.h, cfl, fastest_dt, max_clusters, max_diff, time0,
-                                    int(recon), C.byref(h)))
+        check(load().exw_lts_create2(op.h, cfl, fastest_dt, max_clusters, max_diff, time0,
+                                     int(recon), LTS_SPEED_AWARE if speed_aware else 0,
+                                     C.byref(h)))
         self.h, self.op = h, op
         nc, diff, nu, gdt = C.c_int(), C.c_int(), C.c_int(), C.c_double()
         check(load().exw_lts_info(h, C.byref(nc), C.byref(diff), C.byref(nu), C.byref(gdt)))

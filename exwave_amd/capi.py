@@ -56,6 +56,24 @@ class Epilogue(C.Structure):
                 ("cell_scale", C.c_void_p)]
 
 
+class Source(C.Structure):
+    """exw_source: Ricker point source of pressure at x (t0 defaults to 1/f0)."""
+    _fields_ = [("x", C.c_double * 3), ("amplitude", C.c_double), ("f0", C.c_double),
+                ("t0", C.c_double)]
+
+    @staticmethod
+    def make(x, amplitude=1.0, f0=10.0, t0=None):
+        s = Source()
+        for i, v in enumerate(x):
+            s.x[i] = float(v)
+        s.amplitude, s.f0 = float(amplitude), float(f0)
+        s.t0 = 1.0 / f0 if t0 is None else float(t0)
+        return s
+
+
+LTS_SPEED_AWARE = 1
+
+
 class Request(C.Structure):
     _fields_ = [("dim", C.c_int), ("degree", C.c_int), ("n_refinements", C.c_int),
                 ("rho", C.c_double), ("c", C.c_double), ("modes", Modes),
@@ -118,6 +136,7 @@ SIGNATURES = [
     ("exw_adapt_box_transfer", _I, [_VP, _I, _I, _I, _IP, _IP, _VP, _VP]),
     ("exw_write_vtu", _I, [_I, _I, _I, _DP, _DP, _DP, _DP, _DP, _DP, _DP, C.c_char_p]),
     ("exw_op_create_nc", _I, [_VP, _I, _I, _I, _DP, _IP, _I, _IP, _DP, _DP, C.POINTER(_VP)]),
+    ("exw_mesh_check", _I, [_I, _I, _DP, _IP, _I, _IP]),
     ("exw_mesh_box_refined", _I, [_I, _I, _I, _D, _I, _IP, _IP, _DP, _IP, _IP, _IP]),
     ("exw_post_pressure", _I, [_VP, _VP, _VP, _VP, _VP]),
     ("exw_post_error", _I, [_VP, C.POINTER(Modes), _D, _VP, _DP]),
@@ -133,6 +152,9 @@ SIGNATURES = [
     ("exw_request_initial_error", _I, [C.POINTER(Request), _DP, _DP, _DP, _IP]),
     ("exw_lts_categorize", _I, [_I, _I, _DP, _IP, _D, _I, _I, _IP, _IP, _IP, _IP, _IP]),
     ("exw_lts_create", _I, [_VP, _D, _D, _I, _I, _D, _I, C.POINTER(_VP)]),
+    ("exw_lts_create2", _I, [_VP, _D, _D, _I, _I, _D, _I, _I, C.POINTER(_VP)]),
+    ("exw_op_set_source", _I, [_VP, C.POINTER(Source)]),
+    ("exw_source_add", _I, [_VP, _D, _D, _VP]),
     ("exw_lts_destroy", _I, [_VP]),
     ("exw_lts_info", _I, [_VP, _IP, _IP, _IP, _DP]),
     ("exw_lts_cells", _I, [_VP, _IP, _IP]),
@@ -222,18 +244,27 @@ def mesh_hypercube(dim, n_intervals, n_refinements=0, transform=TRANSFORM_NONE, 
     return V, nb
 
 
-def tile_order(V, nb, n1d, tile):
-    """Renumber the cells of the lexicographic n1d^3 mesh of mesh_hypercube tile
-    by tile (tiles of tile^3 cells in lexicographic order, cells lexicographic
-    inside a tile), as MatrixFree renumbers cells for locality: the z
-    neighbours of a cell are then tile^2 cells away instead of n1d^2.  Returns
-    (vertices, neighbor) in the new order."""
+def tile_order(V, nb, n1d, tile, blocks=1):
+    """Renumber the cells of the lexicographic n1d^3 mesh of mesh_hypercube for
+    locality, as MatrixFree renumbers cells: tiles of tile^3 cells (tiles in
+    lexicographic order, cells lexicographic inside a tile), so that the z
+    neighbours of a cell are tile^2 cells away instead of n1d^2.  With
+    blocks = b > 1 the mesh is first split into b^3 blocks of (n1d/b)^3 cells
+    numbered one block after the other (tiles inside each): with b^3 equal to
+    the GPU's 8 XCDs, the contiguous cell range each XCD sweeps is one compact
+    block, whose neighbours mostly sit in the same L2.  Returns (vertices,
+    neighbor) in the new order."""
     import numpy as np
-    nt = (n1d + tile - 1) // tile
     idx = np.arange(n1d ** 3)
     x, y, z = idx % n1d, (idx // n1d) % n1d, idx // (n1d * n1d)
-    tx, ty, tz = x // tile, y // tile, z // tile
-    key = (((tz * nt + ty) * nt + tx) * tile + z % tile) * tile * tile + (y % tile) * tile + x % tile
+    nb1 = n1d // blocks if blocks > 1 and n1d % blocks == 0 else n1d
+    nbk = n1d // nb1
+    bx, by, bz = x // nb1, y // nb1, z // nb1
+    lx, ly, lz = x % nb1, y % nb1, z % nb1
+    nt = (nb1 + tile - 1) // tile
+    tx, ty, tz = lx // tile, ly // tile, lz // tile
+    key = ((((bz * nbk + by) * nbk + bx) * nt + tz) * nt + ty) * nt + tx
+    key = ((key * tile + lz % tile) * tile + ly % tile) * tile + lx % tile
     order = np.argsort(key, kind="stable")  # new -> old
     new_of_old = np.empty_like(order)
     new_of_old[order] = np.arange(order.size)

@@ -9,8 +9,13 @@
 #include "../../include/exwave_amd.h"
 #include "launchers.hpp"
 
+struct exw_op_s;
+
 namespace exw {
 void set_last_error(const std::string &msg);
+// u += (int_{t1}^{t2} r dt) b on the operator's source cells s with bit s of
+// sel set (asynchronous on the operator's stream)
+int source_add_selected(exw_op_s *op, double t1, double t2, unsigned sel, double *u);
 }
 
 struct exw_ctx_s {
@@ -26,6 +31,8 @@ struct exw_op_s {
   int *d_neighbor = nullptr;
   double *d_rho = nullptr, *d_speed = nullptr, *d_h = nullptr, *d_vertices = nullptr;
   double *d_coef = nullptr;  // Cartesian meshes: DevMesh::coef
+  int *d_coef_class = nullptr;  // DevMesh::coef_class (NULL: one row per cell)
+  int n_coef_rows = 0;
   double *d_tab = nullptr, *d_errtab = nullptr, *d_cktab = nullptr, *d_posttab = nullptr;
   double *d_partial = nullptr, *d_sum = nullptr;
   // non-conforming faces: the cells that touch them (NcArgs) and the table
@@ -35,9 +42,18 @@ struct exw_op_s {
   int n_partial = 0;
   exw::CkTab ck{};
   std::vector<double> host_tab;
-  // host copies of the connectivity and geometry (local-time-stepping set-up)
-  std::vector<double> h_vertices;
-  std::vector<int> h_neighbor;
+  // host copies of the connectivity, geometry and wave speeds (local-time-
+  // stepping set-up); with hanging faces also the per-cell face kinds and
+  // partners of NcArgs ([n_cells][2 dim] and [n_cells][2 dim][4])
+  std::vector<double> h_vertices, h_speed;
+  std::vector<int> h_neighbor, h_nc_kind, h_nc_other;
+  // point source (exw_op_set_source): the cells holding the point and per
+  // cell the pressure increment per unit time integral (M^-1 delta)
+  int n_src = 0;
+  exw_source src{};
+  std::vector<int> h_src_cells;
+  int *d_src_cells = nullptr;
+  double *d_src_b = nullptr;
   exw::NcArgs nc_args() const { return exw::NcArgs{n_nc, d_nc_cells, d_nc_kind, d_nc_other}; }
   exw::DevMesh view() const {
     exw::DevMesh m;
@@ -48,6 +64,7 @@ struct exw_op_s {
     m.h = d_h;
     m.vertices = d_vertices;
     m.coef = d_coef;
+    m.coef_class = d_coef_class;
     return m;
   }
   int64_t n_post() const {  // post-processed pressure: (k+2)^dim values per cell

@@ -51,6 +51,9 @@ struct DevMesh {
   // Cartesian: per cell the face flux maps [2*dim][8] and the cell-term
   // factors [dim][2] (cart_coef_kernel); NULL where not set up
   const double *__restrict__ coef = nullptr;
+  // optional: coef holds a table of distinct rows and cell c uses row
+  // coef_class[c] (box meshes share a few dozen rows)
+  const int *__restrict__ coef_class = nullptr;
 };
 
 // o1 = b + a1*s*k, o2 = (o1 or c) + a2*s*k, o3 = (o1 or c) + a3*s*k (if o3 is

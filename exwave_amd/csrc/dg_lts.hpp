@@ -30,7 +30,7 @@ enum { LTS_ACC = 0, LTS_UPDATE = 1, LTS_APPLY = 2 };
 
 struct LtsArgs {
   const int *cells;             // entry -> cell
-  const unsigned char *mask;    // entry -> faces to evaluate (bit f); NULL: all
+  const unsigned *mask;         // entry -> faces to evaluate (bit 4 f); NULL: all
   const unsigned char *target;  // LTS_ACC: entry -> 0 (acc0) or 1 (acc1)
   int n;                        // entries
   int mode;
@@ -64,7 +64,9 @@ __global__ void __launch_bounds__((Shape<DIM, K>::BLOCK))
   const int e = blockIdx.x * CPB + lc;
   const bool active = lane_ok && e < a.n;
   const int cell = active ? a.cells[e] : 0;
-  const int mask = active ? (a.mask ? int(a.mask[e]) : (1 << NFACE) - 1) : 0;
+  // bit 4f selects face f (the bits 4f+1..4f+3 address the other subfaces of
+  // a 2:1 face, which nc_face_kernel evaluates: such faces add nothing here)
+  const unsigned mask = active ? (a.mask ? a.mask[e] : 0xFFFFFFFFu) : 0u;
   const bool do_vol = a.mode != LTS_ACC;
   const bool do_face = a.mode != LTS_UPDATE;
   const size_t cbase = (size_t)cell * NF * ND;
@@ -142,7 +144,7 @@ __global__ void __launch_bounds__((Shape<DIM, K>::BLOCK))
   if (lane_ok && do_face) {
     for (int fp = q; fp < NFACE * NT; fp += ND) {
       const int f = fp / NT, t = fp - f * NT, d = f >> 1, s = f & 1;
-      if (!((mask >> f) & 1)) {
+      if (!((mask >> (4 * f)) & 1u) || s_nbr[lc][f] == -NC_BID) {
 #pragma unroll
         for (int k = 0; k < NF; ++k) s_F[lc][f][k][t] = 0.0;
         continue;

@@ -209,6 +209,18 @@ __global__ void __launch_bounds__((Shape<DIM, K>::BLOCK))
   }
 }
 
+// u += I b on the pressure of the source cells selected by `sel` (bit s:
+// source cell s): the time-integrated point source, one lane per node
+static __global__ void __launch_bounds__(256)
+    source_add_kernel(double *__restrict__ u, const int *__restrict__ cells,
+                      const double *__restrict__ b, int n_src, int nd, int dim, unsigned sel,
+                      double I) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = i / nd, q = i - s * nd;
+  if (s >= n_src || !((sel >> s) & 1u)) return;
+  u[((size_t)cells[s] * (dim + 1) + dim) * nd + q] += I * b[(size_t)s * nd + q];
+}
+
 // fixed-order final sum of `n` partial vectors of width `w`
 static __global__ void __launch_bounds__(256)
     sum_partials_kernel(const double *__restrict__ partial, int n, int w, double *__restrict__ out) {

@@ -25,6 +25,9 @@ struct NcArgs {
   const int *kind;    // [n][2 dim]: 0 conforming, 1 fine side, 2 coarse side
   const int *other;   // [n][2 dim][4]: fine side {coarse cell, subface position},
                       // coarse side: the fine cells by subface position
+  // optional [n]: bit 4f+b selects subface b of face f (the fine side's one
+  // subface is bit 4f); NULL: every subface (local time stepping masks)
+  const unsigned *mask = nullptr;
 };
 
 // table: X[N], W[N], L[2][N] (l_i(0), l_i(1)), H[2][N][N] (H_b[t][j] = l_j((x_t + b)/2))
@@ -122,8 +125,15 @@ __global__ void __launch_bounds__((NcShape<DIM, K>::BLOCK))
     return s;
   };
 
-  const int *kind = nc.kind + (size_t)blockIdx.x * NFACE;
   const int *other = nc.other + (size_t)blockIdx.x * NFACE * 4;
+  const unsigned emask = nc.mask ? nc.mask[blockIdx.x] : 0xFFFFFFFFu;
+  int kind[NFACE];  // face kinds, 0 where the mask selects no subface
+#pragma unroll
+  for (int f = 0; f < NFACE; ++f) {
+    const int kd = nc.kind[(size_t)blockIdx.x * NFACE + f];
+    const unsigned sub = (emask >> (4 * f)) & 0xFu;
+    kind[f] = (kd == 2 && sub) || (kd == 1 && (sub & 1u)) ? kd : 0;
+  }
   for (int f = 0; f < NFACE; ++f) {
     const int kd = kind[f];
     if (kd == 0) continue;  // uniform over the group
@@ -188,7 +198,11 @@ __global__ void __launch_bounds__((NcShape<DIM, K>::BLOCK))
         }
       }
       __syncthreads();
-      if (tid < NSUB * NT) {
+      if (tid < NSUB * NT && !((emask >> (4 * f + tid / NT)) & 1u)) {
+        const int b = tid / NT, t = tid - b * NT;  // subface not selected
+#pragma unroll
+        for (int k = 0; k < NF; ++k) s_Fb[b][k][t] = 0.0;
+      } else if (tid < NSUB * NT) {
         const int b = tid / NT, t = tid - b * NT;
         const int nb = other[f * 4 + b];
         const double *un = src + (size_t)nb * NF * ND;

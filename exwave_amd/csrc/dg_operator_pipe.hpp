@@ -376,6 +376,8 @@ __global__ void __launch_bounds__((PipeShape<DIM, K>::BLOCK)) __attribute__((amd
   // the array's base (scalar) + a 32-bit byte offset (vectors below 4 GiB)
   auto issue = [&](int c, const int (&nb)[NFACE], int buf) __attribute__((always_inline)) {
     const uint32_t lbase = pipe::lds_addr(&s_buf[buf][0]);
+    // the cell's row of flux coefficients (its class's row in a shared table)
+    const int cls = mesh.coef_class ? pipe::sld(mesh.coef_class, c) : c;
     const uint32_t own = uint32_t(c) * uint32_t(P::OWN_BYTES);
     static_for<WV>([&](auto W_) __attribute__((always_inline)) {
       constexpr int W = decltype(W_)::value;
@@ -390,7 +392,7 @@ __global__ void __launch_bounds__((PipeShape<DIM, K>::BLOCK)) __attribute__((amd
           const int f = op.fld >> 1, cn = nb[f] >= 0 ? nb[f] : c;
           g = uint32_t(cn) * uint32_t(P::OWN_BYTES);
         } else if constexpr (op.src == DMA_COEF) {
-          g = uint32_t(c) * uint32_t(P::NCOEF * 8);
+          g = uint32_t(cls) * uint32_t(P::NCOEF * 8);
         } else {
           g = own;
         }

@@ -7,9 +7,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -43,6 +45,7 @@ namespace exw {
 // error message of the calling thread, shared by every translation unit
 void set_last_error(const std::string &msg) { g_err = msg; }
 }  // namespace exw
+
 
 extern "C" {
 
@@ -112,6 +115,7 @@ static void free_op(exw_op op) {
   hipFree(op->d_h);
   hipFree(op->d_vertices);
   hipFree(op->d_coef);
+  hipFree(op->d_coef_class);
   hipFree(op->d_tab);
   hipFree(op->d_errtab);
   hipFree(op->d_cktab);
@@ -122,6 +126,8 @@ static void free_op(exw_op op) {
   hipFree(op->d_nctab);
   hipFree(op->d_partial);
   hipFree(op->d_sum);
+  hipFree(op->d_src_cells);
+  hipFree(op->d_src_b);
   delete op;
 }
 
@@ -235,6 +241,49 @@ static int op_create(exw_ctx ctx, int dim, int degree, int n_cells, const double
                      const int *neighbor, int n_hanging, const int *hanging, const double *rho,
                      const double *speed, exw_op *out);
 
+int exw_mesh_check(int dim, int n_cells, const double *vertices, const int *neighbor,
+                   int n_hanging, const int *hanging) {
+  if ((dim != 2 && dim != 3) || n_cells < 1 || !vertices || !neighbor ||
+      n_hanging < 0 || (n_hanging > 0 && !hanging))
+    return fail(EXW_EINVAL, "exw_mesh_check: invalid argument");
+  const int nv = 1 << dim, nface = 2 * dim;
+  for (int c = 0; c < n_cells; ++c) {
+    const double *V = vertices + (size_t)c * nv * dim;
+    double h = 0.0;
+    for (int d = 0; d < dim; ++d) h = std::max(h, std::fabs(V[(nv - 1) * dim + d] - V[d]));
+    for (int f = 0; f < nface; ++f) {
+      const int nb = neighbor[(size_t)c * nface + f];
+      if (nb == EXW_NC_FACE && n_hanging > 0) continue;  // checked with its record
+      if (nb >= n_cells || (nb < 0 && (nb < -3 || nb > -1)))
+        return fail(EXW_EINVAL, "set your boundary ids correctly: 1 - soft wall, 2 - hard wall, "
+                                "3 - first order ABC (or neighbour index out of range)");
+      if (nb < 0) continue;
+      if (neighbor[(size_t)nb * nface + (f ^ 1)] != c)
+        return fail(EXW_EINVAL, "cell " + std::to_string(c) + " face " + std::to_string(f) +
+                                    ": the neighbour does not see this cell across face " +
+                                    std::to_string(f ^ 1) +
+                                    " (faces pair as f and f^1 in standard orientation; other "
+                                    "pairings are not supported)");
+      // standard orientation: corner v of face f (bit d of v = s) is corner
+      // v with bit d flipped of the neighbour's face f^1, at the same point
+      const int d = f >> 1, s = f & 1;
+      const double *W = vertices + (size_t)nb * nv * dim;
+      for (int v = 0; v < nv; ++v) {
+        if (((v >> d) & 1) != s) continue;
+        const int w = v ^ (1 << d);
+        for (int i = 0; i < dim; ++i)
+          if (std::fabs(V[v * dim + i] - W[w * dim + i]) > 1e-10 * h)
+            return fail(EXW_EINVAL,
+                        "cell " + std::to_string(c) + " face " + std::to_string(f) +
+                            ": face not in standard orientation (its corners are not the "
+                            "neighbour's face corners in the same order; rotated or flipped "
+                            "faces are not supported)");
+      }
+    }
+  }
+  return EXW_OK;
+}
+
 int exw_op_create(exw_ctx ctx, int dim, int degree, int n_cells, const double *vertices,
                   const int *neighbor, const double *rho, const double *speed, exw_op *out) {
   return op_create(ctx, dim, degree, n_cells, vertices, neighbor, 0, nullptr, rho, speed, out);
@@ -260,14 +309,9 @@ static int op_create(exw_ctx ctx, int dim, int degree, int n_cells, const double
                                 " not compiled (1.." + std::to_string(MAX_DEGREE) + ")");
   if (n_cells < 1) return fail(EXW_EINVAL, "exw_op_create: n_cells < 1");
   const int nv = 1 << dim, nface = 2 * dim, N = degree + 1;
-  // validate connectivity and materials (source/wave_equation_operations.cc:615-616)
-  for (int64_t i = 0; i < (int64_t)n_cells * nface; ++i) {
-    const int nb = neighbor[i];
-    if (nb == EXW_NC_FACE && n_hanging > 0) continue;
-    if (nb >= n_cells || (nb < 0 && (nb < -3 || nb > -1)))
-      return fail(EXW_EINVAL, "set your boundary ids correctly: 1 - soft wall, 2 - hard wall, "
-                              "3 - first order ABC (or neighbour index out of range)");
-  }
+  // validate connectivity, face orientation and materials
+  // (source/wave_equation_operations.cc:615-616)
+  if (int rc = exw_mesh_check(dim, n_cells, vertices, neighbor, n_hanging, hanging)) return rc;
   std::vector<int> nc_kind, nc_other;
   if (n_hanging > 0) {
     if (int rc = check_hanging(dim, n_cells, vertices, neighbor, n_hanging, hanging, nc_kind,
@@ -330,6 +374,37 @@ static int op_create(exw_ctx ctx, int dim, int degree, int n_cells, const double
       free_op(op);
       return fail(EXW_EDEVICE, "exw_op_create: flux coefficient set-up failed");
     }
+    // rows shared by many cells (same extents, materials on both sides and
+    // boundary pattern) are kept once: the sweeps then read a small table that
+    // stays in L2 instead of a per-cell array from HBM (the same values)
+    const int nco = 2 * dim * 8 + 2 * dim;
+    std::vector<double> rows(n);
+    EXW_HIP(hipMemcpy(rows.data(), op->d_coef, n * sizeof(double), hipMemcpyDeviceToHost));
+    std::map<std::vector<uint64_t>, int> seen;
+    std::vector<int> cls(n_cells);
+    std::vector<double> table;
+    for (int c = 0; c < n_cells && (int)seen.size() <= n_cells / 8; ++c) {
+      std::vector<uint64_t> key(nco);
+      std::memcpy(key.data(), rows.data() + (size_t)c * nco, nco * sizeof(double));
+      auto it = seen.find(key);
+      if (it == seen.end()) {
+        it = seen.emplace(key, (int)seen.size()).first;
+        table.insert(table.end(), rows.begin() + (size_t)c * nco, rows.begin() + (size_t)(c + 1) * nco);
+      }
+      cls[c] = it->second;
+    }
+    if ((int)seen.size() <= n_cells / 8) {
+      double *d_table = nullptr;
+      if ((rc = upload(op, &d_table, table.data(), table.size())) ||
+          (rc = upload(op, &op->d_coef_class, cls.data(), cls.size()))) {
+        hipFree(d_table);
+        free_op(op);
+        return rc;
+      }
+      hipFree(op->d_coef);
+      op->d_coef = d_table;
+      op->n_coef_rows = (int)seen.size();
+    }
   }
   if (n_hanging > 0) {
     // the cells that touch a hanging face, in cell order, with their face kinds
@@ -368,6 +443,9 @@ static int op_create(exw_ctx ctx, int dim, int degree, int n_cells, const double
   op->host_tab = tab;
   op->h_vertices.assign(vertices, vertices + (size_t)n_cells * nv * dim);
   op->h_neighbor.assign(neighbor, neighbor + (size_t)n_cells * nface);
+  op->h_speed.assign(speed, speed + n_cells);
+  op->h_nc_kind = std::move(nc_kind);
+  op->h_nc_other = std::move(nc_other);
   if ((rc = upload(op, &op->d_tab, tab.data(), tab.size())) ||
       (rc = upload(op, &op->d_errtab, etab.data(), etab.size())) ||
       (rc = upload(op, &op->d_cktab, ckt.data(), ckt.size()))) {
@@ -816,6 +894,166 @@ int exw_l2_errors(exw_op op, const exw_modes *modes, double t, const double *u,
   EXW_HIP(hipStreamSynchronize(op->ctx->stream));
   for (int k = 0; k < 3; ++k) out_host[k] = std::sqrt(s[k]);
   return EXW_OK;
+}
+
+}  // extern "C"
+
+namespace {
+// host multilinear map of cell corners V at xi: point x and Jacobian inverse
+// (Ji[e*dim+i] = dxi_e/dx_i); returns det J
+double host_map(int dim, const double *V, const double *xi, double *x, double *Ji) {
+  const int nv = 1 << dim;
+  double J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int i = 0; i < dim; ++i) x[i] = 0.0;
+  for (int v = 0; v < nv; ++v) {
+    double w = 1.0;
+    for (int d = 0; d < dim; ++d) w *= ((v >> d) & 1) ? xi[d] : 1.0 - xi[d];
+    for (int i = 0; i < dim; ++i) x[i] += w * V[v * dim + i];
+    for (int e = 0; e < dim; ++e) {
+      double g = ((v >> e) & 1) ? 1.0 : -1.0;
+      for (int d = 0; d < dim; ++d)
+        if (d != e) g *= ((v >> d) & 1) ? xi[d] : 1.0 - xi[d];
+      for (int i = 0; i < dim; ++i) J[i * dim + e] += g * V[v * dim + i];
+    }
+  }
+  double det;
+  if (dim == 2) {
+    det = J[0] * J[3] - J[1] * J[2];
+    Ji[0] = J[3] / det;
+    Ji[1] = -J[1] / det;
+    Ji[2] = -J[2] / det;
+    Ji[3] = J[0] / det;
+  } else {
+    const double A = J[4] * J[8] - J[5] * J[7], B = J[5] * J[6] - J[3] * J[8],
+                 C = J[3] * J[7] - J[4] * J[6];
+    det = J[0] * A + J[1] * B + J[2] * C;
+    Ji[0] = A / det;
+    Ji[3] = B / det;
+    Ji[6] = C / det;
+    Ji[1] = (J[2] * J[7] - J[1] * J[8]) / det;
+    Ji[4] = (J[0] * J[8] - J[2] * J[6]) / det;
+    Ji[7] = (J[1] * J[6] - J[0] * J[7]) / det;
+    Ji[2] = (J[1] * J[5] - J[2] * J[4]) / det;
+    Ji[5] = (J[2] * J[3] - J[0] * J[5]) / det;
+    Ji[8] = (J[0] * J[4] - J[1] * J[3]) / det;
+  }
+  return det;
+}
+
+// reference coordinates of x0 in the cell by Newton's method; true if they lie
+// in the closed unit cell (1e-10)
+bool locate(int dim, const double *V, const double *x0, double *xi) {
+  const int nv = 1 << dim;
+  double h = 0.0, x[3], Ji[9];
+  for (int d = 0; d < dim; ++d) {
+    xi[d] = 0.5;
+    h = std::max(h, std::fabs(V[(nv - 1) * dim + d] - V[d]));
+  }
+  for (int it = 0; it < 60; ++it) {
+    host_map(dim, V, xi, x, Ji);
+    double rn = 0.0, r[3];
+    for (int i = 0; i < dim; ++i) {
+      r[i] = x[i] - x0[i];
+      rn = std::max(rn, std::fabs(r[i]));
+    }
+    for (int e = 0; e < dim; ++e) {
+      double dx = 0.0;
+      for (int i = 0; i < dim; ++i) dx += Ji[e * dim + i] * r[i];
+      xi[e] -= dx;
+      if (!(std::fabs(xi[e]) < 10.0)) return false;
+    }
+    if (rn <= 1e-14 * h) break;
+  }
+  host_map(dim, V, xi, x, Ji);
+  for (int i = 0; i < dim; ++i)
+    if (std::fabs(x[i] - x0[i]) > 1e-12 * h) return false;
+  for (int d = 0; d < dim; ++d)
+    if (xi[d] < -1e-10 || xi[d] > 1.0 + 1e-10) return false;
+  return true;
+}
+
+double ricker_integral(const exw_source &s, double t1, double t2) {
+  const double a = (M_PI * s.f0) * (M_PI * s.f0);
+  const double u1 = t1 - s.t0, u2 = t2 - s.t0;
+  return s.amplitude * (u2 * std::exp(-a * u2 * u2) - u1 * std::exp(-a * u1 * u1));
+}
+}  // namespace
+
+namespace exw {
+// u += (int_{t1}^{t2} r) b on the operator's source cells selected by sel
+int source_add_selected(exw_op op, double t1, double t2, unsigned sel, double *u) {
+  if (op->n_src == 0 || sel == 0) return EXW_OK;
+  int nd = 1;
+  for (int d = 0; d < op->dim; ++d) nd *= op->degree + 1;
+  const double I = ricker_integral(op->src, t1, t2);
+  const int total = op->n_src * nd;
+  hipLaunchKernelGGL(source_add_kernel, dim3((total + 255) / 256), dim3(256), 0,
+                     op->ctx->stream, u, op->d_src_cells, op->d_src_b, op->n_src, nd, op->dim,
+                     sel, I);
+  EXW_HIP(hipGetLastError());
+  return EXW_OK;
+}
+}  // namespace exw
+
+extern "C" {
+
+int exw_op_set_source(exw_op op, const exw_source *src) {
+  if (!op) return fail(EXW_EINVAL, "exw_op_set_source: null operator");
+  hipFree(op->d_src_cells);
+  hipFree(op->d_src_b);
+  op->d_src_cells = nullptr;
+  op->d_src_b = nullptr;
+  op->n_src = 0;
+  op->h_src_cells.clear();
+  if (!src) return EXW_OK;
+  if (!(src->f0 > 0)) return fail(EXW_EINVAL, "exw_op_set_source: f0 must be positive");
+  const int dim = op->dim, nv = 1 << dim, N = op->degree + 1;
+  int nd = 1;
+  for (int d = 0; d < dim; ++d) nd *= N;
+  std::vector<int> cells;
+  std::vector<std::array<double, 3>> xis;
+  for (int c = 0; c < op->n_cells && cells.size() < 8; ++c) {
+    std::array<double, 3> xi{0, 0, 0};
+    if (locate(dim, op->h_vertices.data() + (size_t)c * nv * dim, src->x, xi.data())) {
+      cells.push_back(c);
+      xis.push_back(xi);
+    }
+  }
+  if (cells.empty()) return fail(EXW_EINVAL, "exw_op_set_source: no cell holds the point");
+  const int ns = (int)cells.size();
+  double X[16], W[16], l[3][16];
+  cx::gauss01(N, X, W);
+  std::vector<double> b((size_t)ns * nd);
+  for (int k = 0; k < ns; ++k) {
+    const double *V = op->h_vertices.data() + (size_t)cells[k] * nv * dim;
+    for (int d = 0; d < dim; ++d) cx::lagrange(N, X, xis[k][d], l[d], nullptr);
+    for (int q = 0; q < nd; ++q) {
+      double xi[3], x[3], Ji[9], w = 1.0, phi = 1.0;
+      int t = q;
+      for (int d = 0; d < dim; ++d) {
+        xi[d] = X[t % N];
+        w *= W[t % N];
+        phi *= l[d][t % N];
+        t /= N;
+      }
+      const double det = host_map(dim, V, xi, x, Ji);
+      b[(size_t)k * nd + q] = phi / ns / (w * det);
+    }
+  }
+  int rc;
+  if ((rc = upload(op, &op->d_src_cells, cells.data(), cells.size())) ||
+      (rc = upload(op, &op->d_src_b, b.data(), b.size())))
+    return rc;
+  op->n_src = ns;
+  op->src = *src;
+  op->h_src_cells = cells;
+  return EXW_OK;
+}
+
+int exw_source_add(exw_op op, double t1, double t2, double *u) {
+  if (!op || !u) return fail(EXW_EINVAL, "exw_source_add: null argument");
+  if (op->n_src == 0) return fail(EXW_EINVAL, "exw_source_add: the operator has no source");
+  return source_add_selected(op, t1, t2, (1u << op->n_src) - 1u, u);
 }
 
 int exw_estimate_error(exw_op op, const double *solution, double *tmp, double *error_per_cell) {

@@ -121,6 +121,17 @@ int exw_memcpy_d2h(exw_ctx ctx, void *dst, const void *src, size_t bytes);
  * EXW_CART_PIPE=0 in the environment selects the latter throughout. */
 int exw_op_create(exw_ctx ctx, int dim, int degree, int n_cells, const double *vertices,
                   const int *neighbor, const double *rho, const double *speed, exw_op *out);
+/* The checks exw_op_create / exw_op_create_nc run on the connectivity, on
+ * the host, no device needed: neighbour indices and boundary ids in range,
+ * every conforming neighbour sees the cell back across face f^1, and every
+ * conforming face is in standard orientation -- corner v of face f (bit d of
+ * v equal to s, f = 2d+s) lies at corner v^(1<<d) of the neighbour (1e-10 of
+ * the cell size).  deal.II meshes may carry rotated or flipped faces
+ * (MatrixFree's face orientation); such a mesh is rejected with EXW_EINVAL
+ * rather than run with wrong face pairings.  EXW_NC_FACE entries are checked
+ * against their records by exw_op_create_nc. */
+int exw_mesh_check(int dim, int n_cells, const double *vertices, const int *neighbor,
+                   int n_hanging, const int *hanging);
 int exw_op_destroy(exw_op op);
 int64_t exw_op_n_dofs(exw_op op);
 int exw_op_n_cells(exw_op op);
@@ -179,6 +190,27 @@ int exw_l2_errors(exw_op op, const exw_modes *modes, double t, const double *u,
 /* E = 1/2 int rho|v|^2 + p^2/(rho c^2) with QGauss(k+1); synchronises. */
 int exw_energy(exw_op op, const double *u, double *out_host);
 
+/* ---- point source ---------------------------------------------------------
+ * No counterpart in the reference (the two-material refined-box workload):
+ * dp/dt gains r(t) delta(x - x0) with the Ricker wavelet
+ *   r(t) = amplitude (1 - 2 a tau^2) exp(-a tau^2),  a = (pi f0)^2, tau = t - t0.
+ * delta acts on the DG space as q -> q(x0), shared equally by the n_s cells
+ * whose closure holds x0; cell K's pressure receives M^-1 of (1/n_s) phi_i(x0)
+ * per unit time integral.  An update over [t1, t2] adds the wavelet's exact
+ * time integral, amplitude [tau exp(-a tau^2)] from t1 - t0 to t2 - t0, times
+ * that vector: the time-integrated point source of ADER schemes (it enters the
+ * update, not the Cauchy-Kowalevski predictor). */
+typedef struct exw_source {
+  double x[3];
+  double amplitude, f0, t0;
+} exw_source;
+/* locate x0 (at most 8 cells) and set the operator's source; NULL removes it.
+ * EXW_EINVAL if no cell holds x0.  Synchronises. */
+int exw_op_set_source(exw_op op, const exw_source *src);
+/* u += (int_{t1}^{t2} r dt) M^-1 delta_x0 on the pressure (after an ADER step
+ * from t1 to t2); asynchronous */
+int exw_source_add(exw_op op, double t1, double t2, double *u);
+
 /* ---- ADER with local time stepping --------------------------------------
  * WaveEquationOperationADERLTS (source/wave_equation_operations.cc:1385-1692)
  * with ClusterManager (include/cluster_manager.templates.h:24-1049): cells are
@@ -197,6 +229,19 @@ int exw_lts_categorize(int dim, int n_cells, const double *vertices, const int *
  * use_ader_post.  The operator must outlive the handle. */
 int exw_lts_create(exw_op op, double cfl, double fastest_dt, int max_clusters, int max_diff,
                    double time0, int recon, exw_lts *out);
+/* exw_lts_create with flags.  EXW_LTS_SPEED_AWARE clusters cells by
+ * cfl * min vertex distance / c: the step each cell's own CFL limit allows
+ * (the reference clusters by cfl * min vertex distance only,
+ * include/cluster_manager.templates.h:57-66, while its global step divides by
+ * c, source/explicit_wave.cc:529-545).  Operators with 2:1 hanging faces
+ * (exw_op_create_nc) are taken by both entry points: a cell's neighbours
+ * across such a face are the coarse cell or the fine cells of the subfaces
+ * (include/cluster_manager.templates.h:118-125), and each subface is a face of
+ * its own in the update masks.  A point source set on the operator is added
+ * to every update of its cells over that update's interval. */
+#define EXW_LTS_SPEED_AWARE 1
+int exw_lts_create2(exw_op op, double cfl, double fastest_dt, int max_clusters, int max_diff,
+                    double time0, int recon, int flags, exw_lts *out);
 int exw_lts_destroy(exw_lts L);
 /* global_dt = the slowest cluster's step, by which the time control advances */
 int exw_lts_info(exw_lts L, int *n_clusters, int *cluster_diff, int *n_updates,

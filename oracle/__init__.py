@@ -63,6 +63,30 @@ class Exact(C.Structure):
         return Exact.make(dim, [[membrane_modes] * dim])
 
 
+class Source(C.Structure):
+    """ox_source: Ricker point source of pressure at x."""
+    _fields_ = [("x", C.c_double * 3), ("amplitude", C.c_double), ("f0", C.c_double),
+                ("t0", C.c_double)]
+
+    @staticmethod
+    def make(x, amplitude=1.0, f0=10.0, t0=None):
+        s = Source()
+        for i, v in enumerate(x):
+            s.x[i] = float(v)
+        s.amplitude, s.f0 = float(amplitude), float(f0)
+        s.t0 = 1.0 / f0 if t0 is None else float(t0)
+        return s
+
+    def value(self, t):
+        return lib().ox_ricker(C.byref(self), t)
+
+    def integral(self, t1, t2):
+        return lib().ox_ricker_integral(C.byref(self), t1, t2)
+
+
+LTS_SPEED_AWARE = 1
+
+
 class Problem(C.Structure):
     _fields_ = [("dim", C.c_int), ("degree", C.c_int), ("n_intervals", C.c_int),
                 ("n_refinements", C.c_int), ("transform", C.c_int),
@@ -124,6 +148,18 @@ def lib():
         L.ox_lts_schedule.argtypes = [C.c_void_p, ip, dp]
         L.ox_lts_cells.argtypes = [C.c_void_p, ip, ip]
         L.ox_lts_step.argtypes = [C.c_void_p, C.c_double, dp, dp]
+        L.ox_lts_create2.restype = C.c_void_p
+        L.ox_lts_create2.argtypes = [C.POINTER(_Mesh), C.c_int, C.c_double, C.c_double, C.c_int,
+                                     C.c_int, C.c_double, C.c_int, C.c_int]
+        L.ox_lts_set_source.argtypes = [C.c_void_p, C.POINTER(Source)]
+        L.ox_ricker.restype = C.c_double
+        L.ox_ricker.argtypes = [C.POINTER(Source), C.c_double]
+        L.ox_ricker_integral.restype = C.c_double
+        L.ox_ricker_integral.argtypes = [C.POINTER(Source), C.c_double, C.c_double]
+        L.ox_source_cells.restype = C.c_int
+        L.ox_source_cells.argtypes = [C.POINTER(_Mesh), C.c_int, C.POINTER(Source), ip, dp]
+        L.ox_source_add.argtypes = [C.POINTER(_Mesh), C.c_int, C.POINTER(Source), C.c_double,
+                                    C.c_double, dp]
         L.ox_gauss_lobatto.argtypes = [C.c_int, dp]
         L.ox_estimate_error.argtypes = [C.POINTER(_Mesh), C.c_int, dp, dp]
         L.ox_adapt_transfer.argtypes = [C.c_int, C.c_int, C.c_int, ip, ip, dp, dp]
@@ -267,6 +303,53 @@ class Mesh:
 
     def post_error(self, degree, exact, t, post):
         return lib().ox_post_error(self._m, degree, C.byref(exact), t, _p(post))
+
+    def source_cells(self, degree, src):
+        """(cells holding the source point, their nodal vectors b [n_s, (k+1)^dim])."""
+        nd = (degree + 1) ** self.dim
+        cells = np.zeros(8, dtype=np.int32)
+        b = np.zeros(8 * nd)
+        n = lib().ox_source_cells(self._m, degree, C.byref(src),
+                                  cells.ctypes.data_as(C.POINTER(C.c_int)), _p(b))
+        return cells[:n].copy(), b[:n * nd].reshape(n, nd).copy()
+
+    def source_add(self, degree, src, t1, t2, u):
+        lib().ox_source_add(self._m, degree, C.byref(src), t1, t2, _p(u))
+
+
+class Lts:
+    """ox_lts: ADER with local time stepping on an oracle mesh."""
+
+    def __init__(self, mesh, degree, cfl, fastest_dt, max_clusters, max_diff, time0=0.0,
+                 recon=0, speed_aware=False, source=None):
+        L = lib()
+        self.mesh = mesh
+        self.h = L.ox_lts_create2(mesh._m, degree, cfl, fastest_dt, max_clusters, max_diff,
+                                  time0, int(recon), LTS_SPEED_AWARE if speed_aware else 0)
+        assert self.h, "ox_lts_create2 failed"
+        if source is not None:
+            self._src = source
+            L.ox_lts_set_source(self.h, C.byref(source))
+        self.n_clusters = L.ox_lts_n_clusters(self.h)
+        self.cluster_diff = L.ox_lts_cluster_diff(self.h)
+        self.global_dt = L.ox_lts_global_dt(self.h)
+
+    def cells(self):
+        cl = np.zeros(self.mesh.n_cells, np.int32)
+        fl = np.zeros(self.mesh.n_cells, np.int32)
+        ip = C.POINTER(C.c_int)
+        lib().ox_lts_cells(self.h, cl.ctypes.data_as(ip), fl.ctypes.data_as(ip))
+        return cl, fl
+
+    def step(self, time_now, u):
+        out = np.zeros_like(u)
+        lib().ox_lts_step(self.h, time_now, _p(np.ascontiguousarray(u)), _p(out))
+        return out
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.ox_lts_free(self.h)
+            self.h = None
 
 
 def run(problem: Problem):

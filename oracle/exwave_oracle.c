@@ -464,9 +464,11 @@ static double eval_at(const ox_basis *b, int dim, const double *uk, const double
  *   coarse side: the sum over the 2^(dim-1) subfaces of the same integrals,
  *     with this cell's trace, geometry and test functions at the mapped
  *     points (xi_t + b_t)/2 and the weights w_t/2 per tangential direction.
- * Same flux (source/wave_equation_operations.cc:480-551) on both sides. */
+ * Same flux (source/wave_equation_operations.cc:480-551) on both sides,
+ * times boundary_fac `fac`.  sb_only >= 0 restricts the coarse side to that
+ * subface (one MatrixFree face of the pair; local time stepping). */
 static void cell_face_nc(const ox_mesh *m, const ox_basis *b, int c, int f, const double *src,
-                         double *r) {
+                         int sb_only, double fac, double *r) {
   const int dim = m->dim, n = b->n, nf = dim + 1, nface = 2 * dim;
   const int nd = ipow(n, dim), nv = 1 << dim, nt = ipow(n, dim - 1), nsub = 1 << (dim - 1);
   const int rec = 2 + nsub;
@@ -479,6 +481,7 @@ static void cell_face_nc(const ox_mesh *m, const ox_basis *b, int c, int f, cons
   const int stride_d = ipow(n, d);
   const int nsb = pos >= 0 ? 1 : nsub;
   for (int sb = 0; sb < nsb; ++sb) {
+    if (pos < 0 && sb_only >= 0 && sb != sb_only) continue;
     const int bpos = pos >= 0 ? pos : sb;
     const int other = pos >= 0 ? R[0] : R[2 + sb];
     const double *uo = src + (long)other * nf * nd;
@@ -523,7 +526,7 @@ static void cell_face_nc(const ox_mesh *m, const ox_basis *b, int c, int f, cons
         }
         um[k] = eval_at(b, dim, uo + k * nd, xn);
       }
-      face_flux(dim, nrm, up, um, rho, cc, m->rho[other], m->speed[other], 0, 1.0, val);
+      face_flux(dim, nrm, up, um, rho, cc, m->rho[other], m->speed[other], 0, fac, val);
       if (pos >= 0) {
         for (int k = 0; k < nf; ++k)
           for (int i = 0; i < n; ++i)
@@ -656,7 +659,7 @@ static void apply_cell(const ox_mesh *m, const ox_basis *b, int c, const double 
   for (int f = 0; f < 2 * dim; ++f) {
     const int nb = m->neighbor[c * 2 * dim + f];
     if (nb == OX_NC_FACE)
-      cell_face_nc(m, b, c, f, src, r);
+      cell_face_nc(m, b, c, f, src, -1, 1.0, r);
     else
       cell_face(m, b, c, f, u, nb >= 0 ? src + (long)nb * nf * nd : NULL, 1.0, r);
   }
@@ -1709,6 +1712,98 @@ double ox_post_error(const ox_mesh *m, int degree, const ox_exact *e, double t,
 }
 
 /* ------------------------------------------------------------------------ */
+/* Point source (no counterpart in the reference; see exwave_oracle.h)        */
+/* ------------------------------------------------------------------------ */
+
+double ox_ricker(const ox_source *s, double t) {
+  const double a = (M_PI * s->f0) * (M_PI * s->f0), tau = t - s->t0;
+  return s->amplitude * (1.0 - 2.0 * a * tau * tau) * exp(-a * tau * tau);
+}
+
+/* d/dt [tau exp(-a tau^2)] = (1 - 2 a tau^2) exp(-a tau^2) */
+double ox_ricker_integral(const ox_source *s, double t1, double t2) {
+  const double a = (M_PI * s->f0) * (M_PI * s->f0);
+  const double u1 = t1 - s->t0, u2 = t2 - s->t0;
+  return s->amplitude * (u2 * exp(-a * u2 * u2) - u1 * exp(-a * u1 * u1));
+}
+
+/* reference coordinates of x in cell c by Newton's method on the multilinear
+ * map; 1 if they lie in the closed unit cell (tolerance 1e-10) */
+static int locate_in_cell(const ox_mesh *m, int c, const double *x, double *xi) {
+  const int dim = m->dim, nv = 1 << dim;
+  const double *V = m->vertices + (long)c * nv * dim;
+  for (int d = 0; d < dim; ++d) xi[d] = 0.5;
+  double h = 0.0;
+  for (int d = 0; d < dim; ++d) h = fmax(h, fabs(V[(nv - 1) * dim + d] - V[d]));
+  for (int it = 0; it < 60; ++it) {
+    double y[3], J[9], Ji[9], r[3], rn = 0.0;
+    multilinear_point(dim, V, xi, y);
+    multilinear_jacobian(dim, V, xi, J, Ji);
+    for (int i = 0; i < dim; ++i) {
+      r[i] = y[i] - x[i];
+      rn = fmax(rn, fabs(r[i]));
+    }
+    for (int e = 0; e < dim; ++e) {
+      double dx = 0.0;
+      for (int i = 0; i < dim; ++i) dx += Ji[e * dim + i] * r[i];
+      xi[e] -= dx;
+      if (!(fabs(xi[e]) < 10.0)) return 0; /* far outside: diverging */
+    }
+    if (rn <= 1e-14 * h) break;
+  }
+  double y[3], rn = 0.0;
+  multilinear_point(dim, V, xi, y);
+  for (int i = 0; i < dim; ++i) rn = fmax(rn, fabs(y[i] - x[i]));
+  if (rn > 1e-12 * h) return 0;
+  for (int d = 0; d < dim; ++d)
+    if (xi[d] < -1e-10 || xi[d] > 1.0 + 1e-10) return 0;
+  return 1;
+}
+
+int ox_source_cells(const ox_mesh *m, int degree, const ox_source *s, int *cells, double *b) {
+  const int dim = m->dim, nv = 1 << dim;
+  ox_basis B;
+  ox_basis_init(&B, degree);
+  const int n = B.n, nd = ipow(n, dim);
+  double xis[8][3];
+  int ns = 0;
+  for (int c = 0; c < m->n_cells && ns < 8; ++c)
+    if (locate_in_cell(m, c, s->x, xis[ns])) cells[ns++] = c;
+  for (int k = 0; k < ns; ++k) {
+    const double *V = m->vertices + (long)cells[k] * nv * dim;
+    double l[3][OX_MAX_N];
+    for (int d = 0; d < dim; ++d) ox_lagrange(n, B.x, xis[k][d], l[d], NULL);
+    for (int q = 0; q < nd; ++q) {
+      double xi[3], J[9], Ji[9], w = 1.0, phi = 1.0;
+      int t = q;
+      for (int d = 0; d < dim; ++d) {
+        xi[d] = B.x[t % n];
+        w *= B.w[t % n];
+        phi *= l[d][t % n];
+        t /= n;
+      }
+      const double det = multilinear_jacobian(dim, V, xi, J, Ji);
+      b[(long)k * nd + q] = phi / ns / (w * det);
+    }
+  }
+  return ns;
+}
+
+void ox_source_add(const ox_mesh *m, int degree, const ox_source *s, double t1, double t2,
+                   double *u) {
+  const int nd = ipow(degree + 1, m->dim);
+  int cells[8];
+  double *b = (double *)malloc(sizeof(double) * 8 * nd);
+  const int ns = ox_source_cells(m, degree, s, cells, b);
+  const double I = ox_ricker_integral(s, t1, t2);
+  for (int k = 0; k < ns; ++k) {
+    double *p = u + ((long)cells[k] * (m->dim + 1) + m->dim) * nd;
+    for (int q = 0; q < nd; ++q) p[q] += I * b[(long)k * nd + q];
+  }
+  free(b);
+}
+
+/* ------------------------------------------------------------------------ */
 /* ADER with local time stepping (WaveEquationOperationADERLTS,               */
 /* source/wave_equation_operations.cc:1385-1692, and ClusterManager,          */
 /* include/cluster_manager.templates.h:24-1049), restated per cell.           */
@@ -1745,6 +1840,11 @@ struct ox_lts {
   /* times of the current cycle */
   double t1fa, t2fa, t1sl, t2sl, t1sa, t2sa, dt;
   int fm_considered;
+  /* point source (ox_lts_set_source): cells and their M^-1 delta vectors */
+  int n_src;
+  int src_cells[8];
+  double *src_b;
+  ox_source source;
 };
 
 static double cell_min_vertex_distance(const ox_mesh *m, int c) {
@@ -1760,15 +1860,44 @@ static double cell_min_vertex_distance(const ox_mesh *m, int c) {
   return sqrt(mind);
 }
 
+/* The cells across face f of cell c, as the MatrixFree faces the reference
+ * loops over see them: none at a boundary, one across a conforming face or
+ * from the fine side of a 2:1 face (the coarse cell), the 2^(dim-1) fine cells
+ * by subface position from the coarse side (cell->neighbor_child_on_subface,
+ * include/cluster_manager.templates.h:118-125).  Returns the count. */
+static int face_cells(const ox_mesh *m, int c, int f, int *nbs) {
+  const int nface = 2 * m->dim, nsub = 1 << (m->dim - 1);
+  const int nb = m->neighbor[c * nface + f];
+  if (nb >= 0) {
+    nbs[0] = nb;
+    return 1;
+  }
+  if (nb != OX_NC_FACE) return 0;
+  const int *R = m->hanging + (long)m->nc_rec[c * nface + f] * (2 + nsub);
+  if (m->nc_pos[c * nface + f] >= 0) {
+    nbs[0] = R[0];
+    return 1;
+  }
+  for (int b = 0; b < nsub; ++b) nbs[b] = R[2 + b];
+  return nsub;
+}
+
 /* iterate_cluster_categorization (include/cluster_manager.templates.h:24-262)
- * on one process: cells visited in active-cell order, ids changed in place. */
-static void lts_categorize(ox_lts *L, double cfl, int max_clusters, int max_diff) {
+ * on one process: cells visited in active-cell order, ids changed in place,
+ * neighbours across 2:1 faces through their subfaces (:118-125).  The
+ * reference's per-cell step is cfl * minimum_vertex_distance (:57-66, no wave
+ * speed); with speed_aware the step is divided by the cell's wave speed, the
+ * CFL rule of the global step (source/explicit_wave.cc:529-545). */
+static void lts_categorize(ox_lts *L, double cfl, int max_clusters, int max_diff,
+                           int speed_aware) {
   const ox_mesh *m = L->m;
   const int n = m->n_cells, nface = 2 * m->dim;
+  int nbs[4];
   double *ts = (double *)malloc(sizeof(double) * n);
   double minimal = 1e300;
   for (int c = 0; c < n; ++c) {
-    ts[c] = cfl * cell_min_vertex_distance(m, c); /* no wave speed here (:60) */
+    ts[c] = cfl * cell_min_vertex_distance(m, c);
+    if (speed_aware) ts[c] /= m->speed[c];
     if (ts[c] < minimal) minimal = ts[c];
   }
   const double eps = minimal / 10.0;
@@ -1793,12 +1922,14 @@ static void lts_categorize(ox_lts *L, double cfl, int max_clusters, int max_diff
         if (id[c] != cl) continue;
         int anyf = 0;
         for (int f = 0; f < nface; ++f) {
-          int nb = m->neighbor[c * nface + f];
-          if (nb >= 0 && id[nb] < cl) anyf = 1;
+          const int k = face_cells(m, c, f, nbs);
+          for (int j = 0; j < k; ++j)
+            if (id[nbs[j]] < cl) anyf = 1;
         }
         for (int f = 0; f < nface; ++f) {
-          int nb = m->neighbor[c * nface + f];
-          if (nb >= 0 && id[nb] > cl) id[nb] = anyf ? cl : cl + 1;
+          const int k = face_cells(m, c, f, nbs);
+          for (int j = 0; j < k; ++j)
+            if (id[nbs[j]] > cl) id[nbs[j]] = anyf ? cl : cl + 1;
         }
       }
     fs = 0;
@@ -1806,13 +1937,15 @@ static void lts_categorize(ox_lts *L, double cfl, int max_clusters, int max_diff
       const int cur = id[c];
       int fa = 0, sl = 0;
       for (int f = 0; f < nface; ++f) {
-        int nb = m->neighbor[c * nface + f];
-        if (nb < 0) continue;
-        if (id[nb] < cur)
-          fa = 1;
-        else if (id[nb] > cur)
-          sl = 1;
-        if (id[nb] < cur - 1 || id[nb] > cur + 1) fs = 1;
+        const int k = face_cells(m, c, f, nbs);
+        for (int j = 0; j < k; ++j) {
+          const int nb = nbs[j];
+          if (id[nb] < cur)
+            fa = 1;
+          else if (id[nb] > cur)
+            sl = 1;
+          if (id[nb] < cur - 1 || id[nb] > cur + 1) fs = 1;
+        }
       }
       L->faster[c] = (char)fa;
       L->slower[c] = (char)sl;
@@ -1889,7 +2022,11 @@ static void lts_schedule(ox_lts *L, double time0) {
 
 ox_lts *ox_lts_create(const ox_mesh *m, int degree, double cfl, double fastest_dt,
                       int max_clusters, int max_diff, double time0, int recon) {
-  if (m->n_hanging > 0) return NULL; /* non-conforming faces: not restated for LTS */
+  return ox_lts_create2(m, degree, cfl, fastest_dt, max_clusters, max_diff, time0, recon, 0);
+}
+
+ox_lts *ox_lts_create2(const ox_mesh *m, int degree, double cfl, double fastest_dt,
+                       int max_clusters, int max_diff, double time0, int recon, int flags) {
   ox_lts *L = (ox_lts *)calloc(1, sizeof(ox_lts));
   L->m = m;
   L->degree = degree;
@@ -1907,7 +2044,7 @@ ox_lts *ox_lts_create(const ox_mesh *m, int degree, double cfl, double fastest_d
   L->E = (char *)calloc(n, 1);
   L->nbrU = (char *)calloc(n, 1);
   L->cell_time = (double *)calloc(n, sizeof(double));
-  lts_categorize(L, cfl, max_clusters, max_diff);
+  lts_categorize(L, cfl, max_clusters, max_diff, (flags & OX_LTS_SPEED_AWARE) != 0);
   L->fastest_dt = fastest_dt;
   lts_schedule(L, time0);
   const long N = L->N;
@@ -1939,7 +2076,18 @@ void ox_lts_free(ox_lts *L) {
   free(L->fm);
   free(L->igd);
   free(L->rstate);
+  free(L->src_b);
   free(L);
+}
+
+void ox_lts_set_source(ox_lts *L, const ox_source *src) {
+  free(L->src_b);
+  L->src_b = NULL;
+  L->n_src = 0;
+  if (!src) return;
+  L->source = *src;
+  L->src_b = (double *)malloc(sizeof(double) * 8 * L->nd);
+  L->n_src = ox_source_cells(L->m, L->degree, src, L->src_cells, L->src_b);
 }
 
 int ox_lts_n_clusters(const ox_lts *L) { return L->n_clusters; }
@@ -1979,15 +2127,20 @@ static void lts_ck_cells(ox_lts *L, const double *src, const char *flag, double 
   }
 }
 
-/* contribution of face f of cell c (evaluated from tempsrc on both sides,
- * boundary_fac -1 as in local_apply_ader_face :1514-1537) added into out */
-static void lts_face_into(ox_lts *L, int c, int f, double *out, work *W) {
+/* contribution of face f of cell c, across slot j of face_cells (the subface
+ * of a 2:1 face seen from its coarse side), evaluated from tempsrc on both
+ * sides with boundary_fac -1 as in local_apply_ader_face :1514-1537, added
+ * into out */
+static void lts_face_into(ox_lts *L, int c, int f, int j, double *out, work *W) {
   const ox_mesh *m = L->m;
   const long cs = (long)L->nf * L->nd;
   const int nb = m->neighbor[c * 2 * m->dim + f];
   memset(W->r, 0, sizeof(double) * cs);
-  cell_face(m, &L->b, c, f, L->tempsrc + c * cs, nb >= 0 ? L->tempsrc + (long)nb * cs : NULL,
-            -1.0, W->r);
+  if (nb == OX_NC_FACE)
+    cell_face_nc(m, &L->b, c, f, L->tempsrc, j, -1.0, W->r);
+  else
+    cell_face(m, &L->b, c, f, L->tempsrc + c * cs, nb >= 0 ? L->tempsrc + (long)nb * cs : NULL,
+              -1.0, W->r);
   for (long i = 0; i < cs; ++i) out[c * cs + i] += W->r[i];
 }
 
@@ -2000,13 +2153,15 @@ static void lts_update_elements(ox_lts *L, double *lstate, int ac, int write_fm)
   const double tol = L->rel_tol * L->fastest_dt;
   const int *cl = L->cluster;
   char *U = L->U, *E = L->E, *nbrU = L->nbrU;
+  int nbs[4];
   work W = work_alloc(m->dim, L->b.n);
   for (int c = 0; c < n; ++c) {
     nbrU[c] = 0;
     if (U[c]) continue;
     for (int f = 0; f < nface; ++f) {
-      int nb = m->neighbor[c * nface + f];
-      if (nb >= 0 && U[nb]) nbrU[c] = 1;
+      const int k = face_cells(m, c, f, nbs);
+      for (int j = 0; j < k; ++j)
+        if (U[nbs[j]]) nbrU[c] = 1;
     }
   }
   /* contributions across the faces to the faster (ac-1) and slower (ac+1)
@@ -2024,12 +2179,15 @@ static void lts_update_elements(ox_lts *L, double *lstate, int ac, int write_fm)
     for (int c = 0; c < n; ++c) {
       if (!E[c]) continue;
       for (int f = 0; f < nface; ++f) {
-        int nb = m->neighbor[c * nface + f];
-        if (nb < 0 || !E[nb]) continue;
-        if (cl[c] == ac && cl[nb] != ac && nbrU[nb])
-          lts_face_into(L, c, f, L->dst, &W);
-        else if (cl[c] != ac && cl[nb] == ac && nbrU[c] && write_fm)
-          lts_face_into(L, c, f, L->fm, &W);
+        const int k = face_cells(m, c, f, nbs);
+        for (int j = 0; j < k; ++j) {
+          const int nb = nbs[j];
+          if (!E[nb]) continue;
+          if (cl[c] == ac && cl[nb] != ac && nbrU[nb])
+            lts_face_into(L, c, f, j, L->dst, &W);
+          else if (cl[c] != ac && cl[nb] == ac && nbrU[c] && write_fm)
+            lts_face_into(L, c, f, j, L->fm, &W);
+        }
       }
     }
   }
@@ -2039,13 +2197,16 @@ static void lts_update_elements(ox_lts *L, double *lstate, int ac, int write_fm)
   for (int c = 0; c < n; ++c) {
     if (!E[c]) continue;
     for (int f = 0; f < nface; ++f) {
-      int nb = m->neighbor[c * nface + f];
-      int take;
-      if (nb < 0)
-        take = 1;
-      else
-        take = (U[c] && U[nb]) || (U[c] && E[nb] && cl[nb] == ac) || (E[c] && U[nb] && cl[c] == ac);
-      if (take) lts_face_into(L, c, f, L->dst, &W);
+      const int k = face_cells(m, c, f, nbs);
+      if (k == 0) {
+        lts_face_into(L, c, f, 0, L->dst, &W); /* boundary face */
+        continue;
+      }
+      for (int j = 0; j < k; ++j) {
+        const int nb = nbs[j];
+        if ((U[c] && U[nb]) || (U[c] && E[nb] && cl[nb] == ac) || (E[c] && U[nb] && cl[c] == ac))
+          lts_face_into(L, c, f, j, L->dst, &W);
+      }
     }
   }
   /* second ADER sweep on the update cells (local_apply_secondader_domain
@@ -2081,6 +2242,15 @@ static void lts_update_elements(ox_lts *L, double *lstate, int ac, int write_fm)
   for (long i = 0; i < L->N; ++i) {
     lstate[i] -= L->dst[i];
     L->dst[i] = 0.0;
+  }
+  /* the point source over each updated source cell's own interval
+   * [t_K, t_K + dt] (no counterpart in the reference) */
+  for (int s = 0; s < L->n_src; ++s) {
+    const int c = L->src_cells[s];
+    if (!U[c]) continue;
+    const double I = ox_ricker_integral(&L->source, L->cell_time[c], L->cell_time[c] + L->dt);
+    double *pc = lstate + c * cs + (long)m->dim * L->nd;
+    for (int q = 0; q < L->nd; ++q) pc[q] += I * L->src_b[(long)s * L->nd + q];
   }
   work_free(&W);
 }
@@ -2149,16 +2319,18 @@ void ox_lts_step(ox_lts *L, double time_now, const double *src, double *dst) {
       for (int c = 0; c < n; ++c) {
         Ucopy[c] = 0;
         if (cl[c] != other) continue;
+        int nbs[4];
         if (side == 0) {
           /* the last face with a neighbour decides (:635-645) */
           for (int f = 0; f < nface; ++f) {
-            int nb = m->neighbor[c * nface + f];
-            if (nb >= 0) Ucopy[c] = tf[nb];
+            const int k = face_cells(m, c, f, nbs);
+            for (int j = 0; j < k; ++j) Ucopy[c] = tf[nbs[j]];
           }
         } else {
           for (int f = 0; f < nface; ++f) {
-            int nb = m->neighbor[c * nface + f];
-            if (nb >= 0 && tf[nb]) Ucopy[c] = 1;
+            const int k = face_cells(m, c, f, nbs);
+            for (int j = 0; j < k; ++j)
+              if (tf[nbs[j]]) Ucopy[c] = 1;
           }
         }
       }

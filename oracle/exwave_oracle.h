@@ -204,6 +204,30 @@ void ox_post_set_solver(double abs_tol, double rel_tol, int max_iter_factor, int
 double ox_post_error(const ox_mesh *m, int degree, const ox_exact *e, double t,
                      const double *post);
 
+/* Point source of pressure (config-4 workload; no counterpart in the
+ * reference): dp/dt gains r(t) delta(x - x0) with the Ricker wavelet
+ *   r(t) = amplitude (1 - 2 a tau^2) exp(-a tau^2),  a = (pi f0)^2, tau = t - t0.
+ * Discretely, delta is the functional q -> q(x0) on the DG space, shared
+ * equally by the n_s cells whose closure holds x0 (a point on a face, edge or
+ * vertex), so the pressure right-hand side of cell K is
+ * (1/n_s) phi_i(x0), and M^-1 of it is b_K.  A time update over [t1, t2]
+ * adds (int_{t1}^{t2} r dt) b_K to the pressure of K: the source's exact time
+ * integral added to the time-integrated update, as ADER point-source
+ * implementations do. */
+typedef struct ox_source {
+  double x[3];
+  double amplitude, f0, t0;
+} ox_source;
+double ox_ricker(const ox_source *s, double t);
+/* closed form: amplitude [tau exp(-a tau^2)] between t1 - t0 and t2 - t0 */
+double ox_ricker_integral(const ox_source *s, double t1, double t2);
+/* the cells holding x0 (<= 8, returned count n_s) and per cell the nodal
+ * vector b (n_s * (k+1)^dim values, pressure field) */
+int ox_source_cells(const ox_mesh *m, int degree, const ox_source *s, int *cells, double *b);
+/* u += (int_{t1}^{t2} r dt) b on the pressure of the source cells */
+void ox_source_add(const ox_mesh *m, int degree, const ox_source *s, double t1, double t2,
+                   double *u);
+
 /* ADER with local time stepping: WaveEquationOperationADERLTS
  * (source/wave_equation_operations.cc:1385-1692) + ClusterManager
  * (include/cluster_manager.templates.h:24-1049), per cell.
@@ -212,6 +236,14 @@ double ox_post_error(const ox_mesh *m, int degree, const ox_exact *e, double t,
 typedef struct ox_lts ox_lts;
 ox_lts *ox_lts_create(const ox_mesh *m, int degree, double cfl, double fastest_dt,
                       int max_clusters, int max_diff, double time0, int recon);
+/* flags: OX_LTS_SPEED_AWARE clusters by cfl * min vertex distance / c (the
+ * reference: cfl * min vertex distance).  Meshes with 2:1 hanging faces are
+ * taken: their faces are evaluated per subface like any other face. */
+#define OX_LTS_SPEED_AWARE 1
+ox_lts *ox_lts_create2(const ox_mesh *m, int degree, double cfl, double fastest_dt,
+                       int max_clusters, int max_diff, double time0, int recon, int flags);
+/* add the point source to every update of its cells (NULL: none) */
+void ox_lts_set_source(ox_lts *L, const ox_source *src);
 void ox_lts_free(ox_lts *L);
 int ox_lts_n_clusters(const ox_lts *L);
 int ox_lts_cluster_diff(const ox_lts *L);

@@ -234,3 +234,55 @@ def test_adapt_box_refine():
     fl[4 + 1] = 1  # base 2 is the third output cell block: bases 0 (4 cells), 1 -> index 5
     rn = X.adapt_box_refine(dim, n, ro, fl)
     assert rn[0] == 0 and rn[4] == 1 and rn[2] == 1 and rn.sum() == 2
+
+
+def _check(dim, V, nb):
+    import ctypes as C
+
+    import exwave_amd.capi as capi
+    L = capi.load()
+    V = np.ascontiguousarray(V, dtype=np.float64)
+    nb = np.ascontiguousarray(nb, dtype=np.int32)
+    return L.exw_mesh_check(dim, V.shape[0], V.ctypes.data_as(C.POINTER(C.c_double)),
+                            nb.ctypes.data_as(C.POINTER(C.c_int)), 0, None)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_mesh_check_rejects_non_standard_orientation(dim):
+    """exw_mesh_check (run by exw_op_create): generated meshes pass; two cells
+    side by side whose shared face runs the other way in one of them (the
+    second cell mirrored in xi_1: a flipped face, faces still paired as 1/0),
+    a cell whose faces pair as f/f rather than f/f^1 (a rotated or mirrored
+    cell inside a mesh), and a neighbour entry that is not returned are all
+    rejected before anything runs."""
+    import exwave_amd as X
+    V, nb = X.mesh_hypercube(dim, 3, 0, X.TRANSFORM_REFERENCE_SINE, 0.1, 2)
+    assert _check(dim, V, nb) == 0
+    nv = 1 << dim
+    # two unit cells along x, every other face a boundary
+    V2 = np.zeros((2, nv, dim))
+    for c in range(2):
+        for v in range(nv):
+            for d in range(dim):
+                V2[c, v, d] = ((v >> d) & 1) + (c if d == 0 else 0)
+    nb2 = np.full((2, 2 * dim), -2, dtype=np.int32)
+    nb2[0, 1], nb2[1, 0] = 1, 0
+    assert _check(dim, V2, nb2) == 0
+    V2f = V2.copy()
+    V2f[1] = V2[1][[v ^ 2 for v in range(nv)]]  # mirrored in xi_1
+    assert _check(dim, V2f, nb2) == -1  # EXW_EINVAL
+    assert "orientation" in X.load().exw_last_error().decode()
+    # inside a mesh, the mirrored cell's xi_1 faces pair with its neighbours'
+    # faces of the same index
+    c = 4
+    Vf = V.copy()
+    Vf[c] = V[c][[v ^ 2 for v in range(nv)]]
+    nbf = nb.copy()
+    nbf[c, 2], nbf[c, 3] = nb[c, 3], nb[c, 2]
+    assert _check(dim, Vf, nbf) == -1
+    assert "orientation" in X.load().exw_last_error().decode()
+    # a neighbour entry that is not returned
+    nbr = nb.copy()
+    j = int(np.flatnonzero(nbr[:, 1] >= 0)[0])
+    nbr[j, 1] = nbr[j, 1] + 1 if nbr[j, 1] + 1 < V.shape[0] else 0
+    assert _check(dim, V, nbr) < 0

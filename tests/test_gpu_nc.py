@@ -121,5 +121,5 @@ def test_hanging_records_are_checked(X, ctx):
                        hanging=om.hanging[1:])
     op = X.WaveOperator(ctx, 2, 2, om.vertices, om.neighbor, om.rho, om.speed,
                         hanging=om.hanging)
-    with pytest.raises(X.ExwError):  # local time stepping does not take hanging faces
-        X.LocalTimeStepping(op, 0.1, 0.01, 2, 1, 0.0, False)
+    with pytest.raises(X.ExwError):  # bad LTS arguments are rejected on such meshes too
+        X.LocalTimeStepping(op, 0.1, -0.01, 2, 1, 0.0, False)

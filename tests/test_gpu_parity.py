@@ -279,7 +279,7 @@ def test_config1_membrane_2d_k3(X, ctx):
         for a, b in zip(g[2:5], w[2:5]):
             assert abs(a - b) <= 1e-9 * abs(b), (g, w)
     # ||p0||: two orthogonal modes of L2 norm 1/2 each
-    assert abs(rows[0][4] - np.sqrt(0.25 + 0.25)) <= 1e-9
+    assert abs(rows[0][4] - np.sqrt(0.25 + 0.25)) <= 1e-10
     # the t=0 error is that of the Gauss-point interpolant, in closed form
     import ctypes as C
     from exwave_amd import serving
@@ -289,7 +289,9 @@ def test_config1_membrane_2d_k3(X, ctx):
     e0, n0, sc, pr = C.c_double(), C.c_double(), C.c_double(), C.c_int()
     X.check(X.load().exw_request_initial_error(C.byref(r), C.byref(e0), C.byref(n0), C.byref(sc),
                                                C.byref(pr)))
-    assert abs(rows[0][2] - e0.value) <= 1e-9 * e0.value and abs(rows[0][4] - n0.value) <= 1e-12
+    # (the log prints 10-11 significant digits)
+    assert abs(rows[0][2] - e0.value) <= 1e-9 * e0.value
+    assert abs(rows[0][4] - n0.value) <= 1e-10 * n0.value
     # k=3 on 16x16 cells (order h^4): the error stays at the interpolation level
     assert max(r_[2] for r_ in rows) < 1.1 * rows[0][2] and rows[-1][3] < 2e-5
 

