@@ -128,6 +128,7 @@ SIGNATURES = [
     ("exw_time_step_work_vectors", _I, [_I]),
     ("exw_project", _I, [_VP, C.POINTER(Modes), _D, _VP]),
     ("exw_l2_errors", _I, [_VP, C.POINTER(Modes), _D, _VP, _DP]),
+    ("exw_l2_errors_noalloc", _I, [_VP, C.POINTER(Modes), _D, _VP, _VP]),
     ("exw_energy", _I, [_VP, _VP, _DP]),
     ("exw_post_n_values", C.c_int64, [_VP]),
     ("exw_estimate_error", _I, [_VP, _VP, _VP, _VP]),

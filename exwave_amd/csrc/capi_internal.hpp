@@ -21,6 +21,9 @@ int source_add_selected(exw_op_s *op, double t1, double t2, unsigned sel, double
 struct exw_ctx_s {
   int device;
   hipStream_t stream;
+  // this device's copy of the library's static scratch array (the device
+  // address from hipGetSymbolAddress; exw_l2_errors_noalloc)
+  double *scratch = nullptr;
 };
 
 struct exw_op_s {

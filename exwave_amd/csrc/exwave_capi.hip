@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -46,6 +47,38 @@ namespace exw {
 void set_last_error(const std::string &msg) { g_err = msg; }
 }  // namespace exw
 
+// Scratch of exw_l2_errors_noalloc: the per-block partial sums (3 per block)
+// and the three totals.  One copy per GPU, held until the process ends; host
+// code never uses the array's name as an address, only the per-device
+// pointer from hipGetSymbolAddress.
+#define EXW_NOALLOC_MAX_PARTIALS 1024
+static __device__ __align__(256) double scratch[3 * EXW_NOALLOC_MAX_PARTIALS + 4];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+// the device address of `scratch` on the current device (looked up once per
+// device, with that device current)
+int scratch_on_device(int device, double **out) {
+  if (device < 0 || device >= kMaxDevices) return fail(EXW_EINVAL, "device id out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(EXW_EDEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = (double *)p;
+  }
+  *out = g_scratch_ptr[device];
+  return EXW_OK;
+}
+
+__global__ void sqrt3_kernel(const double *__restrict__ sums, double *__restrict__ out) {
+  if (threadIdx.x < 3) out[threadIdx.x] = sqrt(sums[threadIdx.x]);
+}
+}  // namespace
 
 extern "C" {
 
@@ -58,9 +91,12 @@ int exw_ctx_create(int device, void *stream, exw_ctx *out) {
   EXW_HIP(hipGetDeviceCount(&n));
   if (device < 0 || device >= n) return fail(EXW_EINVAL, "exw_ctx_create: bad device id");
   EXW_HIP(hipSetDevice(device));
+  double *scr = nullptr;
+  if (int rc = scratch_on_device(device, &scr)) return rc;
   exw_ctx c = new exw_ctx_s;
   c->device = device;
   c->stream = (hipStream_t)stream;
+  c->scratch = scr;
   *out = c;
   return EXW_OK;
 }
@@ -893,6 +929,30 @@ int exw_l2_errors(exw_op op, const exw_modes *modes, double t, const double *u,
   EXW_HIP(hipMemcpyAsync(s, op->d_sum, sizeof(s), hipMemcpyDeviceToHost, op->ctx->stream));
   EXW_HIP(hipStreamSynchronize(op->ctx->stream));
   for (int k = 0; k < 3; ++k) out_host[k] = std::sqrt(s[k]);
+  return EXW_OK;
+}
+
+int exw_l2_errors_noalloc(exw_op op, const exw_modes *modes, double t, const double *u,
+                          double *out_dev) {
+  if (!op || !modes || !u || !out_dev)
+    return fail(EXW_EINVAL, "exw_l2_errors_noalloc: null argument");
+  if (modes->n_modes < 0 || modes->n_modes > 8) return fail(EXW_EINVAL, "n_modes must be 0..8");
+  const int nb = op->n_partial;  // the block count of exw_l2_errors: same sums
+  if (nb < 1 || nb > EXW_NOALLOC_MAX_PARTIALS)
+    return fail(EXW_EINVAL, "exw_l2_errors_noalloc: " + std::to_string(nb) +
+                                " partial blocks exceed the scratch array (" +
+                                std::to_string(EXW_NOALLOC_MAX_PARTIALS) + ")");
+  double *scr = op->ctx->scratch;
+  if (!scr) return fail(EXW_EDEVICE, "exw_l2_errors_noalloc: no scratch on this context");
+  // every one of the 3*nb partial slots is written by its block, then summed
+  // in the fixed order of exw_l2_errors into scr[3*nb .. 3*nb+2]
+  EXW_HIP(kernels(op->dim).l2err[op->degree](op->view(), op->d_errtab, to_dev(modes), t, u, scr,
+                                              nb, op->ctx->stream));
+  hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(256), 0, op->ctx->stream, scr, nb, 3,
+                     scr + 3 * nb);
+  EXW_HIP(hipGetLastError());
+  hipLaunchKernelGGL(sqrt3_kernel, dim3(1), dim3(64), 0, op->ctx->stream, scr + 3 * nb, out_dev);
+  EXW_HIP(hipGetLastError());
   return EXW_OK;
 }
 

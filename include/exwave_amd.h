@@ -189,6 +189,17 @@ int exw_l2_errors(exw_op op, const exw_modes *modes, double t, const double *u,
                   double *out_host);
 /* E = 1/2 int rho|v|^2 + p^2/(rho c^2) with QGauss(k+1); synchronises. */
 int exw_energy(exw_op op, const double *u, double *out_host);
+/* exw_l2_errors without allocating and without synchronising: the same three
+ * norms (same kernels, same summation order) written to out_dev (device, 3
+ * doubles) in stream order.  No counterpart in the reference: for callers that
+ * cannot allocate device memory or wait -- code under HIP graph capture, tight
+ * serving loops.  The partial sums live in one static device array per GPU
+ * (3*1024+4 doubles, ~24.6 KB, held until the process ends); the call rejects
+ * an operator whose partial-sum count exceeds it.  Every call on a GPU shares
+ * that array: calls that overlap on one GPU (different streams or host
+ * threads) race and must be ordered by the caller. */
+int exw_l2_errors_noalloc(exw_op op, const exw_modes *modes, double t, const double *u,
+                          double *out_dev);
 
 /* ---- point source ---------------------------------------------------------
  * No counterpart in the reference (the two-material refined-box workload):
