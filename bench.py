@@ -72,6 +72,8 @@ def parse():
     ap.add_argument("--nonaffine", action="store_true",
                     help="non-affine 32^3/48^3, k=4/6, FP32 vs FP64 operator (errors + GDoF/s)")
     ap.add_argument("--nonaffine-steps", type=int, default=40)
+    ap.add_argument("--nonaffine-case", action="append", default=None,
+                    help="degree,cells_per_dir,map (axis|product): only this case (repeatable)")
     ap.add_argument("--ader", action="store_true",
                     help="the CK kernel and the ADER step on the membrane workloads")
     ap.add_argument("--ader-case", action="append", default=None,
@@ -170,7 +172,7 @@ def copy_calibration():
                                            "launches": 5}}), flush=True)
 
 
-def nonaffine_measure(steps):
+def nonaffine_measure(steps, cases=None):
     """Non-affine 3D membranes at 32^3 and 48^3 cells, k = 4 and 6: LSRK45 with
     the FP64 operator and with the FP32 operator (FP64 vectors, RK updates and
     norms) from the same projected initial field.
@@ -190,39 +192,49 @@ def nonaffine_measure(steps):
     stream = torch.cuda.current_stream()
     modes = X.Modes.make(DIM, [[1, 1, 1], [2, 3, 1]])
     out = []
-    for k in (4, 6):
-        for n1d in (32, 48):
-            for tname, tr in (("axis", X.TRANSFORM_AXIS_2PI), ("product", X.TRANSFORM_PRODUCT_2PI)):
-                op = X.WaveOperator.hypercube(ctx, DIM, k, n1d, 0, tr, 0.05, 2)
-                dt = 0.2 / k ** 1.5 * op.min_vertex_distance_over_c
-                rec = {"degree": k, "cells_per_dir": n1d, "map": tname, "n_dofs": op.n_dofs,
-                       "steps": steps, "dt": dt}
-                finals = {}
-                for pname, prec in (("fp64", X.FP64), ("fp32", X.FP32)):
-                    a, b, w = op.new_vector(), op.new_vector(), op.new_vector()
-                    op.project(modes, 0.0, b)
-                    ev0 = torch.cuda.Event(enable_timing=True)
-                    ev1 = torch.cuda.Event(enable_timing=True)
-                    ev0.record(stream)
-                    for _ in range(steps):
-                        a, b = b, a
-                        op.time_step(X.LSRK45R2, dt, a, b, w, precision=prec)
-                    ev1.record(stream)
-                    torch.cuda.synchronize()
-                    ms = ev0.elapsed_time(ev1)
-                    rec[f"gdofs_{pname}"] = round(op.n_dofs * 5 * steps / (ms / 1e3) / 1e9, 2)
-                    if tname == "product":
-                        e = op.l2_errors(modes, steps * dt, b)
-                        rec[f"err_p_{pname}"], rec[f"err_v_{pname}"] = e[0], e[1]
-                    finals[pname] = b
-                    del a, w
-                d = finals["fp32"] - finals["fp64"]
-                rec["fp32_vs_fp64_rel_l2"] = float(torch.linalg.vector_norm(d) /
-                                                   torch.linalg.vector_norm(finals["fp64"]))
-                del finals, d
-                torch.cuda.empty_cache()
-                out.append(rec)
-                print(json.dumps(rec), file=sys.stderr, flush=True)
+    maps = {"axis": X.TRANSFORM_AXIS_2PI, "product": X.TRANSFORM_PRODUCT_2PI}
+    todo = cases or [(k, n1d, t) for k in (4, 6) for n1d in (32, 48) for t in ("axis", "product")]
+    for k, n1d, tname in todo:
+        tr = maps[tname]
+        op = X.WaveOperator.hypercube(ctx, DIM, k, n1d, 0, tr, 0.05, 2)
+        dt = 0.2 / k ** 1.5 * op.min_vertex_distance_over_c
+        rec = {"degree": k, "cells_per_dir": n1d, "map": tname, "n_dofs": op.n_dofs,
+               "steps": steps, "dt": dt}
+        finals = {}
+        for pname, prec in (("fp64", X.FP64), ("fp32", X.FP32)):
+            a, b, w = op.new_vector(), op.new_vector(), op.new_vector()
+            op.project(modes, 0.0, b)
+            ev0 = torch.cuda.Event(enable_timing=True)
+            ev1 = torch.cuda.Event(enable_timing=True)
+            ev0.record(stream)
+            for _ in range(steps):
+                a, b = b, a
+                op.time_step(X.LSRK45R2, dt, a, b, w, precision=prec)
+            ev1.record(stream)
+            torch.cuda.synchronize()
+            ms = ev0.elapsed_time(ev1)
+            rec[f"gdofs_{pname}"] = round(op.n_dofs * 5 * steps / (ms / 1e3) / 1e9, 2)
+            # algorithmic bytes per stage: 28.8 B/DoF of FP64 vectors plus the
+            # cell's 8 vertices, neighbour row and material (248 B)
+            stage_bytes = 28.8 * op.n_dofs + 248 * op.n_cells
+            gbs = stage_bytes * 5 * steps / (ms / 1e3) / 1e9
+            rec[f"GBps_{pname}"] = round(gbs, 1)
+            rec[f"frac_hbm_peak_{pname}"] = round(gbs / PEAK_HBM_GBS, 4)
+            if tname == "product":
+                e = op.l2_errors(modes, steps * dt, b)
+                rec[f"err_p_{pname}"], rec[f"err_v_{pname}"] = e[0], e[1]
+            finals[pname] = b
+            del a, w
+        d = finals["fp32"] - finals["fp64"]
+        rec["fp32_vs_fp64_rel_l2"] = float(torch.linalg.vector_norm(d) /
+                                           torch.linalg.vector_norm(finals["fp64"]))
+        del finals, d
+        torch.cuda.empty_cache()
+        out.append(rec)
+        print(json.dumps(rec), file=sys.stderr, flush=True)
+    if cases:
+        print(json.dumps({"nonaffine": out}), flush=True)
+        return
     # oracle cross-check of the literal map at 32^3, k=4 (two steps)
     import oracle as O
     k, n1d, st = 4, 32, 2
@@ -603,7 +615,9 @@ def main():
         copy_calibration()
         return
     if args.nonaffine:
-        nonaffine_measure(args.nonaffine_steps)
+        nonaffine_measure(args.nonaffine_steps,
+                          [(int(c.split(",")[0]), int(c.split(",")[1]), c.split(",")[2])
+                           for c in args.nonaffine_case] if args.nonaffine_case else None)
         return
     if args.refined:
         refined_measure(args.refined_time)

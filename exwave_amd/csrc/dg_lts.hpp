@@ -41,7 +41,28 @@ struct LtsArgs {
   int use_fm;                   // UPDATE: add and clear the flux memory
 };
 
-template <int DIM, int K>
+// On box cells (CART) the Jacobian is diag(h) (mesh.h) instead of the
+// multilinear map of the corners.
+template <int DIM, bool CART>
+__device__ __forceinline__ double lts_jacobian(const double *V, const double *h, const double *xi,
+                                               double *J, double *Ji) {
+  if constexpr (CART) {
+    double det = 1.0;
+#pragma unroll
+    for (int k = 0; k < DIM * DIM; ++k) J[k] = Ji[k] = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      J[d * DIM + d] = h[d];
+      Ji[d * DIM + d] = 1.0 / h[d];
+      det *= h[d];
+    }
+    return det;
+  } else {
+    return multilinear_jacobian<DIM>(V, xi, J, Ji);
+  }
+}
+
+template <int DIM, int K, bool CART>
 __global__ void __launch_bounds__((Shape<DIM, K>::BLOCK))
     dg_lts_kernel(const DevMesh mesh, const double *__restrict__ tab, const LtsArgs a) {
   using S = Shape<DIM, K>;
@@ -72,14 +93,22 @@ __global__ void __launch_bounds__((Shape<DIM, K>::BLOCK))
   const size_t cbase = (size_t)cell * NF * ND;
 
   for (int i = tid; i < TO::SIZE; i += blockDim.x) s_tab[i] = tab[i];
-  for (int i = tid; i < CPB * NVD; i += blockDim.x) {
-    const int e2 = blockIdx.x * CPB + i / NVD;
-    s_V[i / NVD][i % NVD] = e2 < a.n ? mesh.vertices[(size_t)a.cells[e2] * NVD + i % NVD] : 0.0;
+  if constexpr (!CART) {
+    for (int i = tid; i < CPB * NVD; i += blockDim.x) {
+      const int e2 = blockIdx.x * CPB + i / NVD;
+      s_V[i / NVD][i % NVD] = e2 < a.n ? mesh.vertices[(size_t)a.cells[e2] * NVD + i % NVD] : 0.0;
+    }
   }
-  double rho = 1.0, cc = 1.0;
+  double rho = 1.0, cc = 1.0, hc[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) hc[d] = 1.0;
   if (active) {
     rho = mesh.rho[cell];
     cc = mesh.speed[cell];
+    if constexpr (CART) {
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) hc[d] = mesh.h[(size_t)cell * DIM + d];
+    }
     if (q < NFACE) s_nbr[lc][q] = mesh.neighbor[(size_t)cell * NFACE + q];
   }
   const double rho_inv = 1.0 / rho, rc2 = rho * cc * cc;
@@ -108,7 +137,7 @@ __global__ void __launch_bounds__((Shape<DIM, K>::BLOCK))
       xi[d] = X[qi[d]];
       wq *= W[qi[d]];
     }
-    const double det = multilinear_jacobian<DIM>(s_V[lc], xi, J, Ji);
+    const double det = lts_jacobian<DIM, CART>(s_V[lc], hc, xi, J, Ji);
     jxw_node = wq * det;
     if (do_vol) {
       // evaluate_cell (:424-456): strong -rho^-1 grad p, test gradient of p
@@ -190,7 +219,7 @@ __global__ void __launch_bounds__((Shape<DIM, K>::BLOCK))
 #pragma unroll
         for (int e2 = 0; e2 < DIM; ++e2) xi[e2] = e2 == d ? double(s) : X[tdig[j++]];
       }
-      const double det = multilinear_jacobian<DIM>(s_V[lc], xi, J, Ji);
+      const double det = lts_jacobian<DIM, CART>(s_V[lc], hc, xi, J, Ji);
       double nrm[DIM], na = 0.0;
 #pragma unroll
       for (int i = 0; i < DIM; ++i) {

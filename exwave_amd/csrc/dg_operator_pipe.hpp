@@ -30,6 +30,11 @@
 // LDS of one workgroup's two buffers at most: gfx950's 160 KB (3D k=5 runs
 // 2 groups per CU, k=6 one; the host launches the latter nowhere, as the
 // one-cell kernel was measured faster there)
+// the accumulator's DMA with the non-temporal cache policy (1) or not (0)
+#ifndef EXW_PIPE_ACC_NT
+#define EXW_PIPE_ACC_NT 1
+#endif
+
 #ifndef EXW_PIPE_LDS_MAX
 #define EXW_PIPE_LDS_MAX 163840
 #endif
@@ -53,6 +58,20 @@ __device__ __forceinline__ void dma_dword(uint32_t m0, const void *base, uint32_
 // 64 lanes x 16 bytes: global base + voff -> LDS m0 + 16 * lane (gfx950)
 __device__ __forceinline__ void dma_dwordx4(uint32_t m0, const void *base, uint32_t voff) {
   asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, %2" ::"s"(m0), "v"(voff),
+               "s"(base)
+               : "memory");
+}
+
+// the same with the non-temporal cache policy: data read once (the RK
+// accumulator) is marked to leave the L2 to the state vector, whose nodes
+// the neighbouring cells read again
+__device__ __forceinline__ void dma_dwordx4_nt(uint32_t m0, const void *base, uint32_t voff) {
+  asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, %2 nt" ::"s"(m0), "v"(voff),
+               "s"(base)
+               : "memory");
+}
+__device__ __forceinline__ void dma_dword_nt(uint32_t m0, const void *base, uint32_t voff) {
+  asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dword %1, %2 nt" ::"s"(m0), "v"(voff),
                "s"(base)
                : "memory");
 }
@@ -397,10 +416,17 @@ __global__ void __launch_bounds__((PipeShape<DIM, K>::BLOCK)) __attribute__((amd
           g = own;
         }
         if (lane < op.lanes) {
-          if constexpr (op.x4)
-            pipe::dma_dwordx4(lbase + op.loff, base, lane16 + (g + op.goff));
-          else
-            pipe::dma_dword(lbase + op.loff, base, lane4 + (g + op.goff));
+          if constexpr (op.src == DMA_ACC && EXW_PIPE_ACC_NT) {
+            if constexpr (op.x4)
+              pipe::dma_dwordx4_nt(lbase + op.loff, base, lane16 + (g + op.goff));
+            else
+              pipe::dma_dword_nt(lbase + op.loff, base, lane4 + (g + op.goff));
+          } else {
+            if constexpr (op.x4)
+              pipe::dma_dwordx4(lbase + op.loff, base, lane16 + (g + op.goff));
+            else
+              pipe::dma_dword(lbase + op.loff, base, lane4 + (g + op.goff));
+          }
         }
       });
     });

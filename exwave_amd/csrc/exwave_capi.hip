@@ -340,9 +340,10 @@ static int op_create(exw_ctx ctx, int dim, int degree, int n_cells, const double
   if (!ctx || !out || !vertices || !neighbor || !rho || !speed)
     return fail(EXW_EINVAL, "exw_op_create: null argument");
   if (dim != 2 && dim != 3) return fail(EXW_EINVAL, "exw_op_create: dim must be 2 or 3");
-  if (degree < 1 || degree > MAX_DEGREE)
+  if (degree < 1 || degree > max_degree(dim))
     return fail(EXW_EUNSUP, "exw_op_create: degree " + std::to_string(degree) +
-                                " not compiled (1.." + std::to_string(MAX_DEGREE) + ")");
+                                " not compiled (1.." + std::to_string(max_degree(dim)) + " in " +
+                                std::to_string(dim) + "D)");
   if (n_cells < 1) return fail(EXW_EINVAL, "exw_op_create: n_cells < 1");
   const int nv = 1 << dim, nface = 2 * dim, N = degree + 1;
   // validate connectivity, face orientation and materials
@@ -1149,9 +1150,11 @@ int exw_post_error(exw_op op, const exw_modes *modes, double t, const double *po
                    double *out_host) {
   if (!op || !modes || !post || !out_host) return fail(EXW_EINVAL, "exw_post_error: null argument");
   if (modes->n_modes < 0 || modes->n_modes > 8) return fail(EXW_EINVAL, "n_modes must be 0..8");
+  PostErrLaunch l = kernels(op->dim).post_error[op->degree];
+  if (!l) return fail(EXW_EUNSUP, "post-processing error kernel not compiled");
   const int nb = op->n_partial;
-  EXW_HIP(kernels(op->dim).post_error[op->degree](op->view(), op->d_posttab, to_dev(modes), t,
-                                                   post, op->d_partial, nb, op->ctx->stream));
+  EXW_HIP(l(op->view(), op->d_posttab, to_dev(modes), t, post, op->d_partial, nb,
+            op->ctx->stream));
   hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(256), 0, op->ctx->stream, op->d_partial,
                      nb, 1, op->d_sum);
   EXW_HIP(hipGetLastError());

@@ -394,7 +394,8 @@ int launch_sweep(exw_lts L, const List &l, int mode, const double *src, double f
   a.acc1 = L->fm;
   a.out = out;
   a.use_fm = use_fm ? 1 : 0;
-  LHIP(kernels(op->dim).lts[op->degree](op->view(), op->d_tab, a, op->ctx->stream));
+  LHIP(kernels(op->dim).lts[op->degree][op->cart ? 1 : 0](op->view(), op->d_tab, a,
+                                                          op->ctx->stream));
   return EXW_OK;
 }
 
@@ -435,8 +436,15 @@ int run_update(exw_lts L, const UpdatePlan &P, double *state, const double tside
   if ((rc = launch_sweep(L, P.same.face, LTS_ACC, L->tempsrc, -1.0, nullptr, false, true)))
     return rc;
   if ((rc = launch_nc_list(L, P.same.nc[0], L->tempsrc, -1.0, L->fsum))) return rc;
-  if ((rc = launch_ck_list(L, P.upd, state, dt, 0.0, false, L->V))) return rc;
-  if ((rc = launch_sweep(L, P.upd, LTS_UPDATE, L->V, -1.0, state, use_fm, false))) return rc;
+  // the update's volume term needs the CK integral over [0, dt] about the
+  // cells' own time: the same-cluster pass just computed it over [ts0, ts1]
+  // about ts0 for every update cell, the same integral when ts1 - ts0 = dt
+  const double *vsrc = L->tempsrc;
+  if (!(std::fabs((ts[1] - ts[0]) - dt) <= 1e-14 * std::fabs(dt))) {
+    if ((rc = launch_ck_list(L, P.upd, state, dt, 0.0, false, L->V))) return rc;
+    vsrc = L->V;
+  }
+  if ((rc = launch_sweep(L, P.upd, LTS_UPDATE, vsrc, -1.0, state, use_fm, false))) return rc;
   return source_add_selected(L->op, t_cells, t_cells + dt, P.src_sel, state);
 }
 

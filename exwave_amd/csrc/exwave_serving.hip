@@ -64,7 +64,7 @@ int solve_steps(const exw_request &r, double &dt, double &t_end) {
 }
 
 int check_request(const exw_request &r) {
-  if ((r.dim != 2 && r.dim != 3) || r.degree < 1 || r.degree > MAX_DEGREE ||
+  if ((r.dim != 2 && r.dim != 3) || r.degree < 1 || r.degree > max_degree(r.dim) ||
       r.n_refinements < 0 || r.n_refinements > 10 || !(r.rho > 0) || !(r.c > 0) ||
       !(r.final_time > 0) || !(r.cfl_number > 0) || r.modes.n_modes < 0 || r.modes.n_modes > 8 ||
       (r.precision != EXW_FP64 && r.precision != EXW_FP32 && r.precision != EXW_PRECISION_AUTO))

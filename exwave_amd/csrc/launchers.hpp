@@ -10,13 +10,19 @@
 #include "dg_misc.hpp"
 #include "dg_operator.hpp"
 #include "dg_operator_cart.hpp"
+#include "dg_operator_gen.hpp"
 #include "dg_operator_pipe.hpp"
 #include "dg_nc.hpp"
 #include "dg_post.hpp"
 
 namespace exw {
 
-constexpr int MAX_DEGREE = 8;
+// Degrees compiled: 1..12 in 2D (the reference's range,
+// source/wave_equation_operations.cc:1695-1802), 1..9 in 3D, where one lane
+// per node still fits a 1024-lane workgroup (3D k >= 10 has 1331+ nodes).
+constexpr int MAX_DEGREE = 12;
+constexpr int MAX_DEGREE_3D = 9;
+inline int max_degree(int dim) { return dim == 2 ? MAX_DEGREE : MAX_DEGREE_3D; }
 
 // (mesh, src, device basis table, host basis table, epilogue, stream)
 using OpLaunch = hipError_t (*)(const DevMesh &, const double *, const double *, const double *,
@@ -54,7 +60,7 @@ struct DimKernels {
   CoefLaunch coef;
   OpLaunch op[MAX_DEGREE + 1][2][2];  // [k][fp32][cart]
   CkLaunch ck[MAX_DEGREE + 1][2][2];
-  LtsLaunch lts[MAX_DEGREE + 1];
+  LtsLaunch lts[MAX_DEGREE + 1][2];  // [k][cart]
   ProjLaunch project[MAX_DEGREE + 1];
   ErrLaunch l2err[MAX_DEGREE + 1];
   EnergyLaunch energy[MAX_DEGREE + 1];
@@ -74,6 +80,16 @@ struct DimKernels {
 inline bool cart_pipe_enabled() {
   static const bool on = [] {
     const char *v = getenv("EXW_CART_PIPE");
+    return !(v && v[0] == '0');
+  }();
+  return on;
+}
+
+// The general-geometry kernel of one cell per group (dg_operator_gen_kernel)
+// where it applies; EXW_GEN_KERNEL=0 selects dg_operator_kernel throughout.
+inline bool gen_kernel_enabled() {
+  static const bool on = [] {
+    const char *v = getenv("EXW_GEN_KERNEL");
     return !(v && v[0] == '0');
   }();
   return on;
@@ -135,6 +151,15 @@ hipError_t launch_op(const DevMesh &m, const double *src, const double *tab,
   } else {
     using S = Shape<DIM, K>;
     if (e.cell_list) return hipErrorInvalidValue;  // cell lists: Cartesian kernel only
+    if constexpr (GenShape<DIM, K>::ENABLED) {
+      if (gen_kernel_enabled()) {
+        // one cell per group (node/face/node, geometry computed once per cell)
+        if (m.n_cells == 0) return hipSuccess;
+        hipLaunchKernelGGL((dg_operator_gen_kernel<DIM, K, Number>), dim3(m.n_cells),
+                           dim3(GenShape<DIM, K>::BLOCK), 0, s, m, src, tab, e);
+        return hipGetLastError();
+      }
+    }
     const int grid = (m.n_cells + S::CPB - 1) / S::CPB;
     if (grid == 0) return hipSuccess;
     hipLaunchKernelGGL((dg_operator_kernel<DIM, K, Number, CART>), dim3(grid), dim3(S::BLOCK), 0,
@@ -158,12 +183,12 @@ hipError_t launch_ck(const DevMesh &m, const double *src, const double *recon,
   return hipGetLastError();
 }
 
-template <int DIM, int K>
+template <int DIM, int K, bool CART>
 hipError_t launch_lts(const DevMesh &m, const double *tab, const LtsArgs &a, hipStream_t s) {
   using S = Shape<DIM, K>;
   const int grid = (a.n + S::CPB - 1) / S::CPB;
   if (grid == 0) return hipSuccess;
-  hipLaunchKernelGGL((dg_lts_kernel<DIM, K>), dim3(grid), dim3(S::BLOCK), 0, s, m, tab, a);
+  hipLaunchKernelGGL((dg_lts_kernel<DIM, K, CART>), dim3(grid), dim3(S::BLOCK), 0, s, m, tab, a);
   return hipGetLastError();
 }
 
@@ -213,6 +238,14 @@ hipError_t launch_nc(const DevMesh &m, const double *tab, const NcArgs &nc, cons
   return hipGetLastError();
 }
 
+// the post-processing kernel puts one lane on each of the (k+2)^dim
+// Gauss-Lobatto nodes of a cell (3D k = 9: 1331 > 1024 lanes, not compiled)
+template <int DIM, int K>
+constexpr bool post_fits() {
+  constexpr int P = K + 2;
+  return (DIM == 2 ? P * P : P * P * P) <= 1024;
+}
+
 template <int DIM, int K>
 hipError_t launch_post(const DevMesh &m, const double *tab, const double *sol, const double *dudt,
                        double *post, int *iterations, const PostSolver &cfg, hipStream_t s) {
@@ -244,14 +277,15 @@ void fill_dim_kernels(DimKernels &t) {
     t.ck[K][0][1] = launch_ck<DIM, K, double, true>;
     t.ck[K][1][0] = launch_ck<DIM, K, float, false>;
     t.ck[K][1][1] = launch_ck<DIM, K, float, true>;
-    t.lts[K] = launch_lts<DIM, K>;
+    t.lts[K][0] = launch_lts<DIM, K, false>;
+    t.lts[K][1] = launch_lts<DIM, K, true>;
     t.project[K] = launch_project<DIM, K>;
     t.l2err[K] = launch_l2err<DIM, K>;
     t.energy[K] = launch_energy<DIM, K>;
-    t.post[K] = launch_post<DIM, K>;
+    if constexpr (post_fits<DIM, K>()) t.post[K] = launch_post<DIM, K>;
     t.nc[K] = launch_nc<DIM, K>;
     t.estimate[K] = launch_estimate<DIM, K>;
-    t.post_error[K] = launch_post_error<DIM, K>;
+    if constexpr (post_fits<DIM, K>()) t.post_error[K] = launch_post_error<DIM, K>;
     t.block[K] = Shape<DIM, K>::BLOCK;
     t.cells_per_block[K] = Shape<DIM, K>::CPB;
     t.pipe[K] = PipeShape<DIM, K>::ENABLED && (DIM == 2 || K <= 5);
