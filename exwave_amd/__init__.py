@@ -19,7 +19,8 @@ from .capi import (ADER, ADER_ADCONFULL, ADER_LTS, CLRK4, EXPL_EULER, FP32, FP64
                    PRECISION_AUTO, INTEGRATORS, LSRK33R2, LSRK45R2, LSRK45R3, SSPRK,
                    LSRK59R2, TRANSFORM_AXIS_2PI, TRANSFORM_NONE, TRANSFORM_PRODUCT_2PI,
                    TRANSFORM_REFERENCE_SINE, NC_FACE, Epilogue, ExwError, Modes, Problem, check,
-                   load, mesh_box_refined, mesh_hypercube, tile_order, write_vtu, Source,
+                   load, mesh_box_refined, mesh_forest, adapt_forest, mesh_hypercube,
+                   tile_order, write_vtu, Source,
                    LTS_SPEED_AWARE)
 
 
@@ -222,6 +223,17 @@ def adapt_box_transfer(ctx, dim, degree, n_intervals, refine_old, refine_new, ol
     ip = C.POINTER(C.c_int)
     check(load().exw_adapt_box_transfer(ctx.h, dim, degree, n_intervals, ro.ctypes.data_as(ip),
                                         rn.ctypes.data_as(ip), _ptr(old), _ptr(new)))
+
+
+def adapt_forest_transfer(ctx, dim, degree, n_intervals, tree_old, tree_new, old, new):
+    """exw_adapt_forest_transfer between two exw_mesh_forest meshes (device vectors)."""
+    import numpy as np
+    to = np.ascontiguousarray(tree_old, dtype=np.int32).ravel()
+    tn = np.ascontiguousarray(tree_new, dtype=np.int32).ravel()
+    ip = C.POINTER(C.c_int)
+    check(load().exw_adapt_forest_transfer(ctx.h, dim, degree, n_intervals, to.size,
+                                           to.ctypes.data_as(ip), tn.size, tn.ctypes.data_as(ip),
+                                           _ptr(old), _ptr(new)))
 
 
 def lts_categorize(dim, vertices, neighbor, cfl, max_clusters, max_diff):

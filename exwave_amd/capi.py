@@ -139,6 +139,9 @@ SIGNATURES = [
     ("exw_op_create_nc", _I, [_VP, _I, _I, _I, _DP, _IP, _I, _IP, _DP, _DP, C.POINTER(_VP)]),
     ("exw_mesh_check", _I, [_I, _I, _DP, _IP, _I, _IP]),
     ("exw_mesh_box_refined", _I, [_I, _I, _I, _D, _I, _IP, _IP, _DP, _IP, _IP, _IP]),
+    ("exw_mesh_forest", _I, [_I, _I, _I, _D, _I, _I, _IP, _IP, _DP, _IP, _IP, _IP, _IP]),
+    ("exw_adapt_forest", _I, [_I, _I, _I, _IP, _IP, _IP, _IP, _I]),
+    ("exw_adapt_forest_transfer", _I, [_VP, _I, _I, _I, _I, _IP, _I, _IP, _VP, _VP]),
     ("exw_post_pressure", _I, [_VP, _VP, _VP, _VP, _VP]),
     ("exw_post_error", _I, [_VP, C.POINTER(Modes), _D, _VP, _DP]),
     ("exw_mesh_hypercube", _I, [_I, _I, _I, _I, _D, _I, _IP, _DP, _IP]),
@@ -228,6 +231,44 @@ def mesh_box_refined(dim, n_intervals, refine, transform=TRANSFORM_NONE, factor=
                                  V.ctypes.data_as(C.POINTER(C.c_double)), nb.ctypes.data_as(ip),
                                  C.byref(nh), H.ctypes.data_as(ip)))
     return V, nb, H[: nh.value]
+
+
+def mesh_forest(dim, n_intervals, tree, transform=TRANSFORM_NONE, factor=0.0, boundary_id=2):
+    """exw_mesh_forest: (vertices, neighbor, level, hanging records) of the box
+    refined by the depth-first trees in `tree` (1 refined, 0 active)."""
+    import numpy as np
+    L = load()
+    t = np.ascontiguousarray(tree, dtype=np.int32).ravel()
+    ip = C.POINTER(C.c_int)
+    nc, nh = C.c_int(0), C.c_int(0)
+    check(L.exw_mesh_forest(dim, n_intervals, transform, factor, boundary_id, t.size,
+                            t.ctypes.data_as(ip), C.byref(nc), None, None, None, C.byref(nh),
+                            None))
+    V = np.zeros((nc.value, 1 << dim, dim))
+    nb = np.zeros((nc.value, 2 * dim), dtype=np.int32)
+    lev = np.zeros(nc.value, dtype=np.int32)
+    H = np.zeros((max(nh.value, 1), 2 + (1 << (dim - 1))), dtype=np.int32)
+    check(L.exw_mesh_forest(dim, n_intervals, transform, factor, boundary_id, t.size,
+                            t.ctypes.data_as(ip), C.byref(nc),
+                            V.ctypes.data_as(C.POINTER(C.c_double)), nb.ctypes.data_as(ip),
+                            lev.ctypes.data_as(ip), C.byref(nh), H.ctypes.data_as(ip)))
+    return V, nb, lev, H[: nh.value]
+
+
+def adapt_forest(dim, n_intervals, tree, flags):
+    """exw_adapt_forest: the depth-first trees after adapt_mesh's flags."""
+    import numpy as np
+    L = load()
+    t = np.ascontiguousarray(tree, dtype=np.int32).ravel()
+    f = np.ascontiguousarray(flags, dtype=np.int32).ravel()
+    ip = C.POINTER(C.c_int)
+    n = C.c_int(0)
+    check(L.exw_adapt_forest(dim, n_intervals, t.size, t.ctypes.data_as(ip),
+                             f.ctypes.data_as(ip), C.byref(n), None, 0))
+    out = np.zeros(n.value, dtype=np.int32)
+    check(L.exw_adapt_forest(dim, n_intervals, t.size, t.ctypes.data_as(ip),
+                             f.ctypes.data_as(ip), C.byref(n), out.ctypes.data_as(ip), out.size))
+    return out
 
 
 def mesh_hypercube(dim, n_intervals, n_refinements=0, transform=TRANSFORM_NONE, factor=0.0,

@@ -6,12 +6,14 @@
 
 #include <algorithm>
 #include <chrono>
+#include <functional>
 #include <cmath>
 #include <cstdio>
 #include <string>
 #include <vector>
 
 #include "../../include/exwave_amd.h"
+#include "forest.hpp"
 
 namespace {
 
@@ -186,6 +188,95 @@ int exw_mesh_box_refined(int dim, int n_intervals, int transform, double factor,
   return EXW_OK;
 }
 
+int exw_mesh_forest(int dim, int n_intervals, int transform, double factor, int boundary_id,
+                    int n_tree, const int *tree, int *n_cells, double *vertices, int *neighbor,
+                    int *level, int *n_hanging, int *hanging) {
+  if ((dim != 2 && dim != 3) || n_intervals < 1 || transform < 0 || transform > 3 ||
+      boundary_id < 1 || boundary_id > 3 || n_tree < 1 || !tree)
+    return EXW_EINVAL;
+  exw::Forest F;
+  int64_t used = 0;
+  if (!F.parse(dim, n_intervals, tree, n_tree, &used) || used != n_tree) return EXW_EINVAL;
+  const int nv = 1 << dim, nface = 2 * dim, nsub = 1 << (dim - 1);
+  const int64_t nc = (int64_t)F.cells.size();
+  if (nc * nv > 2000000000L) return EXW_EINVAL;
+  // pass 1 checks the faces and counts the hanging ones; pass 2 writes
+  const bool write = vertices && neighbor && hanging;
+  int64_t nh = 0;
+  for (int64_t c = 0; c < nc; ++c) {
+    const exw::ForestCell &X = F.cells[c];
+    const int l = X.level;
+    if (write) {
+      if (level) level[c] = l;
+      int64_t B[3] = {0, 0, 0};
+      for (int d = 0; d < dim; ++d) B[d] = X.I[d] >> l;
+      double PV[8][3];
+      for (int v = 0; v < nv; ++v) {
+        double x[3] = {0, 0, 0};
+        for (int d = 0; d < dim; ++d) x[d] = double(B[d] + ((v >> d) & 1)) / n_intervals;
+        transform_vertex(dim, transform, factor, x);
+        for (int d = 0; d < dim; ++d) PV[v][d] = x[d];
+      }
+      const double scale = 1.0 / double((int64_t)1 << l);
+      for (int v = 0; v < nv; ++v) {
+        double xi[3];
+        for (int d = 0; d < dim; ++d)
+          xi[d] = double(X.I[d] - (B[d] << l) + ((v >> d) & 1)) * scale;
+        double *out = vertices + (c * nv + v) * dim;
+        for (int i = 0; i < dim; ++i) out[i] = 0.0;
+        for (int u = 0; u < nv; ++u) {
+          double w = 1.0;
+          for (int d = 0; d < dim; ++d) w *= ((u >> d) & 1) ? xi[d] : 1.0 - xi[d];
+          for (int i = 0; i < dim; ++i) out[i] += w * PV[u][i];
+        }
+      }
+    }
+    for (int f = 0; f < nface; ++f) {
+      const int d = f >> 1, s = f & 1;
+      int64_t J[3] = {X.I[0], X.I[1], X.I[2]};
+      J[d] += s ? 1 : -1;
+      int nb;
+      if (J[d] < 0 || J[d] >= F.extent(l)) {
+        nb = -boundary_id;
+      } else {
+        const int r = F.find(l, J);
+        if (r >= 0) {
+          nb = r;  // conforming
+        } else if (r == -1) {
+          // coarse side: the neighbour's children touching the face must be active
+          int fine[4];
+          for (int b = 0; b < nsub; ++b) {
+            int64_t K[3] = {2 * J[0], 2 * J[1], 2 * J[2]};
+            K[d] += s ? 0 : 1;
+            int j = 0;
+            for (int e = 0; e < dim; ++e)
+              if (e != d) K[e] += (b >> j++) & 1;
+            fine[b] = F.find(l + 1, K);
+            if (fine[b] < 0) return EXW_EINVAL;  // more than one level across a face
+          }
+          if (write) {
+            int *rec = hanging + nh * (2 + nsub);
+            rec[0] = int(c);
+            rec[1] = f;
+            for (int b = 0; b < nsub; ++b) rec[2 + b] = fine[b];
+          }
+          ++nh;
+          nb = EXW_NC_FACE;
+        } else {
+          // fine side: the parent of the same-level neighbour must be active
+          int64_t P[3] = {J[0] >> 1, J[1] >> 1, J[2] >> 1};
+          if (l == 0 || F.find(l - 1, P) < 0) return EXW_EINVAL;
+          nb = EXW_NC_FACE;
+        }
+      }
+      if (write) neighbor[c * nface + f] = nb;
+    }
+  }
+  if (n_cells) *n_cells = int(nc);
+  if (n_hanging) *n_hanging = int(nh);
+  return EXW_OK;
+}
+
 namespace {
 
 // The mesh, operator and state vectors of a run; replaced by adapt_mesh.
@@ -195,6 +286,7 @@ struct RunState {
   double *buf = nullptr, *post = nullptr, *err = nullptr;
   double *sol = nullptr, *tmp = nullptr, *work = nullptr;
   int n_cells = 0;
+  std::vector<int> level;  // adaptive runs: the cells' refinement levels
   void release() {
     if (post) exw_free(ctx, post);
     if (err) exw_free(ctx, err);
@@ -205,9 +297,9 @@ struct RunState {
   }
 };
 
-// mesh (hypercube, or the box with refine[] when adaptive), operator and
+// mesh (hypercube, or the forest tree[] when adaptive), operator and
 // vectors; the solution is left uninitialised
-int run_setup(exw_ctx ctx, const exw_problem *p, bool adaptive, const std::vector<int> &refine,
+int run_setup(exw_ctx ctx, const exw_problem *p, bool adaptive, const std::vector<int> &tree,
               int nwork, RunState &st) {
   st.ctx = ctx;
   int n_cells = 0, n_hanging = 0, rc;
@@ -215,16 +307,18 @@ int run_setup(exw_ctx ctx, const exw_problem *p, bool adaptive, const std::vecto
   std::vector<double> V;
   std::vector<int> nb, H;
   if (adaptive) {
-    if ((rc = exw_mesh_box_refined(p->dim, p->n_intervals, p->transform, p->transform_factor,
-                                   p->boundary_id, refine.data(), &n_cells, nullptr, nullptr,
-                                   &n_hanging, nullptr)))
+    const int nt = (int)tree.size();
+    if ((rc = exw_mesh_forest(p->dim, p->n_intervals, p->transform, p->transform_factor,
+                              p->boundary_id, nt, tree.data(), &n_cells, nullptr, nullptr,
+                              nullptr, &n_hanging, nullptr)))
       return rc;
     V.resize((size_t)n_cells * nv * p->dim);
     nb.resize((size_t)n_cells * 2 * p->dim);
     H.resize((size_t)std::max(n_hanging, 1) * (2 + nv / 2));
-    if ((rc = exw_mesh_box_refined(p->dim, p->n_intervals, p->transform, p->transform_factor,
-                                   p->boundary_id, refine.data(), &n_cells, V.data(), nb.data(),
-                                   &n_hanging, H.data())))
+    st.level.resize(n_cells);
+    if ((rc = exw_mesh_forest(p->dim, p->n_intervals, p->transform, p->transform_factor,
+                              p->boundary_id, nt, tree.data(), &n_cells, V.data(), nb.data(),
+                              st.level.data(), &n_hanging, H.data())))
       return rc;
   } else {
     if ((rc = exw_mesh_hypercube(p->dim, p->n_intervals, p->n_refinements, p->transform,
@@ -261,17 +355,28 @@ int exw_run(exw_ctx ctx, const exw_problem *p, char *log, int log_len, double *w
   if (!ctx || !p) return EXW_EINVAL;
   const bool adaptive = p->n_adaptive_refinements > 0;
   const bool lts = p->integrator == EXW_ADER_LTS;
-  if (adaptive && (p->n_adaptive_refinements != 1 || p->n_refinements != 0 || lts ||
-                   p->adaptive_refinement_interval < 1))
-    return EXW_EUNSUP;  // one adaptive level on the coarse mesh, no LTS
+  if (adaptive && (lts || p->adaptive_refinement_interval < 1 || p->n_refinements < 0 ||
+                   p->n_refinements + p->n_adaptive_refinements > exw::Forest::MAX_LEVEL))
+    return EXW_EUNSUP;  // adaptivity without LTS
   const int nwork = lts ? 0 : exw_time_step_work_vectors(p->integrator);
   if (nwork < 0) return EXW_EINVAL;
   int nbase = 1;
   for (int d = 0; d < p->dim; ++d) nbase *= p->n_intervals;
-  std::vector<int> refine(adaptive ? nbase : 0, 0);
+  // the forest of the globally refined box: levels min_level = n_refinements
+  // .. max_level = min_level + n_adaptive_refinements (explicit_wave.cc:272-273)
+  const int min_level = p->n_refinements, max_level = min_level + p->n_adaptive_refinements;
+  std::vector<int> tree;
+  if (adaptive) {
+    std::function<void(int)> uniform = [&](int depth) {
+      tree.push_back(depth > 0);
+      if (depth > 0)
+        for (int ch = 0; ch < (1 << p->dim); ++ch) uniform(depth - 1);
+    };
+    for (int B = 0; B < nbase; ++B) uniform(min_level);
+  }
   RunState st;
   int rc;
-  if ((rc = run_setup(ctx, p, adaptive, refine, nwork, st))) {
+  if ((rc = run_setup(ctx, p, adaptive, tree, nwork, st))) {
     st.release();
     return rc;
   }
@@ -307,24 +412,27 @@ int exw_run(exw_ctx ctx, const exw_problem *p, char *log, int log_len, double *w
     if ((r = exw_estimate_error(st.op, st.sol, st.tmp, st.err))) return r;
     std::vector<double> err(st.n_cells);
     if ((r = exw_memcpy_d2h(ctx, err.data(), st.err, sizeof(double) * st.n_cells))) return r;
-    std::vector<int> lev, flags(st.n_cells), rnew(nbase);
-    for (int B = 0; B < nbase; ++B)
-      for (int ch = 0; ch < (refine[B] ? (1 << p->dim) : 1); ++ch) lev.push_back(refine[B]);
-    if ((r = exw_refinement_flags(st.n_cells, err.data(), lev.data(), 0, 1, max_err_init,
-                                  flags.data())) ||
-        (r = exw_adapt_box_refine(p->dim, p->n_intervals, refine.data(), flags.data(),
-                                  rnew.data())))
+    std::vector<int> flags(st.n_cells), tnew;
+    int ntn = 0;
+    if ((r = exw_refinement_flags(st.n_cells, err.data(), st.level.data(), min_level, max_level,
+                                  max_err_init, flags.data())) ||
+        (r = exw_adapt_forest(p->dim, p->n_intervals, (int)tree.size(), tree.data(),
+                              flags.data(), &ntn, nullptr, 0)))
+      return r;
+    tnew.resize(ntn);
+    if ((r = exw_adapt_forest(p->dim, p->n_intervals, (int)tree.size(), tree.data(),
+                              flags.data(), &ntn, tnew.data(), ntn)))
       return r;
     RunState nst;
-    if ((r = run_setup(ctx, p, true, rnew, nwork, nst)) ||
-        (r = exw_adapt_box_transfer(ctx, p->dim, p->degree, p->n_intervals, refine.data(),
-                                    rnew.data(), st.sol, nst.sol))) {
+    if ((r = run_setup(ctx, p, true, tnew, nwork, nst)) ||
+        (r = exw_adapt_forest_transfer(ctx, p->dim, p->degree, p->n_intervals, (int)tree.size(),
+                                       tree.data(), ntn, tnew.data(), st.sol, nst.sol))) {
       nst.release();
       return r;
     }
     st.release();
-    st = nst;
-    refine = rnew;
+    st = std::move(nst);
+    tree = std::move(tnew);
     dt = cfl * exw_op_min_vertex_distance_over_c(st.op);  // compute_time_step_size
     return EXW_OK;
   };
@@ -347,10 +455,12 @@ int exw_run(exw_ctx ctx, const exw_problem *p, char *log, int log_len, double *w
   double wall = 0.0;
   rc = exw_project(st.op, &p->modes, 0.0, st.sol);
   if (!rc && adaptive) {
-    // the initial adaptation (explicit_wave.cc:704-716): adapt, project again,
-    // then the largest indicator of the projected field
-    if (!(rc = adapt()) && !(rc = exw_project(st.op, &p->modes, 0.0, st.sol)) &&
-        !(rc = exw_estimate_error(st.op, st.sol, st.tmp, st.err))) {
+    // the initial adaptation (explicit_wave.cc:704-716): n_adaptive_refinements
+    // times adapt and project again, then the largest indicator of the
+    // projected field
+    for (int left = p->n_adaptive_refinements; left > 0 && !rc; --left)
+      if (!(rc = adapt())) rc = exw_project(st.op, &p->modes, 0.0, st.sol);
+    if (!rc && !(rc = exw_estimate_error(st.op, st.sol, st.tmp, st.err))) {
       std::vector<double> err(st.n_cells);
       if (!(rc = exw_memcpy_d2h(ctx, err.data(), st.err, sizeof(double) * st.n_cells))) {
         max_err_init = 0.0;

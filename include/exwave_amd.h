@@ -308,6 +308,41 @@ int exw_adapt_box_transfer(exw_ctx ctx, int dim, int degree, int n_intervals,
                            const int *refine_old, const int *refine_new,
                            const double *old_solution, double *new_solution);
 
+/* ---- several levels of adaptive refinement (adapt_mesh,
+ * source/explicit_wave.cc:265-330, levels n_refinements ..
+ * n_refinements + n_adaptive_refinements, :272-273) -------------------------
+ * A forest of refinement trees over the n_intervals^dim box of
+ * exw_mesh_hypercube: tree[] lists the base cells lexicographically, each
+ * depth first -- 1 = refined (its 2^dim children follow, lexicographic), 0 =
+ * an active cell.  Active cells are numbered in that order; a cell's level is
+ * its depth.  Children are the parent's multilinear map at dyadic points, as
+ * deal.II refines straight cells.  The one-level box meshes above are the
+ * forests of depth <= 1 with the same cell numbering.
+ *
+ * exw_mesh_forest: vertices, neighbours, levels (level may be NULL) and the
+ * hanging-face records of exw_op_create_nc.  Call with vertices = neighbor =
+ * hanging = NULL for n_cells and n_hanging.  EXW_EINVAL for malformed trees
+ * or faces whose cells differ by more than one level. */
+int exw_mesh_forest(int dim, int n_intervals, int transform, double factor, int boundary_id,
+                    int n_tree, const int *tree, int *n_cells, double *vertices, int *neighbor,
+                    int *level, int *n_hanging, int *hanging);
+/* Triangulation::execute_coarsening_and_refinement of p4est-backed deal.II
+ * (parallel::distributed::Triangulation) after adapt_mesh's flags
+ * (flags[c] = 1 refine, -1 coarsen, 0 keep, per active cell): a family is
+ * coarsened only when all its children are active and flagged; the mesh stays
+ * 2:1 balanced across faces, edges and corners -- the coarser cell of a
+ * violation is refined until none is left, and a coarsening that would create
+ * one is not done.  Writes the new trees into tree_new (capacity entries;
+ * NULL: only *n_tree_new). */
+int exw_adapt_forest(int dim, int n_intervals, int n_tree, const int *tree, const int *flags,
+                     int *n_tree_new, int *tree_new, int capacity);
+/* SolutionTransfer::interpolate between two forests of exw_adapt_forest
+ * (every cell kept, refined once or coarsened once), the rules of
+ * exw_adapt_box_transfer, on device vectors. */
+int exw_adapt_forest_transfer(exw_ctx ctx, int dim, int degree, int n_intervals, int n_tree_old,
+                              const int *tree_old, int n_tree_new, const int *tree_new,
+                              const double *old_solution, double *new_solution);
+
 /* output_results (source/explicit_wave.cc:339-436) as a VTU file, host arrays:
  * each cell subdivided degree times per direction (build_patches(mapping,
  * fe_degree)), the fields evaluated at the equispaced points and written as
@@ -386,9 +421,10 @@ typedef struct exw_problem {
   exw_modes modes;     /* initial condition and exact solution */
   int lts_max_clusters, lts_max_diff; /* EXW_ADER_LTS: max_n_clusters, max_diff_clusters */
   int post_pressure;   /* also report 'error p post' at every tick (else -1) */
-  /* adapt_mesh every adaptive_refinement_interval steps and once before the
-   * loop (explicit_wave.cc:704-716, 789-791): 0, or 1 with n_refinements = 0
-   * (one level on the n_intervals^dim mesh; not with EXW_ADER_LTS) */
+  /* adapt_mesh every adaptive_refinement_interval steps and
+   * n_adaptive_refinements times before the loop (explicit_wave.cc:704-716,
+   * 789-791) on the forest of exw_mesh_forest, levels n_refinements ..
+   * n_refinements + n_adaptive_refinements; not with EXW_ADER_LTS */
   int n_adaptive_refinements, adaptive_refinement_interval;
 } exw_problem;
 

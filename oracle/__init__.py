@@ -163,6 +163,14 @@ def lib():
         L.ox_gauss_lobatto.argtypes = [C.c_int, dp]
         L.ox_estimate_error.argtypes = [C.POINTER(_Mesh), C.c_int, dp, dp]
         L.ox_adapt_transfer.argtypes = [C.c_int, C.c_int, C.c_int, ip, ip, dp, dp]
+        L.ox_forest_levels.restype = C.c_int
+        L.ox_forest_levels.argtypes = [C.c_int, C.c_int, ip, C.c_int, ip]
+        L.ox_mesh_forest.restype = C.POINTER(_Mesh)
+        L.ox_mesh_forest.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, ip, C.c_int]
+        L.ox_adapt_forest.restype = C.c_int
+        L.ox_adapt_forest.argtypes = [C.c_int, C.c_int, ip, ip, ip, C.c_int]
+        L.ox_adapt_forest_transfer.restype = C.c_int
+        L.ox_adapt_forest_transfer.argtypes = [C.c_int, C.c_int, C.c_int, ip, ip, dp, dp]
         L.ox_post_pressure.argtypes = [C.POINTER(_Mesh), C.c_int, dp, dp]
         L.ox_post_set_solver.argtypes = [C.c_double, C.c_double, C.c_int, C.c_void_p, C.c_int]
         L.ox_post_error.restype = C.c_double
@@ -175,6 +183,56 @@ def _p(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
+def _ip(a):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def forest_levels(dim, n_intervals, tree):
+    t = np.ascontiguousarray(tree, dtype=np.int32)
+    n = lib().ox_forest_levels(dim, n_intervals, _ip(t), t.size, None)
+    if n < 0:
+        raise ValueError("not a forest")
+    lev = np.zeros(n, np.int32)
+    lib().ox_forest_levels(dim, n_intervals, _ip(t), t.size, _ip(lev))
+    return lev
+
+
+def adapt_forest(dim, n_intervals, tree, flags):
+    """ox_adapt_forest: the new depth-first trees after the flags."""
+    t = np.ascontiguousarray(tree, dtype=np.int32)
+    f = np.ascontiguousarray(flags, dtype=np.int32)
+    n = lib().ox_adapt_forest(dim, n_intervals, _ip(t), _ip(f), None, 0)
+    if n < 0:
+        raise ValueError("ox_adapt_forest failed")
+    out = np.zeros(n, np.int32)
+    assert lib().ox_adapt_forest(dim, n_intervals, _ip(t), _ip(f), _ip(out), n) == n
+    return out
+
+
+def adapt_forest_transfer(dim, degree, n_intervals, tree_old, tree_new, old_u, n_new_dofs):
+    to = np.ascontiguousarray(tree_old, dtype=np.int32)
+    tn = np.ascontiguousarray(tree_new, dtype=np.int32)
+    out = np.zeros(n_new_dofs)
+    u = np.ascontiguousarray(old_u, dtype=np.float64)
+    if lib().ox_adapt_forest_transfer(dim, degree, n_intervals, _ip(to), _ip(tn), _p(u), _p(out)):
+        raise ValueError("trees differ by more than one level")
+    return out
+
+
+def uniform_forest(dim, n_intervals, depth):
+    """the depth-first trees of the box refined `depth` times everywhere"""
+    one = []
+
+    def rec(l):
+        one.append(1 if l < depth else 0)
+        if l < depth:
+            for _ in range(1 << dim):
+                rec(l + 1)
+    rec(0)
+    return np.array(one * n_intervals ** dim, np.int32)
+
+
 class Mesh:
     """A hypercube mesh exactly as the reference builds it (see exwave_oracle.c)."""
 
@@ -184,6 +242,20 @@ class Mesh:
         self._m = lib().ox_mesh_hypercube(dim, n_intervals, n_refinements, transform, factor,
                                           boundary_id)
         self.n_cells = self._m.contents.n_cells
+
+    @classmethod
+    def forest(cls, dim, n_intervals, tree, transform=0, factor=0.0, boundary_id=2):
+        """ox_mesh_forest: the box with base cells refined by the depth-first
+        trees in `tree` (1 refined, 0 active)."""
+        t = np.ascontiguousarray(tree, dtype=np.int32).ravel()
+        self = cls.__new__(cls)
+        self.dim = dim
+        m = lib().ox_mesh_forest(dim, n_intervals, transform, factor, boundary_id, _ip(t), t.size)
+        if not m:
+            raise ValueError("ox_mesh_forest: not a 2:1 balanced forest")
+        self._m = m
+        self.n_cells = self._m.contents.n_cells
+        return self
 
     @classmethod
     def box_refined(cls, dim, n_intervals, refine, transform=0, factor=0.0, boundary_id=2):

@@ -1333,69 +1333,504 @@ void ox_adapt_refine(int dim, int n_intervals, const int *refine_old, const int 
   }
 }
 
+/* a child ch of a cell: the parent's polynomial at the child's nodes (all
+ * fields; up and out hold (dim+1) (k+1)^dim values) */
+static void transfer_child(const ox_basis *b, int dim, int ch, const double *up, double *out) {
+  const int n = b->n, nd = ipow(n, dim), nf = dim + 1;
+  for (int f = 0; f < nf; ++f)
+    for (int q = 0; q < nd; ++q) {
+      double xi[3];
+      int t = q;
+      for (int d = 0; d < dim; ++d) {
+        xi[d] = 0.5 * (b->x[t % n] + ((ch >> d) & 1));
+        t /= n;
+      }
+      out[f * nd + q] = eval_at(b, dim, up + f * nd, xi);
+    }
+}
+
+/* a parent from its 2^dim children (consecutive in uc): parent_i =
+ * (1/w_i) int_ref l_i u with QGauss(k+2) on each child */
+static void transfer_parent(const ox_basis *b, int dim, const double *uc, double *out) {
+  const int n = b->n, nd = ipow(n, dim), nf = dim + 1, nch = 1 << dim;
+  const int nq = n + 1, nqd = ipow(nq, dim);
+  double xq[OX_MAX_N], wq[OX_MAX_N];
+  ox_gauss(nq, xq, wq);
+  for (int f = 0; f < nf; ++f)
+    for (int i = 0; i < nd; ++i) {
+      double acc = 0.0, wi = 1.0, li[3][OX_MAX_N];
+      int ti = i;
+      int id[3];
+      for (int d = 0; d < dim; ++d) {
+        id[d] = ti % n;
+        ti /= n;
+        wi *= b->w[id[d]];
+      }
+      for (int ch = 0; ch < nch; ++ch)
+        for (int q = 0; q < nqd; ++q) {
+          double xc[3], xp[3], w = 1.0 / nch;
+          int t = q;
+          for (int d = 0; d < dim; ++d) {
+            xc[d] = xq[t % nq];
+            w *= wq[t % nq];
+            t /= nq;
+            xp[d] = 0.5 * (xc[d] + ((ch >> d) & 1));
+          }
+          double phi = 1.0;
+          for (int d = 0; d < dim; ++d) {
+            ox_lagrange(n, b->x, xp[d], li[d], NULL);
+            phi *= li[d][id[d]];
+          }
+          acc += w * phi * eval_at(b, dim, uc + ((long)ch * nf + f) * nd, xc);
+        }
+      out[f * nd + i] = acc / wi;
+    }
+}
+
 void ox_adapt_transfer(int dim, int degree, int n_intervals, const int *refine_old,
                        const int *refine_new, const double *old_u, double *new_u) {
   ox_basis b;
   ox_basis_init(&b, degree);
-  const int n = b.n, nd = ipow(n, dim), nf = dim + 1, nch = 1 << dim, nb = ipow(n_intervals, dim);
-  const int nq = n + 1, nqd = ipow(nq, dim);
-  double xq[OX_MAX_N], wq[OX_MAX_N];
-  ox_gauss(nq, xq, wq);
+  const int nd = ipow(b.n, dim), nf = dim + 1, nch = 1 << dim, nb = ipow(n_intervals, dim);
+  const long sz = (long)nf * nd;
   long co = 0, cn = 0;
   for (int B = 0; B < nb; ++B) {
     const int ro = refine_old[B] != 0, rn = refine_new[B] != 0;
     if (ro == rn) {
       const int m = ro ? nch : 1;
-      memcpy(new_u + cn * nf * nd, old_u + co * nf * nd, sizeof(double) * m * nf * nd);
+      memcpy(new_u + cn * sz, old_u + co * sz, sizeof(double) * m * sz);
       cn += m;
     } else if (rn) {
-      /* children: the parent's polynomial at the child's nodes */
-      for (int ch = 0; ch < nch; ++ch, ++cn)
-        for (int f = 0; f < nf; ++f)
-          for (int q = 0; q < nd; ++q) {
-            double xi[3];
-            int t = q;
-            for (int d = 0; d < dim; ++d) {
-              xi[d] = 0.5 * (b.x[t % n] + ((ch >> d) & 1));
-              t /= n;
-            }
-            new_u[(cn * nf + f) * nd + q] = eval_at(&b, dim, old_u + (co * nf + f) * nd, xi);
-          }
+      for (int ch = 0; ch < nch; ++ch, ++cn) transfer_child(&b, dim, ch, old_u + co * sz, new_u + cn * sz);
     } else {
-      /* parent_i = (1/w_i) int_ref l_i u, u the children (QGauss(k+2) each) */
-      for (int f = 0; f < nf; ++f)
-        for (int i = 0; i < nd; ++i) {
-          double acc = 0.0, wi = 1.0, li[3][OX_MAX_N];
-          int ti = i;
-          int id[3];
-          for (int d = 0; d < dim; ++d) {
-            id[d] = ti % n;
-            ti /= n;
-            wi *= b.w[id[d]];
-          }
-          for (int ch = 0; ch < nch; ++ch)
-            for (int q = 0; q < nqd; ++q) {
-              double xc[3], xp[3], w = 1.0 / nch;
-              int t = q;
-              for (int d = 0; d < dim; ++d) {
-                xc[d] = xq[t % nq];
-                w *= wq[t % nq];
-                t /= nq;
-                xp[d] = 0.5 * (xc[d] + ((ch >> d) & 1));
-              }
-              double phi = 1.0;
-              for (int d = 0; d < dim; ++d) {
-                ox_lagrange(n, b.x, xp[d], li[d], NULL);
-                phi *= li[d][id[d]];
-              }
-              acc += w * phi * eval_at(&b, dim, old_u + ((co + ch) * nf + f) * nd, xc);
-            }
-          new_u[(cn * nf + f) * nd + i] = acc / wi;
-        }
+      transfer_parent(&b, dim, old_u + co * sz, new_u + cn * sz);
       ++cn;
     }
     co += ro ? nch : 1;
   }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Forests of refinement trees: several levels of local refinement          */
+/* (adapt_mesh, source/explicit_wave.cc:265-330, with min_level =           */
+/* n_refinements and max_level = min_level + n_adaptive_refinements,        */
+/* :272-273).  tree[] lists the base cells of the n_intervals^dim box       */
+/* lexicographically, each depth first: 1 = refined (its 2^dim children     */
+/* follow, lexicographic), 0 = an active cell.  The oracle works on the     */
+/* dense picture: lev[g] over the finest grid (n_intervals 2^L cells per    */
+/* direction) is the level of the active cell that covers fine cell g.      */
+/* ------------------------------------------------------------------------ */
+
+typedef struct fcells {
+  int n, max_level, n_tree;
+  int *level;
+  long *coord; /* n * 3: integer coordinates at the cell's own level */
+} fcells;
+
+static void fcells_push(fcells *C, int *cap, int l, const long *I) {
+  if (C->n == *cap) {
+    *cap = *cap ? 2 * *cap : 64;
+    C->level = (int *)realloc(C->level, sizeof(int) * *cap);
+    C->coord = (long *)realloc(C->coord, sizeof(long) * 3 * *cap);
+  }
+  C->level[C->n] = l;
+  for (int d = 0; d < 3; ++d) C->coord[3 * C->n + d] = I[d];
+  if (l > C->max_level) C->max_level = l;
+  ++C->n;
+}
+
+static int fparse(int dim, const int *tree, int n_tree, int *pos, int l, const long *I,
+                  fcells *C, int *cap) {
+  if (*pos >= n_tree || l > 20) return -1;
+  const int t = tree[(*pos)++];
+  if (t == 0) {
+    fcells_push(C, cap, l, I);
+    return 0;
+  }
+  if (t != 1) return -1;
+  for (int ch = 0; ch < (1 << dim); ++ch) {
+    long J[3] = {0, 0, 0};
+    for (int d = 0; d < dim; ++d) J[d] = 2 * I[d] + ((ch >> d) & 1);
+    if (fparse(dim, tree, n_tree, pos, l + 1, J, C, cap)) return -1;
+  }
+  return 0;
+}
+
+/* the active cells of a forest in its order; n_tree < 0: read to the end of
+ * the last base cell */
+static int forest_cells(int dim, int N, const int *tree, int n_tree, fcells *C) {
+  memset(C, 0, sizeof(*C));
+  int cap = 0, pos = 0;
+  const int nb = ipow(N, dim);
+  for (int B = 0; B < nb; ++B) {
+    long I[3] = {0, 0, 0};
+    int t = B;
+    for (int d = 0; d < dim; ++d) {
+      I[d] = t % N;
+      t /= N;
+    }
+    if (fparse(dim, tree, n_tree < 0 ? 1 << 30 : n_tree, &pos, 0, I, C, &cap)) return -1;
+  }
+  C->n_tree = pos;
+  return 0;
+}
+
+static void fcells_free(fcells *C) {
+  free(C->level);
+  free(C->coord);
+}
+
+typedef struct fgrid {
+  int dim, L;
+  long G, ng;
+} fgrid;
+
+static long fg_at(const fgrid *F, const long *g) {
+  long i = 0;
+  for (int d = F->dim - 1; d >= 0; --d) i = i * F->G + g[d];
+  return i;
+}
+static void fg_coord(const fgrid *F, long i, long *g) {
+  g[0] = g[1] = g[2] = 0;
+  for (int d = 0; d < F->dim; ++d) {
+    g[d] = i % F->G;
+    i /= F->G;
+  }
+}
+/* calls the block of fine cells of the cell (l, I) */
+static void fg_fill(const fgrid *F, int *arr, int l, const long *I, int val) {
+  const long s = 1L << (F->L - l);
+  long lo[3] = {0, 0, 0}, hi[3] = {1, 1, 1};
+  for (int d = 0; d < F->dim; ++d) {
+    lo[d] = I[d] * s;
+    hi[d] = lo[d] + s;
+  }
+  long g[3];
+  for (g[2] = lo[2]; g[2] < hi[2]; ++g[2])
+    for (g[1] = lo[1]; g[1] < hi[1]; ++g[1])
+      for (g[0] = lo[0]; g[0] < hi[0]; ++g[0]) arr[fg_at(F, g)] = val;
+}
+
+int ox_forest_levels(int dim, int n_intervals, const int *tree, int n_tree, int *level) {
+  fcells C;
+  if (forest_cells(dim, n_intervals, tree, n_tree, &C)) return -1;
+  if (level) memcpy(level, C.level, sizeof(int) * C.n);
+  const int n = C.n;
+  fcells_free(&C);
+  return n;
+}
+
+ox_mesh *ox_mesh_forest(int dim, int n_intervals, int transform, double factor, int boundary_id,
+                        const int *tree, int n_tree) {
+  fcells C;
+  if (forest_cells(dim, n_intervals, tree, n_tree, &C)) return NULL;
+  const int nv = 1 << dim, nface = 2 * dim, nsub = 1 << (dim - 1), rec = 2 + nsub;
+  fgrid F = {dim, C.max_level, (long)n_intervals << C.max_level, 0};
+  F.ng = 1;
+  for (int d = 0; d < dim; ++d) F.ng *= F.G;
+  int *owner = (int *)malloc(sizeof(int) * F.ng);
+  for (int c = 0; c < C.n; ++c) fg_fill(&F, owner, C.level[c], C.coord + 3 * c, c);
+  ox_mesh *m = (ox_mesh *)calloc(1, sizeof(ox_mesh));
+  m->dim = dim;
+  m->n_cells = C.n;
+  m->vertices = (double *)malloc(sizeof(double) * C.n * nv * dim);
+  m->neighbor = (int *)malloc(sizeof(int) * C.n * nface);
+  m->rho = (double *)malloc(sizeof(double) * C.n);
+  m->speed = (double *)malloc(sizeof(double) * C.n);
+  m->nc_rec = (int *)malloc(sizeof(int) * C.n * nface);
+  m->nc_pos = (int *)malloc(sizeof(int) * C.n * nface);
+  m->hanging = (int *)malloc(sizeof(int) * ((long)C.n * nface + 1) * rec);
+  m->n_hanging = 0;
+  double mind = 1e300;
+  int bad = 0;
+  for (int c = 0; c < C.n && !bad; ++c) {
+    const int l = C.level[c];
+    const long *I = C.coord + 3 * c, s = 1L << (F.L - l);
+    /* vertices: the base cell's (transformed) multilinear map at dyadic points */
+    long B[3] = {0, 0, 0};
+    for (int d = 0; d < dim; ++d) B[d] = I[d] >> l;
+    double PV[8 * 3];
+    for (int v = 0; v < nv; ++v) {
+      double x[3] = {0.0, 0.0, 0.0};
+      for (int d = 0; d < dim; ++d) x[d] = (double)(B[d] + ((v >> d) & 1)) / n_intervals;
+      transform_vertex(dim, transform, factor, x);
+      for (int d = 0; d < dim; ++d) PV[v * dim + d] = x[d];
+    }
+    for (int v = 0; v < nv; ++v) {
+      double xi[3];
+      for (int d = 0; d < dim; ++d)
+        xi[d] = (double)(I[d] - (B[d] << l) + ((v >> d) & 1)) / (double)(1L << l);
+      multilinear_point(dim, PV, xi, m->vertices + ((long)c * nv + v) * dim);
+    }
+    m->rho[c] = m->speed[c] = 1.0;
+    const double md = min_vertex_distance(dim, m->vertices + (long)c * nv * dim);
+    if (md < mind) mind = md;
+    for (int f = 0; f < nface; ++f) {
+      const int d = f >> 1, sd = f & 1, cf = c * nface + f;
+      m->nc_rec[cf] = m->nc_pos[cf] = -1;
+      long g[3] = {0, 0, 0};
+      for (int e = 0; e < dim; ++e) g[e] = I[e] * s;
+      g[d] = sd ? (I[d] + 1) * s : I[d] * s - 1;
+      if (g[d] < 0 || g[d] >= F.G) {
+        m->neighbor[cf] = -boundary_id;
+        continue;
+      }
+      const int y = owner[fg_at(&F, g)], ly = C.level[y];
+      if (ly == l) {
+        m->neighbor[cf] = y;
+      } else if (ly == l - 1) {
+        m->neighbor[cf] = OX_NC_FACE; /* fine side: filled from the coarse side's record */
+      } else if (ly == l + 1) {
+        int *r = m->hanging + (long)m->n_hanging * rec;
+        r[0] = c;
+        r[1] = f;
+        for (int b = 0; b < nsub; ++b) {
+          long gb[3] = {g[0], g[1], g[2]};
+          int j = 0;
+          for (int e = 0; e < dim; ++e)
+            if (e != d) gb[e] += ((b >> j++) & 1) * (s / 2);
+          r[2 + b] = owner[fg_at(&F, gb)];
+          if (C.level[r[2 + b]] != l + 1) bad = 1;
+        }
+        m->neighbor[cf] = OX_NC_FACE;
+        m->nc_rec[cf] = m->n_hanging++;
+      } else {
+        bad = 1; /* more than one level across a face */
+      }
+    }
+  }
+  for (int h = 0; h < m->n_hanging && !bad; ++h) {
+    const int *r = m->hanging + (long)h * rec;
+    for (int b = 0; b < nsub; ++b) {
+      const int cf = r[2 + b] * nface + (r[1] ^ 1);
+      m->nc_rec[cf] = h;
+      m->nc_pos[cf] = b;
+    }
+  }
+  m->min_vertex_distance_over_c = mind;
+  free(owner);
+  fcells_free(&C);
+  if (bad) {
+    ox_mesh_free(m);
+    return NULL;
+  }
+  return m;
+}
+
+/* a fine cell and a vertex-adjacent one (faces, edges, corners) */
+static int fg_neighbor(const fgrid *F, const long *g, int o, long *h) {
+  for (int d = 0; d < F->dim; ++d) {
+    const int od = (o % 3) - 1;
+    o /= 3;
+    h[d] = g[d] + od;
+    if (h[d] < 0 || h[d] >= F->G) return 0;
+  }
+  if (F->dim == 2) h[2] = 0;
+  return 1;
+}
+
+static int femit(const fgrid *F, const int *lev, int l, const long *I, int *out, int pos,
+                 int cap) {
+  const long s = 1L << (F->L - l);
+  long lo[3] = {0, 0, 0}, hi[3] = {1, 1, 1};
+  for (int d = 0; d < F->dim; ++d) {
+    lo[d] = I[d] * s;
+    hi[d] = lo[d] + s;
+  }
+  int finer = 0;
+  long g[3];
+  for (g[2] = lo[2]; g[2] < hi[2]; ++g[2])
+    for (g[1] = lo[1]; g[1] < hi[1]; ++g[1])
+      for (g[0] = lo[0]; g[0] < hi[0]; ++g[0]) {
+        const int v = lev[fg_at(F, g)];
+        if (v < l) return -1;
+        if (v > l) finer = 1;
+      }
+  if (pos < 0 || (out && pos >= cap)) return -1;
+  if (out) out[pos] = finer;
+  ++pos;
+  if (!finer) return pos;
+  for (int ch = 0; ch < (1 << F->dim); ++ch) {
+    long J[3] = {0, 0, 0};
+    for (int d = 0; d < F->dim; ++d) J[d] = 2 * I[d] + ((ch >> d) & 1);
+    pos = femit(F, lev, l + 1, J, out, pos, cap);
+    if (pos < 0) return -1;
+  }
+  return pos;
+}
+
+int ox_adapt_forest(int dim, int n_intervals, const int *tree, const int *flags, int *tree_new,
+                    int cap) {
+  fcells C;
+  if (forest_cells(dim, n_intervals, tree, -1, &C)) return -1;
+  fgrid F = {dim, C.max_level + 1, (long)n_intervals << (C.max_level + 1), 0};
+  F.ng = 1;
+  for (int d = 0; d < dim; ++d) F.ng *= F.G;
+  const int nn = ipow(3, dim), nch = 1 << dim;
+  int *lev = (int *)malloc(sizeof(int) * F.ng), *owner = (int *)malloc(sizeof(int) * F.ng);
+  int *tmp = (int *)malloc(sizeof(int) * F.ng);
+  for (int c = 0; c < C.n; ++c) {
+    fg_fill(&F, owner, C.level[c], C.coord + 3 * c, c);
+    fg_fill(&F, lev, C.level[c], C.coord + 3 * c, C.level[c]);
+  }
+  /* the cells flagged for refinement */
+  for (int c = 0; c < C.n; ++c)
+    if (flags[c] == 1) fg_fill(&F, lev, C.level[c], C.coord + 3 * c, C.level[c] + 1);
+  /* 2:1 balance across faces, edges and corners by refinement of the coarser side */
+  for (int changed = 1; changed;) {
+    changed = 0;
+    for (long i = 0; i < F.ng; ++i) {
+      long g[3], h[3];
+      fg_coord(&F, i, g);
+      for (int o = 0; o < nn; ++o)
+        if (fg_neighbor(&F, g, o, h) && lev[fg_at(&F, h)] >= lev[i] + 2) {
+          const int l = lev[i];
+          long J[3] = {0, 0, 0};
+          for (int d = 0; d < dim; ++d) J[d] = g[d] >> (F.L - l);
+          fg_fill(&F, lev, l, J, l + 1);
+          changed = 1;
+          break;
+        }
+    }
+  }
+  /* coarsening candidates: families of active cells, all flagged -1 and none
+   * refined above; the parent (level l-1) is given by its first child */
+  int *cand = (int *)malloc(sizeof(int) * (C.n + 1)), nc = 0;
+  for (int c = 0; c < C.n; ++c) {
+    const int l = C.level[c];
+    const long *I = C.coord + 3 * c;
+    int first = l > 0 && flags[c] == -1;
+    for (int d = 0; d < dim && first; ++d) first = (I[d] & 1) == 0;
+    if (!first) continue;
+    int ok = 1;
+    for (int ch = 0; ch < nch && ok; ++ch) {
+      long J[3] = {0, 0, 0};
+      for (int d = 0; d < dim; ++d) J[d] = (I[d] + ((ch >> d) & 1)) << (F.L - l);
+      const int y = owner[fg_at(&F, J)];
+      ok = C.level[y] == l && flags[y] == -1;
+      if (ok) { /* still one active cell of level l after the refinement */
+        long K[3] = {0, 0, 0};
+        for (int d = 0; d < dim; ++d) K[d] = I[d] + ((ch >> d) & 1);
+        const long s = 1L << (F.L - l);
+        long g[3];
+        long lo[3] = {0, 0, 0}, hi[3] = {1, 1, 1};
+        for (int d = 0; d < dim; ++d) {
+          lo[d] = K[d] * s;
+          hi[d] = lo[d] + s;
+        }
+        for (g[2] = lo[2]; g[2] < hi[2] && ok; ++g[2])
+          for (g[1] = lo[1]; g[1] < hi[1] && ok; ++g[1])
+            for (g[0] = lo[0]; g[0] < hi[0] && ok; ++g[0]) ok = lev[fg_at(&F, g)] == l;
+      }
+    }
+    if (ok) cand[nc++] = c;
+  }
+  /* drop the coarsenings that would break the balance, until none does */
+  for (int dropped = 1; dropped;) {
+    dropped = 0;
+    memcpy(tmp, lev, sizeof(int) * F.ng);
+    for (int i = 0; i < nc; ++i) {
+      const int c = cand[i], l = C.level[c];
+      long P[3] = {0, 0, 0};
+      for (int d = 0; d < dim; ++d) P[d] = C.coord[3 * c + d] >> 1;
+      fg_fill(&F, tmp, l - 1, P, l - 1);
+    }
+    for (int i = 0; i < nc; ++i) {
+      const int c = cand[i], l = C.level[c];
+      const long s = 2L << (F.L - l);
+      long lo[3] = {0, 0, 0}, hi[3] = {1, 1, 1};
+      for (int d = 0; d < dim; ++d) {
+        lo[d] = (C.coord[3 * c + d] >> 1) * s;
+        hi[d] = lo[d] + s;
+      }
+      int viol = 0;
+      long g[3], h[3];
+      for (g[2] = lo[2]; g[2] < hi[2] && !viol; ++g[2])
+        for (g[1] = lo[1]; g[1] < hi[1] && !viol; ++g[1])
+          for (g[0] = lo[0]; g[0] < hi[0] && !viol; ++g[0])
+            for (int o = 0; o < nn && !viol; ++o)
+              if (fg_neighbor(&F, g, o, h) && tmp[fg_at(&F, h)] >= l + 1) viol = 1;
+      if (viol) {
+        cand[i--] = cand[--nc];
+        dropped = 1;
+      }
+    }
+  }
+  for (int i = 0; i < nc; ++i) {
+    const int c = cand[i], l = C.level[c];
+    long P[3] = {0, 0, 0};
+    for (int d = 0; d < dim; ++d) P[d] = C.coord[3 * c + d] >> 1;
+    fg_fill(&F, lev, l - 1, P, l - 1);
+  }
+  int pos = 0;
+  const int nb = ipow(n_intervals, dim);
+  for (int B = 0; B < nb && pos >= 0; ++B) {
+    long I[3] = {0, 0, 0};
+    int t = B;
+    for (int d = 0; d < dim; ++d) {
+      I[d] = t % n_intervals;
+      t /= n_intervals;
+    }
+    pos = femit(&F, lev, 0, I, tree_new, pos, cap);
+  }
+  free(cand);
+  free(lev);
+  free(owner);
+  free(tmp);
+  fcells_free(&C);
+  return pos;
+}
+
+/* SolutionTransfer between two forests that differ by at most one level per
+ * cell: kept cells copied, refined ones interpolated, coarsened families
+ * projected (the per-cell rules of ox_adapt_transfer) */
+typedef struct ftr {
+  int dim, nd, nf, nch;
+  const ox_basis *b;
+  const int *to, *tn;
+  int po, pn;
+  long co, cn;
+  const double *old_u;
+  double *new_u;
+} ftr;
+
+static int ftr_walk(ftr *T) {
+  const int a = T->to[T->po++], b = T->tn[T->pn++];
+  const long sz = (long)T->nf * T->nd;
+  if (!a && !b) {
+    memcpy(T->new_u + T->cn * sz, T->old_u + T->co * sz, sizeof(double) * sz);
+    ++T->co;
+    ++T->cn;
+  } else if (!a) {
+    for (int ch = 0; ch < T->nch; ++ch) {
+      if (T->tn[T->pn++] != 0) return -1;
+      transfer_child(T->b, T->dim, ch, T->old_u + T->co * sz, T->new_u + (T->cn + ch) * sz);
+    }
+    ++T->co;
+    T->cn += T->nch;
+  } else if (!b) {
+    for (int ch = 0; ch < T->nch; ++ch)
+      if (T->to[T->po++] != 0) return -1;
+    transfer_parent(T->b, T->dim, T->old_u + T->co * sz, T->new_u + T->cn * sz);
+    T->co += T->nch;
+    ++T->cn;
+  } else {
+    for (int ch = 0; ch < T->nch; ++ch)
+      if (ftr_walk(T)) return -1;
+  }
+  return 0;
+}
+
+int ox_adapt_forest_transfer(int dim, int degree, int n_intervals, const int *tree_old,
+                             const int *tree_new, const double *old_u, double *new_u) {
+  ox_basis b;
+  ox_basis_init(&b, degree);
+  ftr T = {dim, ipow(b.n, dim), dim + 1, 1 << dim, &b, tree_old, tree_new, 0, 0, 0, 0, old_u,
+           new_u};
+  const int nb = ipow(n_intervals, dim);
+  for (int B = 0; B < nb; ++B)
+    if (ftr_walk(&T)) return -1;
+  return 0;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -2358,15 +2793,33 @@ void ox_lts_step(ox_lts *L, double time_now, const double *src, double *dst) {
 /* driver: WaveEquationProblem::run with TimeControl                          */
 /* ------------------------------------------------------------------------ */
 
+/* the depth-first entries of a base cell refined uniformly `depth` times */
+static int uniform_tree(int dim, int depth, int *out, int pos) {
+  out[pos++] = depth > 0;
+  if (depth > 0)
+    for (int ch = 0; ch < (1 << dim); ++ch) pos = uniform_tree(dim, depth - 1, out, pos);
+  return pos;
+}
+
 int ox_run(const ox_problem *P, char *log, int log_len, double *final_energy) {
   const int adaptive = P->n_adaptive_refinements > 0;
-  if (adaptive && (P->n_adaptive_refinements != 1 || P->n_refinements != 0 ||
-                   P->integrator == OX_ADER_LTS))
-    return -1; /* one adaptive level on the coarse mesh only */
-  const int nbase = ipow(P->n_intervals, P->dim);
-  int *refine = (int *)calloc(nbase, sizeof(int));
-  ox_mesh *m = adaptive ? ox_mesh_box_refined(P->dim, P->n_intervals, P->transform,
-                                              P->transform_factor, P->boundary_id, refine)
+  if (adaptive && P->integrator == OX_ADER_LTS) return -1; /* adaptivity without LTS */
+  /* adaptive runs: the forest of the globally refined box, every base cell
+   * refined n_refinements times (levels min_level .. max_level,
+   * source/explicit_wave.cc:272-273) */
+  const int nbase = ipow(P->n_intervals, P->dim), R = P->n_refinements;
+  const int min_level = R, max_level = R + P->n_adaptive_refinements;
+  int tcap = 0, ntree = 0;
+  int *tree = NULL, *tnew = NULL;
+  if (adaptive) {
+    long per = 0, pw = 1; /* entries of a uniform tree of depth R */
+    for (int l = 0; l <= R; ++l, pw *= 1 << P->dim) per += pw;
+    tcap = (int)(per * nbase);
+    tree = (int *)malloc(sizeof(int) * tcap);
+    for (int B = 0; B < nbase; ++B) ntree = uniform_tree(P->dim, R, tree, ntree);
+  }
+  ox_mesh *m = adaptive ? ox_mesh_forest(P->dim, P->n_intervals, P->transform,
+                                         P->transform_factor, P->boundary_id, tree, ntree)
                         : ox_mesh_hypercube(P->dim, P->n_intervals, P->n_refinements,
                                             P->transform, P->transform_factor,
                                             P->boundary_id);
@@ -2404,21 +2857,21 @@ int ox_run(const ox_problem *P, char *log, int log_len, double *final_energy) {
   /* adapt_mesh (source/explicit_wave.cc:265-330) on the box mesh: marking,
    * new mesh, solution transfer, new CFL step */
   double max_err_init = -1.0;
-  int *rnew = (int *)calloc(nbase, sizeof(int));
 #define OX_ADAPT()                                                                         \
   do {                                                                                     \
     double *err = (double *)malloc(sizeof(double) * m->n_cells);                           \
     int *lev = (int *)malloc(sizeof(int) * m->n_cells), *fl = (int *)malloc(sizeof(int) * m->n_cells); \
     ox_estimate_error(m, k, sol, err);                                                     \
-    for (int B = 0, c = 0; B < nbase; ++B)                                                 \
-      for (int ch = 0; ch < (refine[B] ? (1 << P->dim) : 1); ++ch) lev[c++] = refine[B];    \
-    ox_refinement_flags(m->n_cells, err, lev, 0, 1, max_err_init, fl);                     \
-    ox_adapt_refine(P->dim, P->n_intervals, refine, fl, rnew);                             \
-    ox_mesh *m2 = ox_mesh_box_refined(P->dim, P->n_intervals, P->transform,                \
-                                      P->transform_factor, P->boundary_id, rnew);          \
+    ox_forest_levels(P->dim, P->n_intervals, tree, ntree, lev);                            \
+    ox_refinement_flags(m->n_cells, err, lev, min_level, max_level, max_err_init, fl);     \
+    const int nt2 = ox_adapt_forest(P->dim, P->n_intervals, tree, fl, NULL, 0);            \
+    tnew = (int *)realloc(tnew, sizeof(int) * nt2);                                        \
+    ox_adapt_forest(P->dim, P->n_intervals, tree, fl, tnew, nt2);                          \
+    ox_mesh *m2 = ox_mesh_forest(P->dim, P->n_intervals, P->transform,                     \
+                                 P->transform_factor, P->boundary_id, tnew, nt2);          \
     const long N2 = ox_n_dofs(m2, k);                                                      \
     double *s2 = (double *)malloc(sizeof(double) * N2);                                    \
-    ox_adapt_transfer(P->dim, k, P->n_intervals, refine, rnew, sol, s2);                   \
+    ox_adapt_forest_transfer(P->dim, k, P->n_intervals, tree, tnew, sol, s2);              \
     free(sol);                                                                             \
     sol = s2;                                                                              \
     tmp = (double *)realloc(tmp, sizeof(double) * N2);                                     \
@@ -2428,7 +2881,10 @@ int ox_run(const ox_problem *P, char *log, int log_len, double *final_energy) {
     ox_mesh_free(m);                                                                       \
     m = m2;                                                                                \
     N = N2;                                                                                \
-    memcpy(refine, rnew, sizeof(int) * nbase);                                             \
+    int *sw_ = tree;                                                                       \
+    tree = tnew;                                                                           \
+    tnew = sw_;                                                                            \
+    ntree = nt2;                                                                           \
     /* compute_time_step_size (explicit_wave.cc:326-327, 529-545) */                       \
     dt = cfl * m->min_vertex_distance_over_c;                                              \
     free(err);                                                                             \
@@ -2437,8 +2893,10 @@ int ox_run(const ox_problem *P, char *log, int log_len, double *final_energy) {
   } while (0)
   if (adaptive) {
     /* the initial adaptation (explicit_wave.cc:704-716) */
-    OX_ADAPT();
-    ox_project(m, k, &P->exact, 0.0, sol);
+    for (int left = P->n_adaptive_refinements; left > 0; --left) {
+      OX_ADAPT();
+      ox_project(m, k, &P->exact, 0.0, sol);
+    }
     double *err = (double *)malloc(sizeof(double) * m->n_cells);
     ox_estimate_error(m, k, sol, err);
     max_err_init = 0.0;
@@ -2478,8 +2936,8 @@ int ox_run(const ox_problem *P, char *log, int log_len, double *final_energy) {
   }
 #undef OX_LOG
 #undef OX_ADAPT
-  free(refine);
-  free(rnew);
+  free(tree);
+  free(tnew);
   if (final_energy) *final_energy = ox_energy(m, k, sol);
   ox_lts_free(lts);
   free(post);

@@ -162,7 +162,7 @@ typedef struct ox_problem {
   ox_exact exact;
   int lts_max_clusters, lts_max_diff; /* max_n_clusters, max_diff_clusters */
   int post_pressure;                   /* also log 'error p post' (else -1) */
-  int n_adaptive_refinements;          /* 0 or 1 (one level on the n_intervals^dim mesh) */
+  int n_adaptive_refinements;          /* levels above n_refinements (forest meshes; no LTS) */
   int adaptive_refinement_interval;
 } ox_problem;
 int ox_run(const ox_problem *p, char *log, int log_len, double *final_state_energy);
@@ -185,6 +185,29 @@ void ox_adapt_refine(int dim, int n_intervals, const int *refine_old, const int 
                      int *refine_new);
 void ox_adapt_transfer(int dim, int degree, int n_intervals, const int *refine_old,
                        const int *refine_new, const double *old_u, double *new_u);
+
+/* Several levels: forests of refinement trees over the n_intervals^dim box
+ * (transform applied to the base vertices, children the multilinear map at
+ * dyadic points).  tree[] lists the base cells lexicographically, each depth
+ * first: 1 = refined (its 2^dim children follow, lexicographic), 0 = an active
+ * cell; active cells are numbered in that order and the level of a cell is its
+ * depth.  n_tree < 0 reads as far as the trees go.
+ * ox_forest_levels: the cells' levels (level may be NULL); returns n_cells.
+ * ox_mesh_forest: NULL unless faces differ by at most one level. */
+int ox_forest_levels(int dim, int n_intervals, const int *tree, int n_tree, int *level);
+ox_mesh *ox_mesh_forest(int dim, int n_intervals, int transform, double factor, int boundary_id,
+                        const int *tree, int n_tree);
+/* The new forest after adapt_mesh's flags (1 refine, -1 coarsen, per active
+ * cell) as p4est-backed deal.II makes it: families are coarsened only when all
+ * children are active and flagged; the mesh is kept 2:1 balanced across faces,
+ * edges and corners -- the coarser cell of a violation is refined until none is
+ * left, and a coarsening that would create one is not done.  Returns the
+ * length of tree_new (NULL: the length only), -1 on error or if cap is short. */
+int ox_adapt_forest(int dim, int n_intervals, const int *tree, const int *flags, int *tree_new,
+                    int cap);
+/* the transfer of ox_adapt_transfer between two forests of ox_adapt_forest */
+int ox_adapt_forest_transfer(int dim, int degree, int n_intervals, const int *tree_old,
+                             const int *tree_new, const double *old_u, double *new_u);
 
 /* Post-processed pressure (compute_post_pressure,
  * source/wave_equation_operations.cc:786-848): per cell the local Neumann

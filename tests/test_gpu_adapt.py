@@ -127,3 +127,30 @@ def test_refine_then_coarsen_is_identity(X, ctx):
     X.adapt_box_transfer(ctx, dim, k, n, r0, r1, a, b)
     X.adapt_box_transfer(ctx, dim, k, n, r1, r0, b, c)
     assert np.max(np.abs(c.cpu().numpy() - u)) <= 1e-13 * np.max(np.abs(u))
+
+
+@pytest.mark.parametrize("dim,k,n,seed", [(2, 3, 3, 1), (3, 3, 2, 2), (2, 12, 2, 3),
+                                          (3, 9, 1, 4)])
+def test_forest_transfer_matches_oracle(X, ctx, dim, k, n, seed):
+    """exw_adapt_forest_transfer over several levels (kept, refined and
+    coarsened cells of a random adaptation) against the oracle's transfer"""
+    import torch
+    print("forest transfer", dim, k, n, seed, flush=True)
+    rng = np.random.default_rng(seed)
+    tree = O.uniform_forest(dim, n, 1)
+    for _ in range(2):
+        lev = O.forest_levels(dim, n, tree)
+        fl = rng.choice([-1, 0, 1], size=lev.size, p=[0.4, 0.3, 0.3]).astype(np.int32)
+        fl[(fl == 1) & (lev >= 3)] = 0
+        fl[(fl == -1) & (lev <= 0)] = 0
+        new = X.adapt_forest(dim, n, tree, fl)
+        nd = (dim + 1) * (k + 1) ** dim
+        n_old, n_new = lev.size, O.forest_levels(dim, n, new).size
+        u = rng.standard_normal(n_old * nd)
+        want = O.adapt_forest_transfer(dim, k, n, tree, new, u, n_new * nd)
+        a = torch.from_numpy(u).cuda()
+        b = torch.zeros(n_new * nd, dtype=torch.float64, device="cuda")
+        X.adapt_forest_transfer(ctx, dim, k, n, tree, new, a, b)
+        got = b.cpu().numpy()
+        assert np.max(np.abs(got - want)) <= 1e-12 * np.max(np.abs(want))
+        tree = new
