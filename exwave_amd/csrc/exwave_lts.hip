@@ -78,6 +78,10 @@ struct UpdatePlan {
   Pass same;
   List upd;
   unsigned src_sel = 0;  // source cells (bit s) among the update cells
+  // Box meshes: the update cells whose neighbours all belong to the cluster
+  // (no hanging face) take the plain operator sweep, state += M^-1 A(CK), as
+  // global ADER does; same.face and upd restricted to the others
+  List plain, rest_face, rest_upd;
 };
 struct ClusterPlan {
   UpdatePlan main;
@@ -267,7 +271,7 @@ UpdatePlan plan_update(exw_lts L, ListBuilder &B, NcBuilder &NB, const std::vect
   const int *cl = L->cluster.data();
   const std::vector<int> &off = L->soff;
   const std::vector<Slot> &sl = L->slots;
-  std::vector<char> nbrU(n, 0), E(n, 0);
+  std::vector<char> nbrU(n, 0), E(n, 0), plain(n, 0);
   for (int c = 0; c < n; ++c) {
     if (U[c]) continue;
     for (int i = off[c]; i < off[c + 1]; ++i)
@@ -325,19 +329,28 @@ UpdatePlan plan_update(exw_lts L, ListBuilder &B, NcBuilder &NB, const std::vect
       E[c] = cl[c] == X && (U[c] || nbrU[c]);
       if (E[c]) ck.push_back(c);
     }
+    std::vector<int> rfc;
+    std::vector<unsigned> rfmask;
     for (int c = 0; c < n; ++c) {
       if (!U[c]) continue;
       unsigned mc = 0, mn = 0;
+      bool inner = op->cart;  // every neighbour in the cluster, no hanging face
       for (int i = off[c]; i < off[c + 1]; ++i) {
         const int nb = sl[i].nb;
+        if (is_nc(c, sl[i].f) || (nb >= 0 && cl[nb] != X)) inner = false;
         if (!(nb < 0 || U[nb] || (E[nb] && cl[nb] == X))) continue;
         const unsigned bit = 1u << (4 * sl[i].f + sl[i].j);
         (is_nc(c, sl[i].f) ? mn : mc) |= bit;
       }
+      plain[c] = inner;
       if (mc) {
         fc.push_back(c);
         fmask.push_back(mc);
         ftarget.push_back(0);
+        if (!inner) {
+          rfc.push_back(c);
+          rfmask.push_back(mc);
+        }
       }
       if (mn) {
         ncc.push_back(c);
@@ -349,13 +362,21 @@ UpdatePlan plan_update(exw_lts L, ListBuilder &B, NcBuilder &NB, const std::vect
     P.same.ck = B.add(ck, zm, none);
     P.same.face = B.add(fc, fmask, ftarget);
     P.same.nc[0] = NB.add(op, ncc, ncm);
+    P.rest_face = B.add(rfc, rfmask, std::vector<unsigned char>(rfc.size(), 0));
   }
-  std::vector<int> uc;
+  std::vector<int> uc, pc, rc;
   for (int c = 0; c < n; ++c)
-    if (U[c]) uc.push_back(c);
+    if (U[c]) {
+      uc.push_back(c);
+      (plain[c] ? pc : rc).push_back(c);
+    }
   std::vector<unsigned> zm(uc.size(), 0);
   std::vector<unsigned char> none(uc.size(), 0);
   P.upd = B.add(uc, zm, none);
+  P.plain = B.add(pc, std::vector<unsigned>(pc.size(), 0),
+                  std::vector<unsigned char>(pc.size(), 0));
+  P.rest_upd = B.add(rc, std::vector<unsigned>(rc.size(), 0),
+                     std::vector<unsigned char>(rc.size(), 0));
   for (int s = 0; s < op->n_src; ++s)
     if (U[op->h_src_cells[s]]) P.src_sel |= 1u << s;
   return P;
@@ -399,6 +420,22 @@ int launch_sweep(exw_lts L, const List &l, int mode, const double *src, double f
   return EXW_OK;
 }
 
+// state += M^-1 A(src) on the listed cells: the Cartesian operator sweep
+// with a cell list, the second sweep of global ADER (exw_ader_step)
+int launch_plain(exw_lts L, const List &l, const double *src, double *state) {
+  if (l.n == 0) return EXW_OK;
+  exw_op op = L->op;
+  OpLaunch fn = kernels(op->dim).op[op->degree][0][1];
+  Epilogue ep{};
+  ep.b = state;
+  ep.o1 = state;
+  ep.a1 = 1.0;
+  ep.cell_list = L->d_cells + l.off;
+  ep.n_list = l.n;
+  LHIP(fn(op->view(), src, op->d_tab, op->host_tab.data(), ep, op->ctx->stream));
+  return EXW_OK;
+}
+
 // the subface integrals of the listed hanging-face entries (masks per
 // subface), from src on both sides, times fac, added into out
 int launch_nc_list(exw_lts L, const List &l, const double *src, double fac, double *out) {
@@ -432,19 +469,28 @@ int run_update(exw_lts L, const UpdatePlan &P, double *state, const double tside
     if ((rc = launch_nc_list(L, S.nc[0], L->tempsrc, -1.0, L->fsum))) return rc;
     if ((rc = launch_nc_list(L, S.nc[1], L->tempsrc, -1.0, L->fm))) return rc;
   }
+  // the update's volume term needs the CK integral over [0, dt] about the
+  // cells' own time: the same-cluster pass computes it over [ts0, ts1] about
+  // ts0 for every update cell, the same integral when ts1 - ts0 = dt
+  const bool reuse = std::fabs((ts[1] - ts[0]) - dt) <= 1e-14 * std::fabs(dt);
+  // then the inner cells' face and volume terms all come from that one
+  // integral: their update is the plain operator sweep (no flux memory or
+  // face sum can be pending on them, all their neighbours being in the cluster)
+  const bool fast = reuse && P.plain.n > 0;
   if ((rc = launch_ck_list(L, P.same.ck, state, ts[1], ts[0], true, L->tempsrc))) return rc;
-  if ((rc = launch_sweep(L, P.same.face, LTS_ACC, L->tempsrc, -1.0, nullptr, false, true)))
+  if ((rc = launch_sweep(L, fast ? P.rest_face : P.same.face, LTS_ACC, L->tempsrc, -1.0, nullptr,
+                         false, true)))
     return rc;
   if ((rc = launch_nc_list(L, P.same.nc[0], L->tempsrc, -1.0, L->fsum))) return rc;
-  // the update's volume term needs the CK integral over [0, dt] about the
-  // cells' own time: the same-cluster pass just computed it over [ts0, ts1]
-  // about ts0 for every update cell, the same integral when ts1 - ts0 = dt
   const double *vsrc = L->tempsrc;
-  if (!(std::fabs((ts[1] - ts[0]) - dt) <= 1e-14 * std::fabs(dt))) {
+  if (!reuse) {
     if ((rc = launch_ck_list(L, P.upd, state, dt, 0.0, false, L->V))) return rc;
     vsrc = L->V;
   }
-  if ((rc = launch_sweep(L, P.upd, LTS_UPDATE, vsrc, -1.0, state, use_fm, false))) return rc;
+  if ((rc = launch_sweep(L, fast ? P.rest_upd : P.upd, LTS_UPDATE, vsrc, -1.0, state, use_fm,
+                         false)))
+    return rc;
+  if (fast && (rc = launch_plain(L, P.plain, vsrc, state))) return rc;
   return source_add_selected(L->op, t_cells, t_cells + dt, P.src_sel, state);
 }
 
