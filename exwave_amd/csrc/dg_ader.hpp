@@ -59,6 +59,19 @@ struct CkShape {
   }();
 };
 
+// Cells per workgroup of the CK kernel.  Its 18 barrier-separated passes per
+// cell wait on every wave of the group, so fewer cells per group pay: in 3D
+// one cell per group from 100 nodes on (k=4, 64^3: 1.041 against 1.117 ms
+// with two), two below (k=3: 0.315 against 0.323 ms with four); 2D keeps the
+// operator's grouping.
+template <int DIM, int K>
+struct CkGroup {
+  static constexpr int ND = Shape<DIM, K>::ND;
+  static constexpr int MAXC = DIM == 3 ? (ND >= 100 ? 1 : 2) : 16;
+  static constexpr int CPB = Shape<DIM, K>::CPB < MAXC ? Shape<DIM, K>::CPB : MAXC;
+  static constexpr int BLOCK = ((CPB * ND + 63) / 64) * 64;
+};
+
 template <int DIM, int N>
 __device__ __forceinline__ constexpr int ipow_c() {
   return DIM == 2 ? N * N : N * N * N;
@@ -232,14 +245,14 @@ __device__ __forceinline__ void pass(const Number *M, const Number (*in)[LD], Nu
 }  // namespace ck
 
 template <int DIM, int K, typename Number, bool CART>
-__global__ void __launch_bounds__((Shape<DIM, K>::BLOCK))
+__global__ void __launch_bounds__((CkGroup<DIM, K>::BLOCK))
     ader_ck_kernel(const DevMesh mesh, const double *__restrict__ src,
                    const double *__restrict__ recon, const double *__restrict__ cktab,
                    const CkTab ckt, double t2, double t1, double te_all, const CellSel sel,
                    double *__restrict__ tempsrc) {
   using S = Shape<DIM, K>;
   using CS = CkShape<K>;
-  constexpr int N = S::N, ND = S::ND, NF = S::NF, CPB = S::CPB;
+  constexpr int N = S::N, ND = S::ND, NF = S::NF, CPB = CkGroup<DIM, K>::CPB;
   constexpr int NL = CS::N_LEVELS, NSTEPS = CS::N_STEPS;
   // the lower levels' contributions, one after the other in s_c
   constexpr int LOW = CS::low_off(DIM, NL) > 0 ? CS::low_off(DIM, NL) : 1;

@@ -172,13 +172,13 @@ template <int DIM, int K, typename Number, bool CART>
 hipError_t launch_ck(const DevMesh &m, const double *src, const double *recon,
                      const double *cktab, const CkTab &ck, double t2, double t1, double te,
                      const CellSel *sel, double *out, hipStream_t s) {
-  using S = Shape<DIM, K>;
+  using G = CkGroup<DIM, K>;
   CellSel cs{};
   if (sel) cs = *sel;
   const int n = cs.list ? cs.n : m.n_cells;
-  const int grid = (n + S::CPB - 1) / S::CPB;
+  const int grid = (n + G::CPB - 1) / G::CPB;
   if (grid == 0) return hipSuccess;
-  hipLaunchKernelGGL((ader_ck_kernel<DIM, K, Number, CART>), dim3(grid), dim3(S::BLOCK), 0, s,
+  hipLaunchKernelGGL((ader_ck_kernel<DIM, K, Number, CART>), dim3(grid), dim3(G::BLOCK), 0, s,
                      m, src, recon, cktab, ck, t2, t1, te, cs, out);
   return hipGetLastError();
 }
