@@ -355,9 +355,9 @@ int exw_run(exw_ctx ctx, const exw_problem *p, char *log, int log_len, double *w
   if (!ctx || !p) return EXW_EINVAL;
   const bool adaptive = p->n_adaptive_refinements > 0;
   const bool lts = p->integrator == EXW_ADER_LTS;
-  if (adaptive && (lts || p->adaptive_refinement_interval < 1 || p->n_refinements < 0 ||
+  if (adaptive && (p->adaptive_refinement_interval < 1 || p->n_refinements < 0 ||
                    p->n_refinements + p->n_adaptive_refinements > exw::Forest::MAX_LEVEL))
-    return EXW_EUNSUP;  // adaptivity without LTS
+    return EXW_EINVAL;
   const int nwork = lts ? 0 : exw_time_step_work_vectors(p->integrator);
   if (nwork < 0) return EXW_EINVAL;
   int nbase = 1;
@@ -430,10 +430,24 @@ int exw_run(exw_ctx ctx, const exw_problem *p, char *log, int log_len, double *w
       nst.release();
       return r;
     }
+    // make_dofs: compute_time_step_size (explicit_wave.cc:222, 529-545); with
+    // ADER-LTS the operator's setup builds new clusters from that step,
+    // starting at the current time, and the time control takes the slowest
+    // cluster's step (wave_equation_operations.cc:353-379,
+    // include/cluster_manager.templates.h:320-484), which adapt_mesh keeps
+    // (explicit_wave.cc:324-327)
+    exw_lts_destroy(L);  // it refers to the old operator
+    L = nullptr;
     st.release();
     st = std::move(nst);
     tree = std::move(tnew);
-    dt = cfl * exw_op_min_vertex_distance_over_c(st.op);  // compute_time_step_size
+    dt = cfl * exw_op_min_vertex_distance_over_c(st.op);
+    if (lts) {
+      if ((r = exw_lts_create(st.op, cfl, dt, p->lts_max_clusters, p->lts_max_diff, time,
+                              p->ader_recon, &L)))
+        return r;
+      exw_lts_info(L, nullptr, nullptr, nullptr, &dt);
+    }
     return EXW_OK;
   };
   int pos = 0;

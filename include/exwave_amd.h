@@ -424,7 +424,8 @@ typedef struct exw_problem {
   /* adapt_mesh every adaptive_refinement_interval steps and
    * n_adaptive_refinements times before the loop (explicit_wave.cc:704-716,
    * 789-791) on the forest of exw_mesh_forest, levels n_refinements ..
-   * n_refinements + n_adaptive_refinements; not with EXW_ADER_LTS */
+   * n_refinements + n_adaptive_refinements; with EXW_ADER_LTS the clusters are
+   * rebuilt after every adaptation, as the operator's setup does in make_dofs */
   int n_adaptive_refinements, adaptive_refinement_interval;
 } exw_problem;
 

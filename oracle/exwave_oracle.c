@@ -2803,7 +2803,6 @@ static int uniform_tree(int dim, int depth, int *out, int pos) {
 
 int ox_run(const ox_problem *P, char *log, int log_len, double *final_energy) {
   const int adaptive = P->n_adaptive_refinements > 0;
-  if (adaptive && P->integrator == OX_ADER_LTS) return -1; /* adaptivity without LTS */
   /* adaptive runs: the forest of the globally refined box, every base cell
    * refined n_refinements times (levels min_level .. max_level,
    * source/explicit_wave.cc:272-273) */
@@ -2885,8 +2884,18 @@ int ox_run(const ox_problem *P, char *log, int log_len, double *final_energy) {
     tree = tnew;                                                                           \
     tnew = sw_;                                                                            \
     ntree = nt2;                                                                           \
-    /* compute_time_step_size (explicit_wave.cc:326-327, 529-545) */                       \
+    /* make_dofs: compute_time_step_size (explicit_wave.cc:222, 529-545); with   \
+     * ADER-LTS the operator's setup then builds new clusters from it, starting   \
+     * at the current time, and sets the slowest cluster's step (:1353-1379,     \
+     * include/cluster_manager.templates.h:320-484); adapt_mesh keeps that step \
+     * (explicit_wave.cc:324-327) */                                             \
     dt = cfl * m->min_vertex_distance_over_c;                                              \
+    if (lts) {                                                                             \
+      ox_lts_free(lts);                                                                    \
+      lts = ox_lts_create(m, k, cfl, dt, P->lts_max_clusters, P->lts_max_diff, time,       \
+                          P->ader_recon);                                                  \
+      dt = ox_lts_global_dt(lts);                                                          \
+    }                                                                                      \
     free(err);                                                                             \
     free(lev);                                                                             \
     free(fl);                                                                              \

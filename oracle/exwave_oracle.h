@@ -162,7 +162,7 @@ typedef struct ox_problem {
   ox_exact exact;
   int lts_max_clusters, lts_max_diff; /* max_n_clusters, max_diff_clusters */
   int post_pressure;                   /* also log 'error p post' (else -1) */
-  int n_adaptive_refinements;          /* levels above n_refinements (forest meshes; no LTS) */
+  int n_adaptive_refinements;          /* levels above n_refinements (forest meshes) */
   int adaptive_refinement_interval;
 } ox_problem;
 int ox_run(const ox_problem *p, char *log, int log_len, double *final_state_energy);

@@ -268,3 +268,28 @@ def test_malformed_and_unbalanced_forests_are_rejected():
         X.mesh_forest(dim, n, bad)
     with pytest.raises(ValueError):
         O.Mesh.forest(dim, n, bad)
+
+
+def test_oracle_adaptive_lts_run_rebuilds_clusters():
+    """ADER-LTS with adaptivity (the operator's setup rebuilds the clusters in
+    make_dofs after every adaptation): the run keeps the global step of the
+    new clusters (twice the refined cells' CFL step here) and its errors stay
+    at the level of the adaptive global-ADER run of the same problem"""
+    def prob(integ):
+        p = O.Problem()
+        p.dim, p.degree, p.n_intervals, p.n_refinements = 2, 3, 4, 0
+        p.transform, p.transform_factor, p.boundary_id = O.TRANSFORM_AXIS_2PI, 0.03, 2
+        p.integrator, p.cfl_number = integ, 0.2
+        p.max_time_steps, p.final_time, p.output_every_time = 100000, 0.15, 0.05
+        p.post_pressure, p.n_adaptive_refinements, p.adaptive_refinement_interval = 0, 2, 6
+        p.lts_max_clusters, p.lts_max_diff, p.ader_recon = 3, 1, 0
+        p.exact = O.Exact.make(2, [[2, 1], [1, 3]])
+        return p
+    sa, ra, _ = O.run(prob(O.ADER))
+    sl, rl, _ = O.run(prob(O.ADER_LTS))
+    assert len(ra) == len(rl) == 4
+    # two clusters after the first adaptation: half as many global steps
+    assert sl < sa and abs(2 * sl - sa) <= 2
+    for a, b in zip(ra, rl):
+        for j in (2, 3):
+            assert np.isfinite(b[j]) and abs(b[j] - a[j]) <= 0.1 * a[j]

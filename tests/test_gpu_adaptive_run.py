@@ -15,6 +15,8 @@ CASES = [  # dim, intervals, degree, transform, factor, integrator, interval, mo
     (2, 8, 3, O.TRANSFORM_AXIS_2PI, 0.03, O.LSRK45R2, 10, [[2, 1], [1, 3]], 0, 1),
     (2, 6, 4, O.TRANSFORM_AXIS_2PI, 0.03, O.ADER, 5, [[1, 2], [3, 1]], 0, 1),
     (3, 4, 2, O.TRANSFORM_AXIS_2PI, 0.03, O.LSRK45R2, 8, [[1, 2, 3], [2, 1, 1]], 0, 1),
+    # ADER-LTS: clusters rebuilt after every adaptation (make_dofs -> setup)
+    (2, 4, 3, O.TRANSFORM_AXIS_2PI, 0.03, O.ADER_LTS, 6, [[2, 1], [1, 3]], 0, 2),
     # several levels on the forest (levels n_refinements .. + n_adaptive)
     (2, 4, 3, O.TRANSFORM_AXIS_2PI, 0.03, O.LSRK45R2, 10, [[2, 1], [1, 3]], 1, 2),
     (2, 5, 2, O.TRANSFORM_AXIS_2PI, 0.03, O.ADER, 6, [[1, 2], [3, 1]], 0, 3),
@@ -49,6 +51,7 @@ def test_adaptive_run_matches_oracle(X, ctx, case):
         p.integrator, p.cfl_number = integ, 0.2
         p.max_time_steps, p.final_time, p.output_every_time = 100000, 0.15, 0.05
         p.post_pressure, p.n_adaptive_refinements, p.adaptive_refinement_interval = 1, nada, interval
+        p.lts_max_clusters, p.lts_max_diff, p.ader_recon = 3, 1, 0
     po.exact = O.Exact.make(dim, modes)
     px.precision = X.FP64
     px.modes = X.Modes.make(dim, modes)
