@@ -286,3 +286,37 @@ def test_mesh_check_rejects_non_standard_orientation(dim):
     j = int(np.flatnonzero(nbr[:, 1] >= 0)[0])
     nbr[j, 1] = nbr[j, 1] + 1 if nbr[j, 1] + 1 < V.shape[0] else 0
     assert _check(dim, V, nbr) < 0
+
+
+def test_host_entry_points_reject_empty_and_malformed_input():
+    """Empty meshes, empty indicator arrays and malformed forests are
+    rejected with EXW_EINVAL by the host-side entry points (no device use)."""
+    import ctypes as C
+
+    import exwave_amd as X
+    L = X.load()
+    ip, dp = C.POINTER(C.c_int), C.POINTER(C.c_double)
+    V = np.zeros((1, 4, 2))
+    nb = -np.ones((1, 4), np.int32)
+    vp, nbp = V.ctypes.data_as(dp), nb.ctypes.data_as(ip)
+    assert L.exw_mesh_check(2, 0, vp, nbp, 0, None) < 0          # no cells
+    assert L.exw_mesh_check(4, 1, vp, nbp, 0, None) < 0          # bad dimension
+    out = np.zeros(4, np.int32)
+    cl, fl = out.ctypes.data_as(ip), out.ctypes.data_as(ip)
+    a, b, c = C.c_int(), C.c_int(), C.c_int()
+    assert L.exw_lts_categorize(2, 0, vp, nbp, 0.1, 3, 1, cl, fl, C.byref(a), C.byref(b),
+                                C.byref(c)) < 0
+    err = np.zeros(1)
+    assert L.exw_refinement_flags(0, err.ctypes.data_as(dp), out.ctypes.data_as(ip), 0, 1,
+                                  1.0, out.ctypes.data_as(ip)) < 0
+    # forests: wrong length, entries other than 0/1, faces two levels apart
+    for tree in ([1, 0, 0, 0], [2], [1, 0, 1, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0]):
+        with pytest.raises(X.ExwError):
+            X.mesh_forest(2, 1, tree)
+    with pytest.raises(X.ExwError):
+        X.adapt_forest(2, 1, [1, 0, 0, 0], [0, 0, 0])
+    # a one-cell forest, flagged, refines; a one-level family coarsens back
+    t1 = X.adapt_forest(2, 1, [0], [1])
+    assert list(t1) == [1, 0, 0, 0, 0]
+    assert list(X.adapt_forest(2, 1, t1, [-1, -1, -1, -1])) == [0]
+    assert list(X.adapt_forest(2, 1, t1, [-1, -1, -1, 0])) == list(t1)
