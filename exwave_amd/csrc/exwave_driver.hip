@@ -490,7 +490,8 @@ int exw_run(exw_ctx ctx, const exw_problem *p, char *log, int log_len, double *w
       exw_sync(ctx);
       auto t0 = std::chrono::steady_clock::now();
       if (lts)
-        rc = exw_lts_step(L, time, st.tmp, st.sol);  // ArbitraryHighOrderDGLTS (:54-63)
+        // ArbitraryHighOrderDGLTS (:54-63), in place on the state
+        rc = exw_lts_step(L, time, st.tmp, st.tmp), std::swap(st.sol, st.tmp);
       else
         rc = exw_time_step(st.op, p->integrator, dt, st.tmp, st.sol, st.work, p->ader_recon,
                            p->precision);

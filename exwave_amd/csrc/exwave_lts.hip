@@ -691,15 +691,17 @@ int exw_lts_cells(exw_lts L, int *cluster, int *flags) {
 }
 
 int exw_lts_step(exw_lts L, double time_now, const double *src, double *dst) {
-  if (!L || !src || !dst || src == dst) return lfail(EXW_EINVAL, "exw_lts_step: bad argument");
+  if (!L || !src || !dst) return lfail(EXW_EINVAL, "exw_lts_step: bad argument");
   exw_op op = L->op;
   const int64_t N = op->n_dofs();
   hipStream_t st = op->ctx->stream;
   const double D = L->fastest, tol = L->rel_tol * D;
   const int nc = L->n_clusters;
   int rc;
-  // state = src, updated in place in dst (ClusterManager::perform_time_step, :488-803)
-  LHIP(hipMemcpyAsync(dst, src, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
+  // state = src, updated in place in dst (ClusterManager::perform_time_step,
+  // :488-803); src is read by nothing else, so src == dst steps in place
+  if (dst != src)
+    LHIP(hipMemcpyAsync(dst, src, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
   double *state = dst;
   if (L->recon && (rc = exw_op_apply(op, state, L->igd, EXW_FP64))) return rc;
   for (size_t cy = 0; cy < L->order.size(); ++cy) {

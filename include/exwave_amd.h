@@ -260,7 +260,8 @@ int exw_lts_info(exw_lts L, int *n_clusters, int *cluster_diff, int *n_updates,
 int exw_lts_cells(exw_lts L, int *cluster, int *flags);
 /* ArbitraryHighOrderDGLTS::perform_time_step (source/time_integrators.cc:54-63):
  * dst = state after one global step; time_now = the time after
- * TimeControl::advance_time_step.  src and dst are distinct device vectors. */
+ * TimeControl::advance_time_step.  src == dst steps in place (no copy of the
+ * state); otherwise src is left unchanged. */
 int exw_lts_step(exw_lts L, double time_now, const double *src, double *dst);
 
 /* ---- problem setup and the whole solve ---------------------------------- */

@@ -59,14 +59,20 @@ def test_lts_steps_match_oracle(X, ctx, case, recon):
         t = 0.0
         for step in range(3):
             t += lts.global_dt
-            lts.step(t, src, dst)
+            if step == 1:  # the second step in place (src == dst)
+                lts.step(t, src, src)
+                cur = src
+            else:
+                lts.step(t, src, dst)
+                cur = dst
             out = np.zeros_like(uo)
             L.ox_lts_step(h, t, O._p(uo), O._p(out))
             uo = out
-            got = dst.cpu().numpy()
+            got = cur.cpu().numpy()
             rel = np.max(np.abs(got - uo)) / np.max(np.abs(uo))
             assert rel <= 1e-12, (step, rel)
-            src, dst = dst, src
+            if step != 1:
+                src, dst = dst, src
     finally:
         L.ox_lts_free(h)
 
