@@ -70,6 +70,9 @@ struct CkGroup {
   static constexpr int MAXC = DIM == 3 ? (ND >= 100 ? 1 : 2) : 16;
   static constexpr int CPB = Shape<DIM, K>::CPB < MAXC ? Shape<DIM, K>::CPB : MAXC;
   static constexpr int BLOCK = ((CPB * ND + 63) / 64) * 64;
+  // 3D k=6 sits one register above 6 waves per SIMD (81 VGPRs): held to 6 it
+  // spills 2 and runs 0.633 against 0.735 ms at 32^3 (FP64); elsewhere no hint
+  static constexpr int MIN_WAVES = (DIM == 3 && K == 6) ? 6 : 1;
 };
 
 template <int DIM, int N>
@@ -217,8 +220,9 @@ constexpr bool lines_pay() {
 // One tensor pass along direction DIR of the matrix M (NO x NI, row-major) on
 // every field: the input has sizes (A0, A1, A2) with A_DIR = NI, the output
 // the same with NO in direction DIR.  Lane o computes output entry o.
-template <int DIM, int DIR, int NI, int NO, int A0, int A1, int A2, int LD, typename Number>
-__device__ __forceinline__ void pass(const Number *M, const Number (*in)[LD], Number (*out)[LD],
+template <int DIM, int DIR, int NI, int NO, int A0, int A1, int A2, int LD, typename Number,
+          int LDI = LD>
+__device__ __forceinline__ void pass(const Number *M, const Number (*in)[LDI], Number (*out)[LD],
                                      int o) {
   constexpr int O0 = DIR == 0 ? NO : A0, O1 = DIR == 1 ? NO : A1, O2 = DIR == 2 ? NO : A2;
   constexpr int TOTAL = O0 * O1 * (DIM == 3 ? O2 : 1);
@@ -242,10 +246,37 @@ __device__ __forceinline__ void pass(const Number *M, const Number (*in)[LD], Nu
   }
 }
 
+// The same pass, its output entry o returned in res (all fields) instead of
+// stored: the last interpolation pass, whose entry o is lane o's own node.
+template <int DIM, int DIR, int NI, int NO, int A0, int A1, int A2, int LDI, typename Number>
+__device__ __forceinline__ bool pass_reg(const Number *M, const Number (*in)[LDI], int o,
+                                         Number *res) {
+  constexpr int O0 = DIR == 0 ? NO : A0, O1 = DIR == 1 ? NO : A1, O2 = DIR == 2 ? NO : A2;
+  constexpr int TOTAL = O0 * O1 * (DIM == 3 ? O2 : 1);
+  constexpr int SI[3] = {1, A0, A0 * A1};
+  if (o >= TOTAL) return false;
+  int idx[3] = {o % O0, (o / O0) % O1, DIM == 3 ? o / (O0 * O1) : 0};
+  const int r = idx[DIR];
+  idx[DIR] = 0;
+  const int b = idx[0] * SI[0] + idx[1] * SI[1] + (DIM == 3 ? idx[2] * SI[2] : 0);
+  Number mr[NI];
+#pragma unroll
+  for (int c = 0; c < NI; ++c) mr[c] = M[r * NI + c];
+#pragma unroll
+  for (int f = 0; f <= DIM; ++f) {
+    Number acc = Number(0);
+#pragma unroll
+    for (int c = 0; c < NI; ++c) acc += mr[c] * in[f][b + c * SI[DIR]];
+    res[f] = acc;
+  }
+  return true;
+}
+
 }  // namespace ck
 
 template <int DIM, int K, typename Number, bool CART>
 __global__ void __launch_bounds__((CkGroup<DIM, K>::BLOCK))
+    __attribute__((amdgpu_waves_per_eu(CkGroup<DIM, K>::MIN_WAVES)))
     ader_ck_kernel(const DevMesh mesh, const double *__restrict__ src,
                    const double *__restrict__ recon, const double *__restrict__ cktab,
                    const CkTab ckt, double t2, double t1, double te_all, const CellSel sel,
@@ -335,7 +366,11 @@ __global__ void __launch_bounds__((CkGroup<DIM, K>::BLOCK))
   }
 #pragma unroll
   for (int f = 0; f < NF; ++f) contrib[f] = Number(a2 - a1) * u[f] + c1 * up[f];
-  __syncthreads();
+  // the node form of the derivative reads other nodes' entries of s_a: wait
+  // for every lane before overwriting; the line form's node half (and the
+  // reconstruction input) read only the lane's own entries
+  if constexpr (!(CART && ck::lines_fit<DIM, N, ND>() && ck::lines_pay<DIM, K, Number>()))
+    __syncthreads();
   if (lane_ok) {
 #pragma unroll
     for (int f = 0; f < NF; ++f) s_a[lc][f][q] = -up[f];
@@ -407,41 +442,50 @@ __global__ void __launch_bounds__((CkGroup<DIM, K>::BLOCK))
     }
   });
 
-  // interpolate the lower levels' contributions back up and add
+  // interpolate the lower levels' contributions back up and add: the first
+  // pass of a level reads its contribution where it lies in s_c, the last one
+  // hands lane o its own output entry (level 0: the lane's node)
   if constexpr (NL > 1) {
     ck::static_for<NL - 1>([&](auto L_) {
       constexpr int LV = NL - 1 - decltype(L_)::value;  // NL-1 .. 1
       constexpr int NLO = CS::nl(LV), NHI = NLO + 2;
-      constexpr int NDHI = DIM == 2 ? NHI * NHI : NHI * NHI * NHI;
       constexpr int OFF = CS::low_off(DIM, LV);
       const Number *Vt = s_ck + ckt.off_V[LV];
-      // copy the level's contribution to s_a, then DIM passes s_a <-> s_b
-      if (lane_ok && q < CS::nd(DIM, LV))
-#pragma unroll
-        for (int f = 0; f < NF; ++f) s_a[lc][f][q] = s_c[lc][f][OFF + q];
-      __syncthreads();
-      int par = 0;
+      const Number(*lvl)[LOW] = reinterpret_cast<const Number(*)[LOW]>(&s_c[lcc][0][OFF]);
+      bool in_b = true;  // the latest pass output is in s_b (else s_a)
       ck::static_for<DIM>([&](auto D_) {
         constexpr int DIR = decltype(D_)::value;
         constexpr int A0 = DIR > 0 ? NHI : NLO, A1 = DIR > 1 ? NHI : NLO, A2 = NLO;
-        Number(*in)[ND] = par ? s_b[lcc] : s_a[lcc];
-        Number(*out)[ND] = par ? s_a[lcc] : s_b[lcc];
-        if (lane_ok) ck::pass<DIM, DIR, NLO, NHI, A0, A1, A2, ND>(Vt, in, out, q);
-        __syncthreads();
-        par ^= 1;
-      });
-      // the interpolated contribution (level LV-1) is in s_b (DIM odd) or s_a
-      Number(*res)[ND] = par ? s_b[lcc] : s_a[lcc];
-      if (lane_ok && q < NDHI) {
-        if constexpr (LV - 1 == 0) {
-#pragma unroll
-          for (int f = 0; f < NF; ++f) contrib[f] += res[f][q];
+        if constexpr (DIR < DIM - 1) {
+          if constexpr (DIR == 0) {
+            if (lane_ok) ck::pass<DIM, DIR, NLO, NHI, A0, A1, A2, ND, Number, LOW>(Vt, lvl, s_b[lcc], q);
+            in_b = true;
+          } else {
+            if (lane_ok)
+              ck::pass<DIM, DIR, NLO, NHI, A0, A1, A2, ND>(Vt, in_b ? s_b[lcc] : s_a[lcc],
+                                                         in_b ? s_a[lcc] : s_b[lcc], q);
+            in_b = !in_b;
+          }
+          __syncthreads();
         } else {
+          Number res[NF];
+          bool have = false;
+          if (lane_ok)
+            have = ck::pass_reg<DIM, DIR, NLO, NHI, A0, A1, A2, ND>(
+                Vt, in_b ? s_b[lcc] : s_a[lcc], q, res);
+          if (have) {
+            if constexpr (LV - 1 == 0) {
 #pragma unroll
-          for (int f = 0; f < NF; ++f) s_c[lc][f][CS::low_off(DIM, LV - 1) + q] += res[f][q];
+              for (int f = 0; f < NF; ++f) contrib[f] += res[f];
+            } else {
+#pragma unroll
+              for (int f = 0; f < NF; ++f) s_c[lc][f][CS::low_off(DIM, LV - 1) + q] += res[f];
+            }
+          }
+          // the next level's first pass reads s_c; the last level needs no wait
+          if constexpr (LV - 1 > 0) __syncthreads();
         }
-      }
-      __syncthreads();
+      });
     });
   }
 
