@@ -37,6 +37,7 @@ and how many results stay within 1.1x of the one-at-a-time FP64 L2 errors.
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -111,12 +112,14 @@ def stage_kernel(k, fp64):
     return "cart", f"dg_operator_cart_kernel<3,{k},{'double' if fp64 else 'float'}>"
 
 
-def stream_ceiling(reps=10):
-    """Measured HBM stream rates on this GPU (torch's elementwise kernels on
-    1 GiB FP64 vectors, HIP events on the launch stream, best of `reps`):
-    copy (1 read + 1 write), add (2 reads + 1 write), and the stage's own
-    pattern done in two passes (b += a; c = b + s*a: 2+1 and 2+1).  The best
-    rate is the practical ceiling a bandwidth-bound kernel can reach here."""
+def stream_ceiling(reps=10, ctx=None):
+    """Measured HBM stream rates on this GPU (1 GiB FP64 vectors, HIP events on
+    the launch stream, best of `reps`): torch's copy (1 read + 1 write) and add
+    (2 reads + 1 write), and, with `ctx`, the library's unfused RKVectorUpdater
+    (exw_rk_update: reads a, b, writes b, c -- the fused stage's own 2 + 2
+    pattern without the operator).  The best torch rate is the practical
+    ceiling of a bandwidth-bound kernel here; the updater's is the ceiling of
+    the stage's access pattern."""
     import torch
     n = 1 << 27
     stream = torch.cuda.current_stream()
@@ -139,6 +142,16 @@ def stream_ceiling(reps=10):
            "add_gbs": best(lambda: torch.add(a, b, alpha=0.5, out=c), 24 * n)}
     out["stream_ceiling_gbs"] = max(out["copy_gbs"], out["add_gbs"])
     out["method"] = "torch copy_ / add on 1 GiB FP64 vectors, best of %d, HIP events" % reps
+    if ctx is not None:
+        import exwave_amd as X
+        L = X.load()
+
+        def upd():
+            X.check(L.exw_rk_update(ctx.h, n, 1e-3, 2e-3, 0, C.c_void_p(a.data_ptr()),
+                                    C.c_void_p(b.data_ptr()), C.c_void_p(c.data_ptr())))
+        out["stage_pattern_gbs"] = best(upd, 32 * n)
+        out["stage_pattern_method"] = ("exw_rk_update (RKVectorUpdater, not last: reads 2 and "
+                                       "writes 2 FP64 vectors, the fused stage's pattern)")
     del a, b, c
     torch.cuda.empty_cache()
     return out
@@ -728,10 +741,12 @@ def main():
     rl["achieved_tflops"] = round(rl["flops_per_launch"] / (stage_ms / 1e3) / 1e12, 2)
     rl["flop_frac"] = round(rl["achieved_tflops"] / PEAK_FP64_TFLOPS, 4)
     try:
-        sc = stream_ceiling()
+        sc = stream_ceiling(ctx=ctx)
         rl["stream_ceiling_gbs"] = sc["stream_ceiling_gbs"]
         rl["stream"] = sc
         rl["frac_of_stream_ceiling"] = round(achieved / sc["stream_ceiling_gbs"], 4)
+        if sc.get("stage_pattern_gbs"):
+            rl["frac_of_stage_pattern_ceiling"] = round(achieved / sc["stage_pattern_gbs"], 4)
     except RuntimeError as ex:  # out of memory next to the workload: report none
         rl["stream_ceiling_gbs"] = None
         rl["stream_error"] = str(ex)[:200]

@@ -568,20 +568,39 @@ int exw_rk_stage(exw_op op, const double *src, const double *b, double *y_out, d
 }  // extern "C"
 
 namespace {
+// two entries per lane (16-byte accesses) where all three vectors are 16-byte
+// aligned, streaming (non-temporal) loads and stores: each entry is touched once
+__device__ __forceinline__ void rk_update_entry(double up, double f1, double f2, int is_last,
+                                                double *v2, double *v3) {
+  if (is_last) {
+    __builtin_nontemporal_store(__builtin_nontemporal_load(v3) + f1 * up, v2);
+  } else {
+    const double y = __builtin_nontemporal_load(v2) + f1 * up;
+    __builtin_nontemporal_store(y, v2);
+    __builtin_nontemporal_store(y + f2 * up, v3);
+  }
+}
+typedef double dbl2 __attribute__((ext_vector_type(2)));
 __global__ void __launch_bounds__(256)
     rk_update_kernel(int64_t n, double f1, double f2, int is_last, const double *__restrict__ v1,
                      double *__restrict__ v2, double *__restrict__ v3) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const double up = v1[i];
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const bool pairs = ((uintptr_t)v1 | (uintptr_t)v2 | (uintptr_t)v3) % 16 == 0;
+  const int64_t np = pairs ? n / 2 : 0;
+  const dbl2 *p1 = reinterpret_cast<const dbl2 *>(v1);
+  dbl2 *p2 = reinterpret_cast<dbl2 *>(v2), *p3 = reinterpret_cast<dbl2 *>(v3);
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < np; j += stride) {
+    const dbl2 up = __builtin_nontemporal_load(p1 + j);
     if (is_last) {
-      v2[i] = v3[i] + f1 * up;
+      __builtin_nontemporal_store(__builtin_nontemporal_load(p3 + j) + f1 * up, p2 + j);
     } else {
-      const double y = v2[i] + f1 * up;
-      v2[i] = y;
-      v3[i] = y + f2 * up;
+      const dbl2 y = __builtin_nontemporal_load(p2 + j) + f1 * up;
+      __builtin_nontemporal_store(y, p2 + j);
+      __builtin_nontemporal_store(y + f2 * up, p3 + j);
     }
   }
+  for (int64_t i = 2 * np + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    rk_update_entry(__builtin_nontemporal_load(v1 + i), f1, f2, is_last, v2 + i, v3 + i);
 }
 // out = sum_i coef[i] * in[i] over n_in <= 16 vectors (out may alias an input)
 struct LinComb {
