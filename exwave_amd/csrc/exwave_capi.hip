@@ -330,6 +330,9 @@ int exw_op_create_nc(exw_ctx ctx, int dim, int degree, int n_cells, const double
                      const double *speed, exw_op *out) {
   if (n_hanging < 0 || (n_hanging > 0 && !hanging))
     return fail(EXW_EINVAL, "exw_op_create_nc: bad hanging records");
+  if (n_hanging > 0 && (dim == 2 || dim == 3) && degree >= 1 && degree <= max_degree(dim) &&
+      !kernels(dim).nc[degree])
+    return fail(EXW_EUNSUP, "exw_op_create_nc: hanging faces not supported at this degree");
   return op_create(ctx, dim, degree, n_cells, vertices, neighbor, n_hanging, hanging, rho, speed,
                    out);
 }
@@ -1141,9 +1144,10 @@ int exw_estimate_error(exw_op op, const double *solution, double *tmp, double *e
     return fail(EXW_EINVAL, "exw_estimate_error: null argument");
   if (tmp == solution) return fail(EXW_EINVAL, "exw_estimate_error: tmp aliases solution");
   // apply(solution, tmp_vector) first (source/wave_equation_operations.cc:876)
+  EstLaunch le = kernels(op->dim).estimate[op->degree];
+  if (!le) return fail(EXW_EUNSUP, "error estimator not compiled for this degree");
   if (int rc = exw_op_apply(op, solution, tmp, EXW_FP64)) return rc;
-  EXW_HIP(kernels(op->dim).estimate[op->degree](op->view(), op->d_tab, solution, tmp,
-                                                 error_per_cell, op->ctx->stream));
+  EXW_HIP(le(op->view(), op->d_tab, solution, tmp, error_per_cell, op->ctx->stream));
   return EXW_OK;
 }
 

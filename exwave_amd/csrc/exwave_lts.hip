@@ -543,6 +543,8 @@ int exw_lts_create2(exw_op op, double cfl, double fastest_dt, int max_clusters, 
   if (!op || !out || !(fastest_dt > 0) || !(cfl > 0) || max_clusters < 1 || max_diff < 1 ||
       (flags & ~EXW_LTS_SPEED_AWARE))
     return lfail(EXW_EINVAL, "exw_lts_create: bad argument");
+  if (!kernels(op->dim).lts[op->degree][0] || !kernels(op->dim).ck[op->degree][0][0])
+    return lfail(EXW_EUNSUP, "exw_lts_create: local time stepping not compiled for this degree");
   const int n = op->n_cells, dim = op->dim;
   exw_lts L = new exw_lts_s;
   L->op = op;

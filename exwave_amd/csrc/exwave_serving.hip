@@ -64,7 +64,10 @@ int solve_steps(const exw_request &r, double &dt, double &t_end) {
 }
 
 int check_request(const exw_request &r) {
+  // the pools sweep cell lists, which the kernels of cells with more nodes than
+  // a workgroup has lanes (3D k >= 10) do not take
   if ((r.dim != 2 && r.dim != 3) || r.degree < 1 || r.degree > max_degree(r.dim) ||
+      (r.dim == 3 && r.degree > 9) ||
       r.n_refinements < 0 || r.n_refinements > 10 || !(r.rho > 0) || !(r.c > 0) ||
       !(r.final_time > 0) || !(r.cfl_number > 0) || r.modes.n_modes < 0 || r.modes.n_modes > 8 ||
       (r.precision != EXW_FP64 && r.precision != EXW_FP32 && r.precision != EXW_PRECISION_AUTO))

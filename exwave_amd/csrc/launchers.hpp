@@ -10,6 +10,7 @@
 #include "dg_misc.hpp"
 #include "dg_operator.hpp"
 #include "dg_operator_cart.hpp"
+#include "dg_operator_big.hpp"
 #include "dg_operator_gen.hpp"
 #include "dg_operator_pipe.hpp"
 #include "dg_nc.hpp"
@@ -18,10 +19,11 @@
 namespace exw {
 
 // Degrees compiled: 1..12 in 2D (the reference's range,
-// source/wave_equation_operations.cc:1695-1802), 1..9 in 3D, where one lane
-// per node still fits a 1024-lane workgroup (3D k >= 10 has 1331+ nodes).
+// source/wave_equation_operations.cc:1695-1802), 1..12 in 3D: up to k = 9 one lane per
+// node fits a 1024-lane workgroup; k >= 10 (1331+ nodes) takes the big-cell
+// operator kernel, and only the operator, projection and norms exist there.
 constexpr int MAX_DEGREE = 12;
-constexpr int MAX_DEGREE_3D = 9;
+constexpr int MAX_DEGREE_3D = 12;
 inline int max_degree(int dim) { return dim == 2 ? MAX_DEGREE : MAX_DEGREE_3D; }
 
 // (mesh, src, device basis table, host basis table, epilogue, stream)
@@ -168,6 +170,18 @@ hipError_t launch_op(const DevMesh &m, const double *src, const double *tab,
   return hipGetLastError();
 }
 
+// cells of more nodes than a workgroup has lanes (3D k >= 10), any geometry;
+// no cell lists (the serving pools stop at 3D k = 9)
+template <int DIM, int K, typename Number>
+hipError_t launch_op_big(const DevMesh &m, const double *src, const double *tab,
+                         const double *host_tab, const Epilogue &e, hipStream_t s) {
+  if (e.cell_list) return hipErrorInvalidValue;
+  if (m.n_cells == 0) return hipSuccess;
+  hipLaunchKernelGGL((dg_operator_big_kernel<DIM, K, Number>), dim3(m.n_cells),
+                     dim3(BigShape<DIM, K>::BLOCK), 0, s, m, src, tab, e);
+  return hipGetLastError();
+}
+
 template <int DIM, int K, typename Number, bool CART>
 hipError_t launch_ck(const DevMesh &m, const double *src, const double *recon,
                      const double *cktab, const CkTab &ck, double t2, double t1, double te,
@@ -268,7 +282,22 @@ hipError_t launch_post_error(const DevMesh &m, const double *tab, const DevModes
 template <int DIM, int K>
 void fill_dim_kernels(DimKernels &t) {
   t.coef = launch_cart_coef<DIM>;
-  if constexpr (K >= 1) {
+  if constexpr (K >= 1 && Shape<DIM, K>::ND > 1024) {
+    // more nodes than lanes: the operator (any geometry and precision), the
+    // projection and the norms; CK, LTS, hanging faces, the error estimator
+    // and the post-processing stay unsupported
+    static_assert(BigShape<DIM, K>::ENABLED, "big-cell kernel for cells of more than 1024 nodes");
+    for (int p = 0; p < 2; ++p)
+      for (int c = 0; c < 2; ++c)
+        t.op[K][p][c] = p == 0 ? launch_op_big<DIM, K, double> : launch_op_big<DIM, K, float>;
+    t.project[K] = launch_project<DIM, K>;
+    t.l2err[K] = launch_l2err<DIM, K>;
+    t.energy[K] = launch_energy<DIM, K>;
+    t.block[K] = BigShape<DIM, K>::BLOCK;
+    t.cells_per_block[K] = 1;
+    t.pipe[K] = false;
+    fill_dim_kernels<DIM, K - 1>(t);
+  } else if constexpr (K >= 1) {
     t.op[K][0][0] = launch_op<DIM, K, double, false>;
     t.op[K][0][1] = launch_op<DIM, K, double, true>;
     t.op[K][1][0] = launch_op<DIM, K, float, false>;
