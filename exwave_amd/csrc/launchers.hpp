@@ -10,6 +10,7 @@
 #include "dg_misc.hpp"
 #include "dg_operator.hpp"
 #include "dg_operator_cart.hpp"
+#include "dg_operator_cartg.hpp"
 #include "dg_operator_big.hpp"
 #include "dg_operator_gen.hpp"
 #include "dg_operator_pipe.hpp"
@@ -97,6 +98,15 @@ inline bool gen_kernel_enabled() {
   return on;
 }
 
+// The gathered-trace Cartesian kernel (dg_operator_cartg_kernel) where it
+// applies; EXW_CART_GATHER=0 selects dg_operator_cart_kernel.  Read at every
+// launch (a scan of the environment, far below a launch's cost), so that one
+// process can run both kernels and compare them.
+inline bool cart_gather_enabled() {
+  const char *v = getenv("EXW_CART_GATHER");
+  return !(v && v[0] == '0');
+}
+
 #ifndef EXW_PIPE_TPG
 #define EXW_PIPE_TPG 2
 #endif
@@ -139,6 +149,17 @@ hipError_t launch_op(const DevMesh &m, const double *src, const double *tab,
       else
         hipLaunchKernelGGL((dg_operator_pipe_kernel<DIM, K, TPG, false>), dim3(grid),
                            dim3(PipeShape<DIM, K>::BLOCK), 0, s, m, src, e);
+      return hipGetLastError();
+    }
+  }
+  if constexpr (CART && CartGShape<DIM, K>::ENABLED) {
+    // large box cells (3D k >= 6): neighbour traces gathered from memory, 7
+    // fields of LDS instead of 16 (3D k=6 32^3 LSRK45 stage: 107 -> 142 GDoF/s)
+    if (cart_gather_enabled()) {
+      const int n = e.cell_list ? e.n_list : m.n_cells;
+      if (n == 0) return hipSuccess;
+      hipLaunchKernelGGL((dg_operator_cartg_kernel<DIM, K, Number>), dim3(n),
+                         dim3(CartGShape<DIM, K>::BLOCK), 0, s, m, src, e);
       return hipGetLastError();
     }
   }
