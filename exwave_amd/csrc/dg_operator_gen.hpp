@@ -10,10 +10,14 @@
 // followed by the inverse mass 1/(w det J) (collocated Gauss basis) and the
 // RK epilogue.  How the data moves:
 //   A (lane = node): the cell's nodes and the node q of each of the 2 dim
-//     neighbours (all fields) are loaded at once; the
-//     node's Jacobian is computed once from the cell's corners (scalar loads)
-//     and kept in registers; each lane also computes the unit normal and the
-//     area element at one or two face points (LDS);
+//     neighbours (all fields) are loaded at once; a few lanes turn the cell's
+//     corners into the coefficients of its Jacobian columns, each a bilinear
+//     (3D) or linear (2D) polynomial in the other reference coordinates; the
+//     node's Jacobian follows from them (27 FMAs in 3D instead of a sum over
+//     the 8 corners) and stays in registers; each lane also computes the
+//     unit normal and the area element at one or two face points from the
+//     cofactors of the Jacobian there (Nanson: det J J^-T e_d = j_{d+1} x
+//     j_{d+2}, no inverse; LDS);
 //   A' (lane = node): the neighbours' (v.n, p) at node q, with n the normal
 //     at the face point of q's line, are staged in LDS -- the trace of v.n is
 //     then a line sum of staged values, as on Cartesian cells; the gradient
@@ -42,9 +46,85 @@ struct GenShape {
   static constexpr int FP_ITER = (NFP + BLOCK - 1) / BLOCK;  // face points per lane
   // cells that fill whole waves and whose LDS fits gfx950's 160 KB
   static constexpr int LDS_DOUBLES = (NF + 2 * NFACE + DIM) * ND + NFACE * NT * (DIM + 1) +
-                                     6 * N + 2 * N * N + (1 << DIM) * DIM + 3 * NFACE;
+                                     6 * N + 2 * N * N + DIM * DIM * (1 << (DIM - 1)) +
+                                     3 * NFACE;
   static constexpr bool ENABLED = ND >= 48 && LDS_DOUBLES * 8 <= 160 * 1024;
 };
+
+// Column e of the Jacobian of a multilinear cell as a polynomial in the other
+// reference coordinates: with D_j = V(corner j, xi_e = 1) - V(corner j,
+// xi_e = 0) over the 2^(dim-1) corners j of the other coordinates (bit 0 of j:
+// the lower other coordinate a, bit 1: the upper one b),
+//   dx/dxi_e = D_0 + xi_a (D_1 - D_0) [+ xi_b (D_2 - D_0) + xi_a xi_b (D_3 - D_2 - D_1 + D_0)]
+// jc[e][m][i]: coefficient m of component i.  Lane (e, i) of the first DIM^2.
+template <int DIM, typename Number>
+__device__ __forceinline__ void jacobian_coefficients(const double *V, int e, int i,
+                                                      Number (*jc)[1 << (DIM - 1)][DIM]) {
+  constexpr int NJ = 1 << (DIM - 1);
+  const int a = e == 0 ? 1 : 0, b = e == 2 ? 1 : 2;  // the other coordinates (3D: a < b)
+  double D[NJ];
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) {
+    const int v0 = ((j & 1) << a) | (DIM == 3 ? ((j >> 1) << b) : 0);
+    D[j] = V[(v0 | (1 << e)) * DIM + i] - V[v0 * DIM + i];
+  }
+  jc[e][0][i] = Number(D[0]);
+  jc[e][1][i] = Number(D[1] - D[0]);
+  if constexpr (DIM == 3) {
+    jc[e][2][i] = Number(D[2] - D[0]);
+    jc[e][3][i] = Number((D[3] - D[2]) - (D[1] - D[0]));
+  }
+}
+
+// J[i*DIM+e] = dx_i/dxi_e at xi from the column coefficients
+template <int DIM, typename Number>
+__device__ __forceinline__ void jacobian_eval(const Number (*jc)[1 << (DIM - 1)][DIM],
+                                              const Number *xi, Number *J) {
+#pragma unroll
+  for (int e = 0; e < DIM; ++e) {
+    const int a = e == 0 ? 1 : 0, b = e == 2 ? 1 : 2;
+    const Number xa = xi[a];
+    if constexpr (DIM == 3) {
+      const Number xb = xi[b], xab = xa * xb;
+#pragma unroll
+      for (int i = 0; i < DIM; ++i)
+        J[i * DIM + e] = jc[e][0][i] + xa * jc[e][1][i] + xb * jc[e][2][i] + xab * jc[e][3][i];
+    } else {
+#pragma unroll
+      for (int i = 0; i < DIM; ++i) J[i * DIM + e] = jc[e][0][i] + xa * jc[e][1][i];
+    }
+  }
+}
+
+// Jinv[e*DIM+i] = dxi_e/dx_i of J; returns det J (as multilinear_jacobian)
+template <int DIM, typename T>
+__device__ __forceinline__ T jacobian_invert(const T *J, T *Jinv) {
+  T det;
+  if constexpr (DIM == 2) {
+    det = J[0] * J[3] - J[1] * J[2];
+    const T id = T(1) / det;
+    Jinv[0] = J[3] * id;
+    Jinv[1] = -J[1] * id;
+    Jinv[2] = -J[2] * id;
+    Jinv[3] = J[0] * id;
+  } else {
+    const T a = J[0], b = J[1], c = J[2], d = J[3], e = J[4], f = J[5], g = J[6], h = J[7],
+            k = J[8];
+    const T A = e * k - f * h, B = f * g - d * k, C = d * h - e * g;
+    det = a * A + b * B + c * C;
+    const T id = T(1) / det;
+    Jinv[0] = A * id;
+    Jinv[3] = B * id;
+    Jinv[6] = C * id;
+    Jinv[1] = (c * h - b * k) * id;
+    Jinv[4] = (a * k - c * g) * id;
+    Jinv[7] = (b * g - a * h) * id;
+    Jinv[2] = (b * f - c * e) * id;
+    Jinv[5] = (c * d - a * f) * id;
+    Jinv[8] = (a * e - b * d) * id;
+  }
+  return det;
+}
 
 template <int DIM, int K, typename Number>
 __global__ void __launch_bounds__((GenShape<DIM, K>::BLOCK))
@@ -64,11 +144,11 @@ __global__ void __launch_bounds__((GenShape<DIM, K>::BLOCK))
   // flux contributions (F writes them over its own entries)
   __shared__ Number s_fp[NFACE][NT][DIM + 1];
   __shared__ Number s_tab[TO::SIZE];
-  // the cell's corners and the neighbours' materials (wave-uniform values,
-  // kept in LDS rather than in registers of every lane)
-  __shared__ double s_V[NV * DIM];
+  // the neighbours' materials and the Jacobian coefficients (wave-uniform
+  // values, kept in LDS rather than in registers of every lane)
   __shared__ Number s_mat[NFACE][2];
   __shared__ int s_nbr[NFACE];
+  __shared__ Number s_jc[DIM][1 << (DIM - 1)][DIM];
 
   const int tid = threadIdx.x;
   const int q = tid < ND ? tid : ND - 1;  // lanes past the last node mirror it
@@ -94,7 +174,9 @@ __global__ void __launch_bounds__((GenShape<DIM, K>::BLOCK))
     for (int k = 0; k < NF; ++k) nu[f][k] = un[k * ND + q];
   }
   const double rho_d = mesh.rho[cell], c_d = mesh.speed[cell];
-  if (tid < NV * DIM) s_V[tid] = mesh.vertices[(size_t)cell * NV * DIM + tid];
+  if (tid < DIM * DIM)
+    jacobian_coefficients<DIM, Number>(mesh.vertices + (size_t)cell * NV * DIM, tid / DIM,
+                                       tid % DIM, s_jc);
   if (tid < NFACE) {
     int nb = nbr[0];
 #pragma unroll
@@ -105,7 +187,7 @@ __global__ void __launch_bounds__((GenShape<DIM, K>::BLOCK))
     s_mat[tid][1] = Number(nb >= 0 ? mesh.speed[nb] : c_d);
   }
   for (int i = tid; i < TO::SIZE; i += BLOCK) s_tab[i] = Number(tab[i]);
-  __syncthreads();  // the table and the corners
+  __syncthreads();  // the table and the Jacobian coefficients
   const Number rho = Number(rho_d), cc = Number(c_d);
   const Number rho_inv = Number(1) / rho, rc2 = rho * cc * cc;
   const Number *X = s_tab + TO::X, *W = s_tab + TO::W;
@@ -120,7 +202,8 @@ __global__ void __launch_bounds__((GenShape<DIM, K>::BLOCK))
       xi[d] = X[qi[d]];
       wq *= W[qi[d]];
     }
-    jxw = wq * multilinear_jacobian<DIM>(s_V, xi, J, Ji);
+    jacobian_eval<DIM>(s_jc, xi, J);
+    jxw = wq * jacobian_invert<DIM>(J, Ji);
   }
   if (tid < ND) {
 #pragma unroll
@@ -134,28 +217,60 @@ __global__ void __launch_bounds__((GenShape<DIM, K>::BLOCK))
       s_g[e][q] = rc2 * jxw * s;
     }
   }
-  // face geometry: unit outward normal J^-T e_d / |.| and area element
-  // det J |J^-T e_d| (the tangential weights are applied at the lifting)
+  // face geometry: n dA = sg det J J^-T e_d, the cofactor row d of J: in 3D
+  // the cross product j_{d+1} x j_{d+2} of the Jacobian columns, in 2D the
+  // rotated column j_{1-d}; the unit normal n = sg sign(det J) c / |c| and
+  // the area element |c| (the tangential weights are applied at the
+  // lifting).  All cofactor rows are formed and row d is selected: the face
+  // direction varies across the lanes, and selecting the operands instead
+  // would cost more selects than the two extra rows cost products.
 #pragma unroll 1
   for (int it = 0; it < S::FP_ITER; ++it) {
     const int fp = tid + it * BLOCK;
     if (fp < NFP) {
       const int f = fp / NT, t = fp - f * NT, d = f >> 1, s = f & 1;
-      const int td[2] = {t % N, DIM == 3 ? t / N : 0};
-      Number xi[DIM], J[DIM * DIM], Jf[DIM * DIM];
-      int j = 0;
+      const Number xa = X[t % N], xb = DIM == 3 ? X[t / N] : Number(0), xs = Number(s);
+      Number xi[DIM], J[DIM * DIM];
+      if constexpr (DIM == 3) {
+        xi[0] = d == 0 ? xs : xa;
+        xi[1] = d == 0 ? xa : (d == 1 ? xs : xb);
+        xi[2] = d == 2 ? xs : xb;
+      } else {
+        xi[0] = d == 0 ? xs : xa;
+        xi[1] = d == 0 ? xa : xs;
+      }
+      jacobian_eval<DIM>(s_jc, xi, J);
+      Number c[DIM], det;
+      if constexpr (DIM == 3) {
+        Number C[3][3];
 #pragma unroll
-      for (int e = 0; e < DIM; ++e) xi[e] = e == d ? Number(s) : X[td[j++]];
-      const Number det = multilinear_jacobian<DIM>(s_V, xi, J, Jf);
+        for (int r = 0; r < 3; ++r) {
+          const int r1 = (r + 1) % 3, r2 = (r + 2) % 3;
+#pragma unroll
+          for (int i = 0; i < 3; ++i) {
+            const int i1 = (i + 1) % 3, i2 = (i + 2) % 3;
+            C[r][i] = J[i1 * 3 + r1] * J[i2 * 3 + r2] - J[i2 * 3 + r1] * J[i1 * 3 + r2];
+          }
+        }
+        det = J[0] * C[0][0] + J[3] * C[0][1] + J[6] * C[0][2];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) c[i] = d == 0 ? C[0][i] : (d == 1 ? C[1][i] : C[2][i]);
+      } else {
+        // cofactor row d of the 2x2 Jacobian: d = 0: (J11, -J01); d = 1: (-J10, J00)
+        c[0] = d == 0 ? J[3] : -J[2];
+        c[1] = d == 0 ? -J[1] : J[0];
+        det = J[0] * J[3] - J[1] * J[2];
+      }
       Number na = Number(0);
 #pragma unroll
-      for (int i = 0; i < DIM; ++i) na += Jf[d * DIM + i] * Jf[d * DIM + i];
+      for (int i = 0; i < DIM; ++i) na += c[i] * c[i];
       na = sqrt(na);
-      const Number sg = s ? Number(1) : Number(-1);
+      const Number sg = (s != 0) == (det > Number(0)) ? Number(1) : Number(-1);
+      const Number inv = recip(na);
 #pragma unroll
-      for (int i = 0; i < DIM; ++i) s_fp[f][t][i] = sg * Jf[d * DIM + i] / na;
+      for (int i = 0; i < DIM; ++i) s_fp[f][t][i] = sg * c[i] * inv;
       // a hanging face contributes nothing here (nc_face_kernel adds it)
-      s_fp[f][t][DIM] = s_nbr[f] == -NC_BID ? Number(0) : fabs(det) * na;
+      s_fp[f][t][DIM] = s_nbr[f] == -NC_BID ? Number(0) : na;
     }
   }
   __syncthreads();
