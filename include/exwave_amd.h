@@ -118,7 +118,10 @@ int exw_memcpy_d2h(exw_ctx ctx, void *dst, const void *src, size_t bytes);
  * are 16-byte aligned (3D) and below 4 GiB and there is no c operand;
  * otherwise the one-cell-per-group Cartesian kernel, with bit-identical
  * results.
- * EXW_CART_PIPE=0 in the environment selects the latter throughout. */
+ * EXW_CART_PIPE=0 in the environment selects the latter throughout.  On box
+ * meshes of 3D degree 6 and up, sweeps sum the neighbours' face traces from
+ * memory instead of staging them (bit-identical; EXW_CART_GATHER=0 selects
+ * the staging kernel). */
 int exw_op_create(exw_ctx ctx, int dim, int degree, int n_cells, const double *vertices,
                   const int *neighbor, const double *rho, const double *speed, exw_op *out);
 /* The checks exw_op_create / exw_op_create_nc run on the connectivity, on
