@@ -82,6 +82,11 @@ __device__ __forceinline__ void jacobian_eval(const Number (*jc)[1 << (DIM - 1)]
                                               const Number *xi, Number *J) {
 #pragma unroll
   for (int e = 0; e < DIM; ++e) {
+    // one column's coefficients at a time: an empty asm that clobbers memory
+    // keeps the compiler from loading all 2 DIM^2 NJC registers of them at
+    // once and holding them (3D k=4: 166 instead of 190 VGPRs, 3 waves per
+    // SIMD instead of 2)
+    asm volatile("" ::: "memory");
     const int a = e == 0 ? 1 : 0, b = e == 2 ? 1 : 2;
     const Number xa = xi[a];
     if constexpr (DIM == 3) {
