@@ -316,8 +316,10 @@ def ck_flops_per_cell(dim, k):
 def ader_measure(cases=None):
     """The ADER step on the membrane workloads (Cartesian, rho = c = 1,
     Dirichlet walls): the Cauchy-Kowalevski kernel alone (exw_ader_ck: reads the
-    state, writes the time-integrated field, 16 B/DoF algorithmic) and the whole
-    step (CK + the sweep with the update epilogue), FP64 and FP32 operator."""
+    state, writes the time-integrated field, 16 B/DoF algorithmic), the whole
+    step (CK + the sweep with the update epilogue) one call at a time, and 20
+    steps in one exw_ader_steps call (the update sweep fused with the next
+    step's CK where that kernel exists), FP64 and FP32 operator."""
     import torch
 
     import exwave_amd as X
@@ -358,6 +360,23 @@ def ader_measure(cases=None):
                         n_cells * ck_flops_per_cell(dim, k) / (ms / 1e3) / 1e12, 2)
                 else:
                     rec[f"step_gdofs_{pname}"] = round(op.n_dofs / (ms / 1e3) / 1e9, 2)
+            # R steps in one call (exw_ader_steps: on box meshes of 3D k = 3..5 each
+            # update sweep runs fused with the next step's CK), per step
+            v = u.clone()
+            w2 = op.new_vector(2)
+            op.ader_steps(1, dt, v, w2, prec)
+            torch.cuda.synchronize()
+            R = 20
+            ev0 = torch.cuda.Event(enable_timing=True)
+            ev1 = torch.cuda.Event(enable_timing=True)
+            ev0.record(stream)
+            op.ader_steps(R, dt, v, w2, prec)
+            ev1.record(stream)
+            torch.cuda.synchronize()
+            ms = ev0.elapsed_time(ev1) / R
+            rec[f"steps_ms_{pname}"] = round(ms, 4)
+            rec[f"steps_gdofs_{pname}"] = round(op.n_dofs / (ms / 1e3) / 1e9, 2)
+            del v, w2
         out.append(rec)
         print(json.dumps(rec), file=sys.stderr, flush=True)
         del u, t, w, op

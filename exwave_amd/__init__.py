@@ -137,6 +137,10 @@ class WaveOperator:
         check(load().exw_time_step(self.h, integrator, dt, _ptr(vec_n), _ptr(vec_np), _ptr(work),
                                    int(ader_recon), precision))
 
+    def ader_steps(self, n_steps, dt, vec, work, precision=FP64):
+        """exw_ader_steps: n_steps global ADER steps on vec in place (work: 2 vectors)."""
+        check(load().exw_ader_steps(self.h, int(n_steps), dt, _ptr(vec), _ptr(work), precision))
+
     @staticmethod
     def work_vectors(integrator):
         return int(load().exw_time_step_work_vectors(integrator))

@@ -124,6 +124,7 @@ SIGNATURES = [
     ("exw_rk_update", _I, [_VP, C.c_int64, _D, _D, _I, _VP, _VP, _VP]),
     ("exw_ader_ck", _I, [_VP, _VP, _VP, _D, _D, _D, _VP, _I]),
     ("exw_ader_step", _I, [_VP, _VP, _VP, _VP, _D, _I, _I]),
+    ("exw_ader_steps", _I, [_VP, _I, _D, _VP, _VP, _I]),
     ("exw_time_step", _I, [_VP, _I, _D, _VP, _VP, _VP, _I, _I]),
     ("exw_time_step_work_vectors", _I, [_I]),
     ("exw_project", _I, [_VP, C.POINTER(Modes), _D, _VP]),

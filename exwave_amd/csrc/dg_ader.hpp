@@ -274,68 +274,27 @@ __device__ __forceinline__ bool pass_reg(const Number *M, const Number (*in)[LDI
 
 }  // namespace ck
 
+// The Cauchy-Kowalevski work of the group's cells once the lane's node value
+// u of its cell is known (in registers): the Taylor levels in LDS, the time
+// integral over [t1, t2] about te written to tempsrc.  Shared by
+// ader_ck_kernel and the fused ADER sweep, so both give the same bits.
 template <int DIM, int K, typename Number, bool CART>
-__global__ void __launch_bounds__((CkGroup<DIM, K>::BLOCK))
-    __attribute__((amdgpu_waves_per_eu(CkGroup<DIM, K>::MIN_WAVES)))
-    ader_ck_kernel(const DevMesh mesh, const double *__restrict__ src,
-                   const double *__restrict__ recon, const double *__restrict__ cktab,
-                   const CkTab ckt, double t2, double t1, double te_all, const CellSel sel,
-                   double *__restrict__ tempsrc) {
+__device__ __forceinline__ void ck_cell(
+    const DevMesh &mesh, const double *__restrict__ recon, const Number *s_ck, const CkTab &ckt,
+    double t2, double t1, double te, Number (*s_a)[Shape<DIM, K>::NF][Shape<DIM, K>::ND],
+    Number (*s_b)[Shape<DIM, K>::NF][Shape<DIM, K>::ND],
+    Number (*s_c)[Shape<DIM, K>::NF]
+                 [CkShape<K>::low_off(DIM, CkShape<K>::N_LEVELS) > 0
+                      ? CkShape<K>::low_off(DIM, CkShape<K>::N_LEVELS)
+                      : 1],
+    const double *Vc, int lc, int lcc, int q, bool lane_ok, bool active, size_t cbase,
+    Number rho_inv, Number rc2, const Number (&ih)[DIM], const Number (&u)[Shape<DIM, K>::NF],
+    double *__restrict__ tempsrc) {
   using S = Shape<DIM, K>;
   using CS = CkShape<K>;
-  constexpr int N = S::N, ND = S::ND, NF = S::NF, CPB = CkGroup<DIM, K>::CPB;
+  constexpr int N = S::N, ND = S::ND, NF = S::NF;
   constexpr int NL = CS::N_LEVELS, NSTEPS = CS::N_STEPS;
-  // the lower levels' contributions, one after the other in s_c
   constexpr int LOW = CS::low_off(DIM, NL) > 0 ? CS::low_off(DIM, NL) : 1;
-
-  __shared__ Number s_a[CPB][NF][ND];
-  __shared__ Number s_b[CPB][NF][ND];
-  __shared__ Number s_c[CPB][NF][LOW];
-  __shared__ double s_V[CART ? 1 : CPB][CART ? 1 : (1 << DIM) * DIM];
-  __shared__ Number s_ck[CS::TAB_SIZE];
-
-  const int tid = threadIdx.x;
-  for (int i = tid; i < CS::TAB_SIZE && i < ckt.size; i += blockDim.x) s_ck[i] = Number(cktab[i]);
-  const int lc = tid / ND;
-  const int q = tid - lc * ND;
-  const bool lane_ok = lc < CPB;
-  const int lcc = lane_ok ? lc : 0;  // LDS slot used by spare lanes (they store nothing)
-  // entry -> cell (identity, or through the selection list)
-  auto cell_of = [&](int e) { return sel.list ? (e < sel.n ? sel.list[e] : -1) : e; };
-  const int cell = cell_of(blockIdx.x * CPB + lc);
-  const bool active = lane_ok && cell >= 0 && cell < mesh.n_cells;
-  const double te = active && sel.cluster ? sel.te[sel.cluster[cell]] : te_all;
-
-  if constexpr (!CART) {
-    constexpr int NVD = (1 << DIM) * DIM;
-    for (int i = tid; i < CPB * NVD; i += blockDim.x) {
-      const int c2 = cell_of(blockIdx.x * CPB + i / NVD);
-      s_V[i / NVD][i % NVD] =
-          c2 >= 0 && c2 < mesh.n_cells ? mesh.vertices[(size_t)c2 * NVD + i % NVD] : 0.0;
-    }
-  }
-  Number rho = Number(1), cc = Number(1), ih[DIM];
-#pragma unroll
-  for (int d = 0; d < DIM; ++d) ih[d] = Number(1);
-  if (active) {
-    rho = Number(mesh.rho[cell]);
-    cc = Number(mesh.speed[cell]);
-    if constexpr (CART) {
-#pragma unroll
-      for (int d = 0; d < DIM; ++d) ih[d] = Number(1) / Number(mesh.h[(size_t)cell * DIM + d]);
-    }
-  }
-  const Number rho_inv = Number(1) / rho, rc2 = rho * cc * cc;
-  const double *Vc = CART ? nullptr : s_V[CART ? 0 : lcc];
-
-  const size_t cbase = (size_t)cell * NF * ND;
-  Number u[NF];
-#pragma unroll
-  for (int f = 0; f < NF; ++f) u[f] = Number(0);
-  if (active) {
-#pragma unroll
-    for (int f = 0; f < NF; ++f) u[f] = Number(src[cbase + f * ND + q]);
-  }
   if (lane_ok) {
 #pragma unroll
     for (int f = 0; f < NF; ++f) s_a[lc][f][q] = u[f];
@@ -493,6 +452,72 @@ __global__ void __launch_bounds__((CkGroup<DIM, K>::BLOCK))
 #pragma unroll
     for (int f = 0; f < NF; ++f) tempsrc[cbase + f * ND + q] = double(contrib[f]);
   }
+}
+
+template <int DIM, int K, typename Number, bool CART>
+__global__ void __launch_bounds__((CkGroup<DIM, K>::BLOCK))
+    __attribute__((amdgpu_waves_per_eu(CkGroup<DIM, K>::MIN_WAVES)))
+    ader_ck_kernel(const DevMesh mesh, const double *__restrict__ src,
+                   const double *__restrict__ recon, const double *__restrict__ cktab,
+                   const CkTab ckt, double t2, double t1, double te_all, const CellSel sel,
+                   double *__restrict__ tempsrc) {
+  using S = Shape<DIM, K>;
+  using CS = CkShape<K>;
+  constexpr int N = S::N, ND = S::ND, NF = S::NF, CPB = CkGroup<DIM, K>::CPB;
+  constexpr int NL = CS::N_LEVELS, NSTEPS = CS::N_STEPS;
+  // the lower levels' contributions, one after the other in s_c
+  constexpr int LOW = CS::low_off(DIM, NL) > 0 ? CS::low_off(DIM, NL) : 1;
+
+  __shared__ Number s_a[CPB][NF][ND];
+  __shared__ Number s_b[CPB][NF][ND];
+  __shared__ Number s_c[CPB][NF][LOW];
+  __shared__ double s_V[CART ? 1 : CPB][CART ? 1 : (1 << DIM) * DIM];
+  __shared__ Number s_ck[CS::TAB_SIZE];
+
+  const int tid = threadIdx.x;
+  for (int i = tid; i < CS::TAB_SIZE && i < ckt.size; i += blockDim.x) s_ck[i] = Number(cktab[i]);
+  const int lc = tid / ND;
+  const int q = tid - lc * ND;
+  const bool lane_ok = lc < CPB;
+  const int lcc = lane_ok ? lc : 0;  // LDS slot used by spare lanes (they store nothing)
+  // entry -> cell (identity, or through the selection list)
+  auto cell_of = [&](int e) { return sel.list ? (e < sel.n ? sel.list[e] : -1) : e; };
+  const int cell = cell_of(blockIdx.x * CPB + lc);
+  const bool active = lane_ok && cell >= 0 && cell < mesh.n_cells;
+  const double te = active && sel.cluster ? sel.te[sel.cluster[cell]] : te_all;
+
+  if constexpr (!CART) {
+    constexpr int NVD = (1 << DIM) * DIM;
+    for (int i = tid; i < CPB * NVD; i += blockDim.x) {
+      const int c2 = cell_of(blockIdx.x * CPB + i / NVD);
+      s_V[i / NVD][i % NVD] =
+          c2 >= 0 && c2 < mesh.n_cells ? mesh.vertices[(size_t)c2 * NVD + i % NVD] : 0.0;
+    }
+  }
+  Number rho = Number(1), cc = Number(1), ih[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) ih[d] = Number(1);
+  if (active) {
+    rho = Number(mesh.rho[cell]);
+    cc = Number(mesh.speed[cell]);
+    if constexpr (CART) {
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) ih[d] = Number(1) / Number(mesh.h[(size_t)cell * DIM + d]);
+    }
+  }
+  const Number rho_inv = Number(1) / rho, rc2 = rho * cc * cc;
+  const double *Vc = CART ? nullptr : s_V[CART ? 0 : lcc];
+
+  const size_t cbase = (size_t)cell * NF * ND;
+  Number u[NF];
+#pragma unroll
+  for (int f = 0; f < NF; ++f) u[f] = Number(0);
+  if (active) {
+#pragma unroll
+    for (int f = 0; f < NF; ++f) u[f] = Number(src[cbase + f * ND + q]);
+  }
+  ck_cell<DIM, K, Number, CART>(mesh, recon, s_ck, ckt, t2, t1, te, s_a, s_b, s_c, Vc, lc, lcc,
+                                q, lane_ok, active, cbase, rho_inv, rc2, ih, u, tempsrc);
 }
 
 }  // namespace exw

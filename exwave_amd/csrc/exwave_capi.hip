@@ -673,6 +673,39 @@ int exw_ader_step(exw_op op, const double *vec_n, double *vec_np, double *work, 
   return exw_op_apply_fused(op, tempsrc, &e, precision);
 }
 
+int exw_ader_steps(exw_op op, int n_steps, double dt, double *vec, double *work,
+                   int precision) {
+  if (!op || !vec || !work) return fail(EXW_EINVAL, "exw_ader_steps: null argument");
+  if (n_steps < 0) return fail(EXW_EINVAL, "exw_ader_steps: negative step count");
+  if (int rc = check_prec(precision)) return rc;
+  if (n_steps == 0) return EXW_OK;
+  const int64_t N = op->n_dofs();
+  double *w[2] = {work, work + N};
+  // the update sweep vec += M^-1 A(w) in place (its epilogue reads b before
+  // writing o1 at the same entry; the neighbours' w is read, never vec)
+  auto sweep = [&](const double *wi) {
+    exw_epilogue e{vec, vec, 1.0, nullptr, nullptr, 0.0, 0, nullptr};
+    return exw_op_apply_fused(op, wi, &e, precision);
+  };
+  int rc;
+  AderFusedLaunch fused = op->cart && op->n_nc == 0
+                              ? kernels(op->dim).ader_fused[op->degree][precision]
+                              : nullptr;
+  if (!fused) {
+    // one exw_ader_step per step, in place
+    for (int s = 0; s < n_steps; ++s)
+      if ((rc = exw_ader_ck(op, vec, nullptr, dt, 0.0, 0.0, w[0], precision)) ||
+          (rc = sweep(w[0])))
+        return rc;
+    return EXW_OK;
+  }
+  if ((rc = exw_ader_ck(op, vec, nullptr, dt, 0.0, 0.0, w[0], precision))) return rc;
+  for (int s = 1; s < n_steps; ++s)
+    EXW_HIP(fused(op->view(), w[(s - 1) & 1], vec, op->d_cktab, op->ck, dt, w[s & 1],
+                  op->ctx->stream));
+  return sweep(w[(n_steps - 1) & 1]);
+}
+
 int exw_time_step_work_vectors(int integrator) {
   switch (integrator) {
     case EXW_EXPL_EULER: return 0;

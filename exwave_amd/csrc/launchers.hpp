@@ -6,6 +6,7 @@
 #include <type_traits>
 
 #include "dg_ader.hpp"
+#include "dg_ader_fused.hpp"
 #include "dg_lts.hpp"
 #include "dg_misc.hpp"
 #include "dg_operator.hpp"
@@ -58,6 +59,9 @@ using PostErrLaunch = hipError_t (*)(const DevMesh &, const double *, const DevM
 
 // (mesh, first cell, cell count, per-cell flux coefficients out, stream)
 using CoefLaunch = hipError_t (*)(const DevMesh &, int, int, double *, hipStream_t);
+// (mesh, w_in, u (in place), CK table, CK offsets, dt, w_out, stream)
+using AderFusedLaunch = hipError_t (*)(const DevMesh &, const double *, double *, const double *,
+                                       const CkTab &, double, double *, hipStream_t);
 
 struct DimKernels {
   CoefLaunch coef;
@@ -71,6 +75,7 @@ struct DimKernels {
   NcLaunch nc[MAX_DEGREE + 1];
   EstLaunch estimate[MAX_DEGREE + 1];
   PostErrLaunch post_error[MAX_DEGREE + 1];
+  AderFusedLaunch ader_fused[MAX_DEGREE + 1][2];  // [k][fp32], Cartesian; NULL: none
   int block[MAX_DEGREE + 1];
   int cells_per_block[MAX_DEGREE + 1];
   // the FP64 Cartesian sweeps of degree k may run dg_operator_pipe_kernel,
@@ -188,6 +193,17 @@ hipError_t launch_op(const DevMesh &m, const double *src, const double *tab,
     hipLaunchKernelGGL((dg_operator_kernel<DIM, K, Number, CART>), dim3(grid), dim3(S::BLOCK), 0,
                        s, m, src, tab, e);
   }
+  return hipGetLastError();
+}
+
+template <int DIM, int K, typename Number>
+hipError_t launch_ader_fused(const DevMesh &m, const double *w_in, double *u, const double *cktab,
+                             const CkTab &ck, double dt, double *w_out, hipStream_t s) {
+  using F = AderFusedShape<DIM, K>;
+  const int grid = (m.n_cells + F::CPB - 1) / F::CPB;
+  if (grid == 0) return hipSuccess;
+  hipLaunchKernelGGL((ader_sweep_ck_kernel<DIM, K, Number>), dim3(grid), dim3(F::BLOCK), 0, s, m,
+                     w_in, u, cktab, ck, dt, w_out);
   return hipGetLastError();
 }
 
@@ -336,6 +352,10 @@ void fill_dim_kernels(DimKernels &t) {
     t.nc[K] = launch_nc<DIM, K>;
     t.estimate[K] = launch_estimate<DIM, K>;
     if constexpr (post_fits<DIM, K>()) t.post_error[K] = launch_post_error<DIM, K>;
+    if constexpr (AderFusedShape<DIM, K>::ENABLED) {
+      t.ader_fused[K][0] = launch_ader_fused<DIM, K, double>;
+      t.ader_fused[K][1] = launch_ader_fused<DIM, K, float>;
+    }
     t.block[K] = Shape<DIM, K>::BLOCK;
     t.cells_per_block[K] = Shape<DIM, K>::CPB;
     t.pipe[K] = PipeShape<DIM, K>::ENABLED && (DIM == 2 || K <= 5);

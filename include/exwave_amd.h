@@ -174,6 +174,15 @@ int exw_ader_ck(exw_op op, const double *src, const double *recon, double t2, do
  * vec_np = vec_n - apply_ader(vec_n).  work: 2*n_dofs doubles (tempsrc, recon). */
 int exw_ader_step(exw_op op, const double *vec_n, double *vec_np, double *work, double dt,
                   int recon, int precision);
+/* n_steps global ADER steps of size dt on vec in place (no reconstruction, no
+ * source): what the reference's time loop does between two output ticks
+ * (ExplicitWaveEquation::run, source/explicit_wave.cc:779-799, calling
+ * perform_time_step once per step).  On box meshes without hanging faces (3D
+ * degree 3-5) each step's update sweep runs fused with the next step's CK
+ * time integral in one kernel; the result is bit-identical to n_steps
+ * exw_ader_step calls.  work: 2*n_dofs doubles (the CK fields of two
+ * consecutive steps). */
+int exw_ader_steps(exw_op op, int n_steps, double dt, double *vec, double *work, int precision);
 /* ExplicitIntegrator<..>::perform_time_step (include/time_integrators.h:25-39)
  * for the integrator ids above.  Same contract as the reference: vec_n is
  * clobbered, the result is left in vec_np.  work: 3*n_dofs doubles (the
