@@ -274,6 +274,27 @@ __device__ __forceinline__ bool pass_reg(const Number *M, const Number (*in)[LDI
 
 }  // namespace ck
 
+// The cell's factors of the CK derivatives: 1/rho, rho c^2 and (Cartesian)
+// 1/h_d; 1 for lanes without a cell.  Reciprocals by the hardware estimate
+// and two Newton steps (recip), not IEEE divisions: every lane computes them.
+template <int DIM, bool CART, typename Number>
+__device__ __forceinline__ void ck_factors(const DevMesh &mesh, int cell, bool active,
+                                           Number &rho_inv, Number &rc2, Number (&ih)[DIM]) {
+  Number rho = Number(1), cc = Number(1);
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) ih[d] = Number(1);
+  if (active) {
+    rho = Number(mesh.rho[cell]);
+    cc = Number(mesh.speed[cell]);
+    if constexpr (CART) {
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) ih[d] = recip(Number(mesh.h[(size_t)cell * DIM + d]));
+    }
+  }
+  rho_inv = recip(rho);
+  rc2 = rho * cc * cc;
+}
+
 // The Cauchy-Kowalevski work of the group's cells once the lane's node value
 // u of its cell is known (in registers): the Taylor levels in LDS, the time
 // integral over [t1, t2] about te written to tempsrc.  Shared by
@@ -494,18 +515,8 @@ __global__ void __launch_bounds__((CkGroup<DIM, K>::BLOCK))
           c2 >= 0 && c2 < mesh.n_cells ? mesh.vertices[(size_t)c2 * NVD + i % NVD] : 0.0;
     }
   }
-  Number rho = Number(1), cc = Number(1), ih[DIM];
-#pragma unroll
-  for (int d = 0; d < DIM; ++d) ih[d] = Number(1);
-  if (active) {
-    rho = Number(mesh.rho[cell]);
-    cc = Number(mesh.speed[cell]);
-    if constexpr (CART) {
-#pragma unroll
-      for (int d = 0; d < DIM; ++d) ih[d] = Number(1) / Number(mesh.h[(size_t)cell * DIM + d]);
-    }
-  }
-  const Number rho_inv = Number(1) / rho, rc2 = rho * cc * cc;
+  Number rho_inv, rc2, ih[DIM];
+  ck_factors<DIM, CART>(mesh, cell, active, rho_inv, rc2, ih);
   const double *Vc = CART ? nullptr : s_V[CART ? 0 : lcc];
 
   const size_t cbase = (size_t)cell * NF * ND;

@@ -96,16 +96,8 @@ __global__ void __launch_bounds__((AderFusedShape<DIM, K>::BLOCK))
   const int lcc = lane_ok ? lck : 0;
   // the same cells on the same lanes: a node lane of the CK is a swept lane
   const bool active = lane_ok && swept;
-  Number rho = Number(1), cc = Number(1), ih[DIM];
-#pragma unroll
-  for (int d = 0; d < DIM; ++d) ih[d] = Number(1);
-  if (active) {
-    rho = Number(mesh.rho[cell]);
-    cc = Number(mesh.speed[cell]);
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) ih[d] = Number(1) / Number(mesh.h[(size_t)cell * DIM + d]);
-  }
-  const Number rho_inv = Number(1) / rho, rc2 = rho * cc * cc;
+  Number rho_inv, rc2, ih[DIM];
+  ck_factors<DIM, true>(mesh, cell, active, rho_inv, rc2, ih);
   Number uc[NF];
 #pragma unroll
   for (int f = 0; f < NF; ++f) uc[f] = active ? Number(un[f]) : Number(0);
