@@ -482,8 +482,41 @@ int exw_run(exw_ctx ctx, const exw_problem *p, char *log, int log_len, double *w
       }
     }
   }
+  // global ADER without reconstruction advances the state in place through
+  // exw_ader_steps, several steps per call up to the next step at which the
+  // loop below does something besides stepping (an adaptation, an output tick,
+  // the end), with the same step count, times and results
+  const bool multi = p->integrator == EXW_ADER && !p->ader_recon;
+  auto steps_to_event = [&]() {
+    unsigned s = step, m = 0;
+    double t = time;
+    while (!(t >= T || s > max_step)) {
+      ++s;
+      t += dt;
+      ++m;
+      if (adaptive && s % p->adaptive_refinement_interval == 0) break;
+      if ((int)(t / tick) != (int)((t - dt) / tick)) break;
+    }
+    return m;
+  };
   if (!rc && !(rc = output())) {
     while (!(time >= T || step > max_step)) {
+      if (multi) {
+        const unsigned m = steps_to_event();
+        exw_sync(ctx);
+        auto t0 = std::chrono::steady_clock::now();
+        if ((rc = exw_ader_steps(st.op, (int)m, dt, st.sol, st.work, p->precision)) ||
+            (rc = exw_sync(ctx)))
+          break;
+        wall += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        for (unsigned j = 0; j < m; ++j) {
+          ++step;
+          time += dt;
+        }
+        if (adaptive && step % p->adaptive_refinement_interval == 0 && (rc = adapt())) break;
+        if ((int)(time / tick) != (int)((time - dt) / tick) && (rc = output())) break;
+        continue;
+      }
       ++step;
       time += dt;
       std::swap(st.sol, st.tmp);
