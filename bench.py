@@ -106,10 +106,16 @@ def measured_traffic(kernel_key):
 def stage_kernel(k, fp64):
     """(family, name) of the kernel that runs the fused LSRK stage on Cartesian
     3D cells: the pipelined FP64 kernel for k = 3..5 (unless EXW_CART_PIPE=0),
-    else dg_operator_cart_kernel."""
+    the gathered-trace kernel for k = 6..9 (unless EXW_CART_GATHER=0), the
+    big-cell kernel from k = 10, else dg_operator_cart_kernel."""
+    prec = "double" if fp64 else "float"
     if fp64 and 3 <= k <= 5 and os.environ.get("EXW_CART_PIPE", "1")[:1] != "0":
         return "pipe", f"dg_operator_pipe_kernel<3,{k},2>"
-    return "cart", f"dg_operator_cart_kernel<3,{k},{'double' if fp64 else 'float'}>"
+    if 6 <= k <= 9 and os.environ.get("EXW_CART_GATHER", "1")[:1] != "0":
+        return "cartg", f"dg_operator_cartg_kernel<3,{k},{prec}>"
+    if k >= 10:
+        return "big", f"dg_operator_big_kernel<3,{k},{prec}>"
+    return "cart", f"dg_operator_cart_kernel<3,{k},{prec}>"
 
 
 def stream_ceiling(reps=10, ctx=None):
