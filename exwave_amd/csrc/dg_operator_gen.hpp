@@ -9,24 +9,23 @@
 //   dp/dt = weak div of rho c^2 w det(J) J^-1 v + lifting of the face terms
 // followed by the inverse mass 1/(w det J) (collocated Gauss basis) and the
 // RK epilogue.  How the data moves:
-//   A (lane = node): the cell's nodes and the node q of each of the 2 dim
-//     neighbours (all fields) are loaded at once; a few lanes turn the cell's
-//     corners into the coefficients of its Jacobian columns, each a bilinear
-//     (3D) or linear (2D) polynomial in the other reference coordinates; the
-//     node's Jacobian follows from them (27 FMAs in 3D instead of a sum over
-//     the 8 corners) and stays in registers; each lane also computes the
-//     unit normal and the area element at one or two face points from the
-//     cofactors of the Jacobian there (Nanson: det J J^-T e_d = j_{d+1} x
-//     j_{d+2}, no inverse; LDS);
-//   A' (lane = node): the neighbours' (v.n, p) at node q, with n the normal
-//     at the face point of q's line, are staged in LDS -- the trace of v.n is
-//     then a line sum of staged values, as on Cartesian cells; the gradient
-//     of p and the test-gradient values rho c^2 w det J (J^-1 v) are formed;
-//   F (lane = face point, and lane = node): the face fluxes from LDS traces,
-//     and the weak divergence at the node;
+//   A (lane = node): the cell's nodes; a few lanes turn the cell's corners
+//     into the coefficients of its Jacobian columns, each a bilinear (3D) or
+//     linear (2D) polynomial in the other reference coordinates;
+//   G: the node's Jacobian from those coefficients (27 FMAs in 3D instead of
+//     a sum over the 8 corners) and its inverse, in registers; the
+//     test-gradient values rho c^2 w det J (J^-1 v) (LDS); per face point the
+//     unit normal and the area element from the cofactors of the Jacobian
+//     there (Nanson: det J J^-T e_d = j_{d+1} x j_{d+2}, no inverse; LDS);
+//   A' (lane = face point, then lane = node): the neighbour's traces of all
+//     fields along the face point's line, summed from its nodes in memory (L2
+//     hits within the tile), then its v.n with this side's normal (LDS: 2
+//     values per face point instead of 2 dim node-aligned fields); the
+//     pressure gradient and dv/dt at the node;
+//   F (lane = face point, and lane = node): the face fluxes, and the weak
+//     divergence at the node;
 //   C (lane = node): lifting, inverse mass, epilogue with streaming stores.
-// Four barriers, no atomics; after the loads only the epilogue
-// operands are read from global memory (issued after the face phase).
+// Four barriers, no atomics.
 #pragma once
 #include <type_traits>
 
@@ -45,9 +44,8 @@ struct GenShape {
   static constexpr int BLOCK = (ND + 63) / 64 * 64;
   static constexpr int FP_ITER = (NFP + BLOCK - 1) / BLOCK;  // face points per lane
   // cells that fill whole waves and whose LDS fits gfx950's 160 KB
-  static constexpr int LDS_DOUBLES = (NF + 2 * NFACE + DIM) * ND + NFACE * NT * (DIM + 1) +
-                                     6 * N + 2 * N * N + DIM * DIM * (1 << (DIM - 1)) +
-                                     3 * NFACE;
+  static constexpr int LDS_DOUBLES = (NF + DIM) * ND + NFACE * NT * (DIM + 3) + 6 * N +
+                                     2 * N * N + DIM * DIM * (1 << (DIM - 1)) + 3 * NFACE;
   static constexpr bool ENABLED = ND >= 48 && LDS_DOUBLES * 8 <= 160 * 1024;
 };
 
@@ -141,9 +139,8 @@ __global__ void __launch_bounds__((GenShape<DIM, K>::BLOCK))
   using TO = TabOff<N>;
 
   __shared__ Number s_u[NF][ND];
-  // neighbour f's (v.n, p) at node-aligned positions (n: this cell's outward
-  // normal at the face point of the node's line)
-  __shared__ Number s_nb[NFACE][2][ND];
+  // per face point the neighbour's traces (v.n with this side's normal, p)
+  __shared__ Number s_tr[NFACE][NT][2];
   __shared__ Number s_g[DIM][ND];
   // per face point: unit outward normal and area element (A), then the face
   // flux contributions (F writes them over its own entries)
@@ -168,16 +165,9 @@ __global__ void __launch_bounds__((GenShape<DIM, K>::BLOCK))
   int nbr[NFACE];
 #pragma unroll
   for (int f = 0; f < NFACE; ++f) nbr[f] = mesh.neighbor[(size_t)cell * NFACE + f];
-  double uo[NF], nu[NFACE][NF];
+  double uo[NF];
 #pragma unroll
   for (int f = 0; f < NF; ++f) uo[f] = src[cbase + f * ND + q];
-#pragma unroll
-  for (int f = 0; f < NFACE; ++f) {
-    // boundary and hanging faces read the cell itself (unused values)
-    const double *un = src + (size_t)(nbr[f] >= 0 ? nbr[f] : cell) * NF * ND;
-#pragma unroll
-    for (int k = 0; k < NF; ++k) nu[f][k] = un[k * ND + q];
-  }
   const double rho_d = mesh.rho[cell], c_d = mesh.speed[cell];
   if (tid < DIM * DIM)
     jacobian_coefficients<DIM, Number>(mesh.vertices + (size_t)cell * NV * DIM, tid / DIM,
@@ -282,20 +272,44 @@ __global__ void __launch_bounds__((GenShape<DIM, K>::BLOCK))
 
   // ---------------- A': neighbour staging, cell term ----------------
   const Number *D = s_tab + TO::D, *L = s_tab + TO::L;
-  if (tid < ND) {
+  // lane = face point: the neighbour's traces of all fields along this face
+  // point's line, summed from its nodes in memory (32-bit offsets: vectors
+  // below 4 GiB), then v.n with this side's normal
+#pragma unroll 1
+  for (int it = 0; it < S::FP_ITER; ++it) {
+    const int fp = tid + it * BLOCK;
+    if (fp < NFP) {
+      const int f = fp / NT, t = fp - f * NT, d = f >> 1, s = f & 1;
+      int base;
+      if constexpr (DIM == 2) {
+        base = d == 0 ? t * N : t;
+      } else {
+        const int t0 = t % N, t1 = t / N;
+        base = d == 0 ? t0 * N + t1 * N * N : (d == 1 ? t0 + t1 * N * N : t0 + t1 * N);
+      }
+      const int sd = d == 0 ? 1 : (d == 1 ? N : N * N);
+      const int nb = s_nbr[f];
+      // boundary and hanging faces read the cell itself (unused values)
+      const uint32_t un = uint32_t(nb >= 0 ? nb : cell) * uint32_t(NF * ND) + uint32_t(base);
+      double gn[NF][N];
 #pragma unroll
-    for (int f = 0; f < NFACE; ++f) {
-      const int d = f >> 1;
-      int t;
-      if constexpr (DIM == 2)
-        t = qi[1 - d];
-      else
-        t = d == 0 ? qi[1] + N * qi[2] : (d == 1 ? qi[0] + N * qi[2] : qi[0] + N * qi[1]);
-      Number vn = Number(0);
+      for (int k = 0; k < NF; ++k)
 #pragma unroll
-      for (int i = 0; i < DIM; ++i) vn += s_fp[f][t][i] * Number(nu[f][i]);
-      s_nb[f][0][q] = vn;
-      s_nb[f][1][q] = Number(nu[f][DIM]);
+        for (int i = 0; i < N; ++i) gn[k][i] = src[un + uint32_t(k * ND + i * sd)];
+      const Number *Lo = L + (1 - s) * N;
+      Number an[NF];
+#pragma unroll
+      for (int k = 0; k < NF; ++k) {
+        Number a = Number(0);
+#pragma unroll
+        for (int i = 0; i < N; ++i) a += Lo[i] * Number(gn[k][i]);
+        an[k] = a;
+      }
+      Number nvm = Number(0);
+#pragma unroll
+      for (int i = 0; i < DIM; ++i) nvm += s_fp[f][t][i] * an[i];
+      s_tr[f][t][0] = nvm;
+      s_tr[f][t][1] = an[DIM];
     }
   }
   Number kv[DIM];
@@ -317,6 +331,11 @@ __global__ void __launch_bounds__((GenShape<DIM, K>::BLOCK))
       kv[i] = -rho_inv * gp;
     }
   }
+  // (empty asm that takes kv: the compiler would otherwise sink its sums
+  // into the last phase and hold the node's inverse Jacobian and gradient
+  // through the face phase, 24 registers instead of 6)
+#pragma unroll
+  for (int i = 0; i < DIM; ++i) asm volatile("" : "+v"(kv[i]));
   __syncthreads();
 
   // ---------------- F: face fluxes (lane = face point), weak divergence ----
@@ -333,24 +352,21 @@ __global__ void __launch_bounds__((GenShape<DIM, K>::BLOCK))
         base = d == 0 ? t0 * N + t1 * N * N : (d == 1 ? t0 + t1 * N * N : t0 + t1 * N);
       }
       const int sd = d == 0 ? 1 : (d == 1 ? N : N * N);
-      const Number *Ls = L + s * N, *Lo = L + (1 - s) * N;
-      Number up[NF], nvm = Number(0), pm = Number(0);
-      // one pass along the line for all six sums (the order of each sum is
-      // unchanged), rolled: an unrolled pass keeps all its LDS loads in flight
-      Number acc[NF];
+      const Number *Ls = L + s * N;
+      // the own traces: one rolled pass along the line for all the sums (an
+      // unrolled pass keeps all its LDS loads in flight); the neighbour's
+      // from the gather phase
+      Number up[NF];
 #pragma unroll
-      for (int k = 0; k < NF; ++k) acc[k] = Number(0);
+      for (int k = 0; k < NF; ++k) up[k] = Number(0);
 #pragma unroll 1
       for (int i = 0; i < N; ++i) {
-        const Number ls = Ls[i], lo = Lo[i];
+        const Number ls = Ls[i];
         const int o = base + i * sd;
 #pragma unroll
-        for (int k = 0; k < NF; ++k) acc[k] += ls * s_u[k][o];
-        nvm += lo * s_nb[f][0][o];
-        pm += lo * s_nb[f][1][o];
+        for (int k = 0; k < NF; ++k) up[k] += ls * s_u[k][o];
       }
-#pragma unroll
-      for (int k = 0; k < NF; ++k) up[k] = acc[k];
+      const Number nvm = s_tr[f][t][0], pm = s_tr[f][t][1];
       Number nrm[DIM], nvp = Number(0);
 #pragma unroll
       for (int i = 0; i < DIM; ++i) {
@@ -369,8 +385,7 @@ __global__ void __launch_bounds__((GenShape<DIM, K>::BLOCK))
     }
   }
   // the epilogue operands, in flight during the weak divergence and the last
-  // barrier (issued after the face phase, whose traces set the register peak:
-  // 166 instead of 222 VGPRs at 3D k=4, 3 waves per SIMD instead of 2)
+  // barrier (issued after the face phase)
   double bv[NF], cv[NF];
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
@@ -387,6 +402,7 @@ __global__ void __launch_bounds__((GenShape<DIM, K>::BLOCK))
     for (int m = 0; m < N; ++m) a += D[m * N + qi[e]] * s_g[e][q0 + m * se];
     kp += a;
   }
+  asm volatile("" : "+v"(kp));  // (the same for the divergence's loaded terms)
   __syncthreads();
 
   // ---------------- C: lifting, inverse mass, epilogue ----------------

@@ -72,6 +72,7 @@ APPLY_CASES = [
     # deformed cells of >= 48 nodes (one cell per group, node/face/node kernel)
     (3, 3, 3, 1, 0.1, 1, 10), (3, 3, 5, 3, 0.04, 3, None), (2, 4, 6, 1, 0.1, 1, 8),
     (2, 3, 7, 3, 0.04, 3, 9), (2, 3, 8, 1, 0.1, 2, None),
+    (3, 2, 7, 1, 0.05, 2, 19), (3, 2, 8, 1, 0.04, 1, None),
     # the highest degrees: 2D up to 12, 3D up to 9
     (2, 2, 9, 1, 0.1, 2, 12), (2, 2, 12, 0, 0.0, 1, 13), (2, 2, 12, 1, 0.1, 3, None),
     (2, 3, 10, 3, 0.04, 2, 14), (3, 2, 9, 0, 0.0, 2, 15), (3, 2, 9, 1, 0.05, 3, None),
