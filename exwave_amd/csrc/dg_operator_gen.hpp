@@ -12,20 +12,20 @@
 //   A (lane = node): the cell's nodes; a few lanes turn the cell's corners
 //     into the coefficients of its Jacobian columns, each a bilinear (3D) or
 //     linear (2D) polynomial in the other reference coordinates;
-//   G: the node's Jacobian from those coefficients (27 FMAs in 3D instead of
-//     a sum over the 8 corners) and its inverse, in registers; the
-//     test-gradient values rho c^2 w det J (J^-1 v) (LDS); per face point the
-//     unit normal and the area element from the cofactors of the Jacobian
-//     there (Nanson: det J J^-T e_d = j_{d+1} x j_{d+2}, no inverse; LDS);
-//   A' (lane = face point, then lane = node): the neighbour's traces of all
-//     fields along the face point's line, summed from its nodes in memory (L2
-//     hits within the tile), then its v.n with this side's normal (LDS: 2
-//     values per face point instead of 2 dim node-aligned fields); the
-//     pressure gradient and dv/dt at the node;
+//   G (lane = node, then lane = face point): the node's Jacobian from those
+//     coefficients (27 FMAs in 3D instead of a sum over the 8 corners) and its
+//     inverse, the test-gradient values rho c^2 w det J (J^-1 v) (LDS), the
+//     pressure gradient and dv/dt; per face point the neighbour's nodes on
+//     the face point's line are loaded (L2 hits within the tile), the unit
+//     normal and area element are formed from the cofactors of the Jacobian
+//     there while they arrive (Nanson: det J J^-T e_d = j_{d+1} x j_{d+2}, no
+//     inverse), then the neighbour's traces and its v.n with this side's
+//     normal (LDS: 2 values per face point instead of 2 dim node-aligned
+//     fields);
 //   F (lane = face point, and lane = node): the face fluxes, and the weak
 //     divergence at the node;
 //   C (lane = node): lifting, inverse mass, epilogue with streaming stores.
-// Four barriers, no atomics.
+// Three barriers, no atomics.
 #pragma once
 #include <type_traits>
 
@@ -182,7 +182,11 @@ __global__ void __launch_bounds__((GenShape<DIM, K>::BLOCK))
     s_mat[tid][1] = Number(nb >= 0 ? mesh.speed[nb] : c_d);
   }
   for (int i = tid; i < TO::SIZE; i += BLOCK) s_tab[i] = Number(tab[i]);
-  __syncthreads();  // the table and the Jacobian coefficients
+  if (tid < ND) {
+#pragma unroll
+    for (int f = 0; f < NF; ++f) s_u[f][q] = Number(uo[f]);
+  }
+  __syncthreads();  // the table, the Jacobian coefficients, the cell's fields
   const Number rho = Number(rho_d), cc = Number(c_d);
   const Number rho_inv = Number(1) / rho, rc2 = rho * cc * cc;
   const Number *X = s_tab + TO::X, *W = s_tab + TO::W;
@@ -201,8 +205,6 @@ __global__ void __launch_bounds__((GenShape<DIM, K>::BLOCK))
     jxw = wq * jacobian_invert<DIM>(J, Ji);
   }
   if (tid < ND) {
-#pragma unroll
-    for (int f = 0; f < NF; ++f) s_u[f][q] = Number(uo[f]);
     // test-gradient values rho c^2 w det J (J^-1 v)_e
 #pragma unroll
     for (int e = 0; e < DIM; ++e) {
@@ -212,7 +214,38 @@ __global__ void __launch_bounds__((GenShape<DIM, K>::BLOCK))
       s_g[e][q] = rc2 * jxw * s;
     }
   }
-  // face geometry: n dA = sg det J J^-T e_d, the cofactor row d of J: in 3D
+  // the cell term dv/dt = -rho^-1 J^-T grad_xi p at the node (s_u is complete)
+  const Number *D = s_tab + TO::D, *L = s_tab + TO::L;
+  Number kv[DIM];
+  {
+    Number gref[DIM];
+#pragma unroll
+    for (int e = 0; e < DIM; ++e) {
+      const int se = pow_n<N>(e), q0 = q - qi[e] * se;
+      Number g = Number(0);
+#pragma unroll
+      for (int m = 0; m < N; ++m) g += D[qi[e] * N + m] * s_u[DIM][q0 + m * se];
+      gref[e] = g;
+    }
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) {
+      Number gp = Number(0);
+#pragma unroll
+      for (int e = 0; e < DIM; ++e) gp += Ji[e * DIM + i] * gref[e];
+      kv[i] = -rho_inv * gp;
+    }
+  }
+  // (empty asm that takes kv: the compiler would otherwise sink its sums
+  // into the last phase and hold the node's inverse Jacobian and gradient
+  // through the face phases, 24 registers instead of 6)
+#pragma unroll
+  for (int i = 0; i < DIM; ++i) asm volatile("" : "+v"(kv[i]));
+
+  // lane = face point: the neighbour's nodes along the face point's line are
+  // loaded first (32-bit offsets: vectors below 4 GiB), the face geometry is
+  // computed while they arrive, then the neighbour's traces of all fields and
+  // its v.n with this side's normal.
+  // Face geometry: n dA = sg det J J^-T e_d, the cofactor row d of J: in 3D
   // the cross product j_{d+1} x j_{d+2} of the Jacobian columns, in 2D the
   // rotated column j_{1-d}; the unit normal n = sg sign(det J) c / |c| and
   // the area element |c| (the tangential weights are applied at the
@@ -224,6 +257,23 @@ __global__ void __launch_bounds__((GenShape<DIM, K>::BLOCK))
     const int fp = tid + it * BLOCK;
     if (fp < NFP) {
       const int f = fp / NT, t = fp - f * NT, d = f >> 1, s = f & 1;
+      int base;
+      if constexpr (DIM == 2) {
+        base = d == 0 ? t * N : t;
+      } else {
+        const int t0 = t % N, t1 = t / N;
+        base = d == 0 ? t0 * N + t1 * N * N : (d == 1 ? t0 + t1 * N * N : t0 + t1 * N);
+      }
+      const int sd = d == 0 ? 1 : (d == 1 ? N : N * N);
+      const int nb = s_nbr[f];
+      // boundary and hanging faces read the cell itself (unused values)
+      const uint32_t un = uint32_t(nb >= 0 ? nb : cell) * uint32_t(NF * ND) + uint32_t(base);
+      double gn[NF][N];
+#pragma unroll
+      for (int k = 0; k < NF; ++k)
+#pragma unroll
+        for (int i = 0; i < N; ++i) gn[k][i] = src[un + uint32_t(k * ND + i * sd)];
+
       const Number xa = X[t % N], xb = DIM == 3 ? X[t / N] : Number(0), xs = Number(s);
       Number xi[DIM], J[DIM * DIM];
       if constexpr (DIM == 3) {
@@ -262,40 +312,15 @@ __global__ void __launch_bounds__((GenShape<DIM, K>::BLOCK))
       na = sqrt(na);
       const Number sg = (s != 0) == (det > Number(0)) ? Number(1) : Number(-1);
       const Number inv = recip(na);
+      Number nrm[DIM];
 #pragma unroll
-      for (int i = 0; i < DIM; ++i) s_fp[f][t][i] = sg * c[i] * inv;
-      // a hanging face contributes nothing here (nc_face_kernel adds it)
-      s_fp[f][t][DIM] = s_nbr[f] == -NC_BID ? Number(0) : na;
-    }
-  }
-  __syncthreads();
-
-  // ---------------- A': neighbour staging, cell term ----------------
-  const Number *D = s_tab + TO::D, *L = s_tab + TO::L;
-  // lane = face point: the neighbour's traces of all fields along this face
-  // point's line, summed from its nodes in memory (32-bit offsets: vectors
-  // below 4 GiB), then v.n with this side's normal
-#pragma unroll 1
-  for (int it = 0; it < S::FP_ITER; ++it) {
-    const int fp = tid + it * BLOCK;
-    if (fp < NFP) {
-      const int f = fp / NT, t = fp - f * NT, d = f >> 1, s = f & 1;
-      int base;
-      if constexpr (DIM == 2) {
-        base = d == 0 ? t * N : t;
-      } else {
-        const int t0 = t % N, t1 = t / N;
-        base = d == 0 ? t0 * N + t1 * N * N : (d == 1 ? t0 + t1 * N * N : t0 + t1 * N);
+      for (int i = 0; i < DIM; ++i) {
+        nrm[i] = sg * c[i] * inv;
+        s_fp[f][t][i] = nrm[i];
       }
-      const int sd = d == 0 ? 1 : (d == 1 ? N : N * N);
-      const int nb = s_nbr[f];
-      // boundary and hanging faces read the cell itself (unused values)
-      const uint32_t un = uint32_t(nb >= 0 ? nb : cell) * uint32_t(NF * ND) + uint32_t(base);
-      double gn[NF][N];
-#pragma unroll
-      for (int k = 0; k < NF; ++k)
-#pragma unroll
-        for (int i = 0; i < N; ++i) gn[k][i] = src[un + uint32_t(k * ND + i * sd)];
+      // a hanging face contributes nothing here (nc_face_kernel adds it)
+      s_fp[f][t][DIM] = nb == -NC_BID ? Number(0) : na;
+
       const Number *Lo = L + (1 - s) * N;
       Number an[NF];
 #pragma unroll
@@ -307,35 +332,11 @@ __global__ void __launch_bounds__((GenShape<DIM, K>::BLOCK))
       }
       Number nvm = Number(0);
 #pragma unroll
-      for (int i = 0; i < DIM; ++i) nvm += s_fp[f][t][i] * an[i];
+      for (int i = 0; i < DIM; ++i) nvm += nrm[i] * an[i];
       s_tr[f][t][0] = nvm;
       s_tr[f][t][1] = an[DIM];
     }
   }
-  Number kv[DIM];
-  {
-    Number gref[DIM];
-#pragma unroll
-    for (int e = 0; e < DIM; ++e) {
-      const int se = pow_n<N>(e), q0 = q - qi[e] * se;
-      Number g = Number(0);
-#pragma unroll
-      for (int m = 0; m < N; ++m) g += D[qi[e] * N + m] * s_u[DIM][q0 + m * se];
-      gref[e] = g;
-    }
-#pragma unroll
-    for (int i = 0; i < DIM; ++i) {
-      Number gp = Number(0);
-#pragma unroll
-      for (int e = 0; e < DIM; ++e) gp += Ji[e * DIM + i] * gref[e];
-      kv[i] = -rho_inv * gp;
-    }
-  }
-  // (empty asm that takes kv: the compiler would otherwise sink its sums
-  // into the last phase and hold the node's inverse Jacobian and gradient
-  // through the face phase, 24 registers instead of 6)
-#pragma unroll
-  for (int i = 0; i < DIM; ++i) asm volatile("" : "+v"(kv[i]));
   __syncthreads();
 
   // ---------------- F: face fluxes (lane = face point), weak divergence ----
