@@ -106,14 +106,30 @@ __global__ void __launch_bounds__(512)
     Number up[NF], um[NF];
 #pragma unroll
     for (int k = 0; k < NF; ++k) up[k] = um[k] = Number(0);
+    // the neighbour's nodes in chunks of GC line points: GC * NF loads in
+    // flight at a time instead of one point's NF (a rolled loop waited for
+    // every point in turn), the sums in the same order
+    constexpr int GC = 4;
 #pragma unroll 1
-    for (int i = 0; i < N; ++i) {
-      const int o = base + i * sd;
-      const Number ls = Ls[i], lo = Lo[i];
+    for (int i0 = 0; i0 < N; i0 += GC) {
+      double g[GC][NF];
 #pragma unroll
-      for (int k = 0; k < NF; ++k) {
-        up[k] += ls * s_u[k][o];
-        um[k] += lo * Number(un[k * ND + o]);
+      for (int j = 0; j < GC; ++j) {
+        const int o = base + (i0 + j < N ? i0 + j : N - 1) * sd;
+#pragma unroll
+        for (int k = 0; k < NF; ++k) g[j][k] = un[k * ND + o];
+      }
+#pragma unroll
+      for (int j = 0; j < GC; ++j) {
+        if (i0 + j < N) {
+          const int o = base + (i0 + j) * sd;
+          const Number ls = Ls[i0 + j], lo = Lo[i0 + j];
+#pragma unroll
+          for (int k = 0; k < NF; ++k) {
+            up[k] += ls * s_u[k][o];
+            um[k] += lo * Number(g[j][k]);
+          }
+        }
       }
     }
     Number xi[DIM], J[DIM * DIM], Jf[DIM * DIM];
