@@ -39,6 +39,14 @@ REQS = [
          final_time=0.12, cfl=0.2),
     dict(id=9, arrival=0.0, dim=3, degree=3, n_refinements=1, rho=0.7, c=0.8, modes=[1, 1, 1],
          final_time=0.2, cfl=0.2),
+    # two instances in the 3D k=6 pool (the gathered-trace kernel over a cell
+    # list, the second starting at a cell > 0) and one in the 3D k=5 pool
+    dict(id=10, arrival=0.0, dim=3, degree=6, n_refinements=1, rho=1.4, c=0.9, modes=[1, 2, 1],
+         final_time=0.05, cfl=0.2),
+    dict(id=11, arrival=0.0, dim=3, degree=6, n_refinements=1, rho=0.8, c=1.3, modes=[2, 1, 1],
+         final_time=0.04, cfl=0.2),
+    dict(id=12, arrival=0.0, dim=3, degree=5, n_refinements=1, rho=1.1, c=1.0, modes=[1, 1, 2],
+         final_time=0.05, cfl=0.2),
 ]
 
 
