@@ -208,65 +208,80 @@ __global__ void __launch_bounds__((CartGShape<DIM, K>::BLOCK))
 
   // ---------------- B: one lane per line ----------------
   // dg_operator_cart_kernel's line phase with the same chains of FMAs per
-  // value, ordered so that fewer values are live at once: v_d (its traces and
-  // the weak derivative: the dp/dt share), then p (its traces, the fluxes, the
-  // lifted dp/dt share, and the gradient: dv_d/dt)
+  // value, in three passes over the line so that at most one line of values is
+  // live at a time: the traces of v_d and p and the fluxes; v_d again (the weak
+  // derivative plus the lifted fluxes: the dp/dt share); p again (the gradient
+  // plus the lifted fluxes: dv_d/dt).  The empty asm with a memory clobber
+  // between the passes makes the compiler read the line from LDS again instead
+  // of keeping the first pass's values (3D k=6: 92 -> 71 VGPRs; k=8: 105 -> 84
+  // and 121 -> 127 GDoF/s)
   if (t < NT) {
     const Number cvv = s_vc[d][0], cvp = s_vc[d][1];
-    Number vo[2], po[2], kp[N];
+    Number fv[2], fp[2];
+    {
+      Number vo[2], po[2];
+      {
+        Number vl[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) vl[i] = s_u[d][base + i * sd];
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+          Number a = Number(0);
+#pragma unroll
+          for (int i = 0; i < N; ++i) a += Number(T.L[s * N + i]) * vl[i];
+          vo[s] = a;
+        }
+      }
+      {
+        Number pl[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) pl[i] = s_u[DIM][base + i * sd];
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+          Number a = Number(0);
+#pragma unroll
+          for (int i = 0; i < N; ++i) a += Number(T.L[s * N + i]) * pl[i];
+          po[s] = a;
+        }
+      }
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        const Number *c = s_fc[2 * d + s];
+        fv[s] = c[0] * vo[s] + c[1] * po[s] + c[2] * tv[s] + c[3] * tp[s];
+        fp[s] = c[4] * vo[s] + c[5] * po[s] + c[6] * tv[s] + c[7] * tp[s];
+      }
+    }
+    asm volatile("" ::: "memory");  // reload, do not keep the first loads
     {
       Number vl[N];
 #pragma unroll
       for (int i = 0; i < N; ++i) vl[i] = s_u[d][base + i * sd];
 #pragma unroll
-      for (int s = 0; s < 2; ++s) {
-        Number a = Number(0);
-#pragma unroll
-        for (int i = 0; i < N; ++i) a += Number(T.L[s * N + i]) * vl[i];
-        vo[s] = a;
-      }
-#pragma unroll
       for (int i = 0; i < N; ++i) {
         Number dv = Number(0);
 #pragma unroll
         for (int m = 0; m < N; ++m) dv += Number(T.DT[i * N + m]) * vl[m];
-        kp[i] = cvp * dv;
+        Number k = cvp * dv;
+        k += Number(T.LW[i]) * fp[0];
+        k += Number(T.LW[N + i]) * fp[1];
+        s_kp[d][base + i * sd] = k;
       }
     }
-    Number pl[N];
+    asm volatile("" ::: "memory");  // reload, do not keep the first loads
+    {
+      Number pl[N];
 #pragma unroll
-    for (int i = 0; i < N; ++i) pl[i] = s_u[DIM][base + i * sd];
+      for (int i = 0; i < N; ++i) pl[i] = s_u[DIM][base + i * sd];
 #pragma unroll
-    for (int s = 0; s < 2; ++s) {
-      Number a = Number(0);
+      for (int i = 0; i < N; ++i) {
+        Number gp = Number(0);
 #pragma unroll
-      for (int i = 0; i < N; ++i) a += Number(T.L[s * N + i]) * pl[i];
-      po[s] = a;
-    }
-    Number fv[2], fp[2];
-#pragma unroll
-    for (int s = 0; s < 2; ++s) {
-      const Number *c = s_fc[2 * d + s];
-      fv[s] = c[0] * vo[s] + c[1] * po[s] + c[2] * tv[s] + c[3] * tp[s];
-      fp[s] = c[4] * vo[s] + c[5] * po[s] + c[6] * tv[s] + c[7] * tp[s];
-    }
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-      Number k = kp[i];
-      k += Number(T.LW[i]) * fp[0];
-      k += Number(T.LW[N + i]) * fp[1];
-      s_kp[d][base + i * sd] = k;
-    }
-    // dv_d/dt over v_d (only this lane reads these nodes of v_d)
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-      Number gp = Number(0);
-#pragma unroll
-      for (int m = 0; m < N; ++m) gp += Number(T.D[i * N + m]) * pl[m];
-      Number k = cvv * gp;
-      k += Number(T.LW[i]) * fv[0];
-      k += Number(T.LW[N + i]) * fv[1];
-      s_u[d][base + i * sd] = k;
+        for (int m = 0; m < N; ++m) gp += Number(T.D[i * N + m]) * pl[m];
+        Number k = cvv * gp;
+        k += Number(T.LW[i]) * fv[0];
+        k += Number(T.LW[N + i]) * fv[1];
+        s_u[d][base + i * sd] = k;
+      }
     }
   }
   __syncthreads();
