@@ -2,7 +2,7 @@
 """Headline benchmark: GDoF/s per RK stage of the fused FP64 DG operator + LSRK45
 stage on one MI355X.
 
-Workload (the reference's 3D membrane at the north-star size): unit cube,
+Workload (the reference's 3D membrane, scaled to one GPU's share): unit cube,
 64^3 Cartesian hexes, degree k=4 (125 nodes/cell, fields v_x, v_y, v_z, p:
 131,072,000 DoFs), rho = c = 1, boundary id 2 (p = 0), initial pressure
 sin(pi x)sin(pi y)sin(pi z) + sin(2 pi x)sin(3 pi y)sin(pi z), v = 0,
