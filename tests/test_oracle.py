@@ -500,3 +500,35 @@ def test_estimate_error_closed_form():
     interior = (m.neighbor >= 0).all(axis=1)
     assert interior.sum() > 0
     assert np.allclose(err[interior], want[interior], rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("dim,k,tr", [(2, 1, 0), (2, 3, 1), (2, 5, 1), (3, 2, 1), (3, 3, 0)])
+def test_energy_of_constant_fields(dim, k, tr):
+    """The discrete energy 1/2 sum_K int_K (rho |v|^2 + p^2 / (rho c^2)) of fields that are
+    constant on every cell is a closed form: with one material over the unit cube (deformed
+    cells tile it exactly, and Gauss quadrature integrates det J of a multilinear map
+    exactly) 1/2 (rho |v|^2 + p^2 / (rho c^2)); with a material per box cell the sum of
+    that over the cells times their volume 1/n^dim.  A dropped det J, a quadrature weight
+    off, rho in place of 1/rho or a missing 1/2 fails it."""
+    n = 3
+    transform = O.TRANSFORM_NONE if tr == 0 else O.TRANSFORM_REFERENCE_SINE
+    m = O.Mesh(dim, n, 0, transform, 0.0 if tr == 0 else 0.1, 2)
+    nc, nd, nf = m.n_cells, (k + 1) ** dim, dim + 1
+    rng = np.random.default_rng(40 + dim * 10 + k)
+    v = rng.normal(size=dim)
+    p = 0.8
+    u = np.empty((nc, nf, nd))
+    u[:, :dim, :] = v[None, :, None]
+    u[:, dim, :] = p
+    rho0, c0 = 1.7, 0.6
+    m.set_material(np.full(nc, rho0), np.full(nc, c0))
+    want = 0.5 * (rho0 * v @ v + p * p / (rho0 * c0 * c0))
+    got = m.energy(k, u.reshape(-1))
+    assert abs(got - want) <= 1e-13 * want
+    if tr == 0:
+        rho = rng.uniform(0.5, 2.0, nc)
+        c = rng.uniform(0.5, 2.0, nc)
+        m.set_material(rho, c)
+        want = np.sum(0.5 * (rho * (v @ v) + p * p / (rho * c * c))) / n ** dim
+        got = m.energy(k, u.reshape(-1))
+        assert abs(got - want) <= 1e-13 * want
