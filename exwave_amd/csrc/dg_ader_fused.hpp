@@ -17,10 +17,13 @@
 // What it saves: the CK kernel's read of u_n+1 and one launch per step, and
 // -- the point -- the CK's LDS-bound arithmetic overlaps the sweep's memory
 // traffic of other groups on the same CU instead of running in a kernel of
-// its own.  Cartesian cells whose sweep and CK groups coincide (3D k = 3..5).
+// its own.  Cartesian cells whose sweep and CK groups coincide (3D k = 3..5);
+// FP64 at 3D k = 4, 5 runs ader_sweepg_ck_kernel (below), whose sweep half
+// gathers the neighbour traces instead of staging the neighbours in LDS.
 #pragma once
 #include "dg_ader.hpp"
 #include "dg_operator_cart.hpp"
+#include "dg_operator_pipe.hpp"
 
 namespace exw {
 
@@ -96,6 +99,265 @@ __global__ void __launch_bounds__((AderFusedShape<DIM, K>::BLOCK))
   const int lcc = lane_ok ? lck : 0;
   // the same cells on the same lanes: a node lane of the CK is a swept lane
   const bool active = lane_ok && swept;
+  Number rho_inv, rc2, ih[DIM];
+  ck_factors<DIM, true>(mesh, cell, active, rho_inv, rc2, ih);
+  Number uc[NF];
+#pragma unroll
+  for (int f = 0; f < NF; ++f) uc[f] = active ? Number(un[f]) : Number(0);
+  ck_cell<DIM, K, Number, true>(mesh, nullptr, lds.ck.tab, ckt, dt, 0.0, 0.0, lds.ck.a, lds.ck.b,
+                                lds.ck.c, nullptr, lck, lcc, q, lane_ok, active,
+                                (size_t)(active ? cell : 0) * NF * ND, rho_inv, rc2, ih, uc,
+                                w_out);
+}
+
+// The same fusion with the sweep half in the gathered-trace form of
+// dg_operator_cartg_kernel, for one-cell groups (3D k = 4, 5; FP64): the line
+// lane of line t of direction d (lanes 0 .. 3 (k+1)^2 - 1, the directions
+// mixed within a wave) sums the neighbour traces of faces 2d, 2d+1 at its face
+// points straight from the neighbours' nodes in memory, instead of the
+// neighbours' fields being staged in LDS.  The LDS of the sweep shrinks to the
+// cell's fields and the lines' dp/dt shares (3D k=4: 7.4 KB; the union is then
+// the CK's 9.4 KB instead of 16 KB) and the registers to 68 (k=4), so a CU
+// holds 14 groups instead of 10 for the CK's latency-bound phases: 3D k=4 64^3
+// 90.1 -> 97.9, k=5 40^3 71.8 -> 84.0 GDoF/s per ADER step (FP64).  The FP32
+// operator keeps the staged form (117 against 107 GDoF/s at k=4).
+// The arithmetic is dg_operator_cart_kernel's expression for expression (the
+// traces' chains over the same values, the line phase in the three passes of
+// dg_operator_cartg_kernel, the pressure shares summed in the same order): the
+// bits of ader_sweep_ck_kernel and of single steps (tested).
+template <int DIM, int K>
+struct AderFusedGShape {
+  using G = CkGroup<DIM, K>;
+  static constexpr int N = K + 1, ND = G::ND, NT = N * N, NLINE = DIM * NT, NF = DIM + 1;
+  static constexpr int BLOCK = G::BLOCK;
+  static constexpr bool ENABLED = DIM == 3 && K >= 4 && K <= 5 && G::CPB == 1 && NLINE <= BLOCK;
+};
+
+// u (n_cells cells, one per group) <- u + M^-1 A(w_in); w_out <- CK(u) over
+// [0, dt] about 0.  w_in and w_out must differ; u is read and written in place.
+template <int DIM, int K, typename Number>
+__global__ void __launch_bounds__((AderFusedGShape<DIM, K>::BLOCK))
+    ader_sweepg_ck_kernel(const DevMesh mesh, const double *__restrict__ w_in, double *u,
+                          const double *__restrict__ cktab, const CkTab ckt, double dt,
+                          double *__restrict__ w_out) {
+  using F = AderFusedGShape<DIM, K>;
+  using CS = CkShape<K>;
+  constexpr int N = F::N, ND = F::ND, NT = F::NT, NF = F::NF, NFACE = 2 * DIM;
+  constexpr int NLINE = F::NLINE, LOW = AderFusedShape<DIM, K>::LOW;
+  constexpr const CartConst<N> &T = CartTable<N>::value;
+  __shared__ __align__(16) union {
+    struct {
+      Number u[NF][ND];
+      Number kp[DIM][ND];
+      Number fc[NFACE][8];
+      Number vc[DIM][2];
+    } sw;
+    struct {
+      Number a[1][NF][ND];
+      Number b[1][NF][ND];
+      Number c[1][NF][LOW];
+      Number tab[CS::TAB_SIZE];
+    } ck;
+  } lds;
+
+  const int tid = threadIdx.x;
+  const int cell = xcd_remap(blockIdx.x, gridDim.x);
+  int nbr[NFACE];
+#pragma unroll
+  for (int f = 0; f < NFACE; ++f) nbr[f] = pipe::sld(mesh.neighbor, (size_t)cell * NFACE + f);
+  const size_t cbase = (size_t)cell * NF * ND;
+  // ---- the update sweep of step n: the cell's fields (LDS-DMA for a 16-byte
+  // aligned FP64 vector, else through registers), the lines' neighbour traces ----
+  const int nl = tid < ND ? tid : ND - 1;
+  constexpr int OWN_BYTES = NF * ND * 8, NDMA = (OWN_BYTES + 1023) / 1024;
+  const bool dma = std::is_same_v<Number, double> && (uintptr_t)w_in % 16 == 0;
+  double uo[NF];
+  if (dma) {
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const uint32_t lbase = pipe::lds_addr(&lds.sw.u[0][0]);
+    const uint32_t lane16 = uint32_t(tid & 63) * 16u;
+    const double *own = w_in + cbase;
+    static_for<NDMA>([&](auto I_) __attribute__((always_inline)) {
+      constexpr int I = decltype(I_)::value;
+      if (wave == I % (F::BLOCK / 64) && int(I * 1024 + lane16) < OWN_BYTES)
+        pipe::dma_dwordx4(lbase + I * 1024, own, lane16 + I * 1024);
+    });
+  } else {
+#pragma unroll
+    for (int f = 0; f < NF; ++f) uo[f] = w_in[cbase + f * ND + nl];
+  }
+  const bool line_ok = tid < NLINE;
+  const int tl = line_ok ? tid : 0;
+  const int d = tl / NT, t = tl - d * NT;
+  int base, sd;
+  {
+    const int t0 = t % N, t1 = t / N;
+    base = d == 0 ? t0 * N + t1 * N * N : (d == 1 ? t0 + t1 * N * N : t0 + t1 * N);
+    sd = d == 0 ? 1 : (d == 1 ? N : N * N);
+  }
+  Number tv[2], tp[2];
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    const int nb = d == 0 ? nbr[s] : (d == 1 ? nbr[2 + s] : nbr[4 + s]);
+    const double *un = w_in + (size_t)(nb >= 0 ? nb : cell) * NF * ND;
+    const double *uv = un + d * ND + base, *up = un + DIM * ND + base;
+    double xv[N], xp[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      xv[i] = uv[i * sd];
+      xp[i] = up[i * sd];
+    }
+    Number vn = Number(0), pn = Number(0);
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const Number lo = Number(T.L[(1 - s) * N + i]);
+      vn += lo * Number(xv[i]);
+      pn += lo * Number(xp[i]);
+    }
+    tv[s] = vn;
+    tp[s] = pn;
+  }
+  // (the sums taken by an empty asm, so that the loads' registers are free
+  // again before the barrier)
+  asm volatile("" : "+v"(tv[0]), "+v"(tv[1]), "+v"(tp[0]), "+v"(tp[1]));
+  if (!dma && tid < ND) {
+#pragma unroll
+    for (int f = 0; f < NF; ++f) lds.sw.u[f][tid] = Number(uo[f]);
+  }
+  if (tid < NFACE) {
+    // face lane: the linear flux map and the cell term's factors
+    // (dg_operator_cart_kernel's face-lane expressions)
+    const int f = tid, fd = f >> 1;
+    int nb = nbr[0];
+#pragma unroll
+    for (int g = 1; g < NFACE; ++g)
+      if (tid == g) nb = nbr[g];
+    const double rho = mesh.rho[cell], cc = mesh.speed[cell];
+    const double rho_m = nb >= 0 ? mesh.rho[nb] : rho, c_m = nb >= 0 ? mesh.speed[nb] : cc;
+    const double ih = recip(mesh.h[(size_t)cell * DIM + fd]);
+    const double sg = (f & 1) ? 1.0 : -1.0;
+    const double ihf = nb == -NC_BID ? 0.0 : ih;
+    double a[4], b[4];
+    flux_matrix<double>(rho, cc, rho_m, c_m, nb >= 0 ? 0 : -nb, a, b);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const double in = (j == 0 || j == 2) ? sg : 1.0;
+      lds.sw.fc[f][j] = Number(a[j] * in * sg * ihf);
+      lds.sw.fc[f][4 + j] = Number(b[j] * in * ihf);
+    }
+    if ((f & 1) == 0) {
+      lds.sw.vc[fd][0] = Number(-ih * recip(rho));
+      lds.sw.vc[fd][1] = Number(rho * cc * cc * ih);
+    }
+  }
+  if (dma) pipe::wait_vm<0>();
+  __syncthreads();
+  double bv[NF];
+#pragma unroll
+  for (int f = 0; f < NF; ++f) bv[f] = __builtin_nontemporal_load(u + cbase + f * ND + nl);
+
+  // the line phase (dg_operator_cartg_kernel's three passes; the memory
+  // clobbers make the second and third pass read the line from LDS again)
+  if (line_ok) {
+    auto &s_u = lds.sw.u;
+    auto &s_kp = lds.sw.kp;
+    const Number cvv = lds.sw.vc[d][0], cvp = lds.sw.vc[d][1];
+    Number fv[2], fp[2];
+    {
+      Number vo[2], po[2];
+      {
+        Number vl[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) vl[i] = s_u[d][base + i * sd];
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+          Number a = Number(0);
+#pragma unroll
+          for (int i = 0; i < N; ++i) a += Number(T.L[s * N + i]) * vl[i];
+          vo[s] = a;
+        }
+      }
+      {
+        Number pl[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) pl[i] = s_u[DIM][base + i * sd];
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+          Number a = Number(0);
+#pragma unroll
+          for (int i = 0; i < N; ++i) a += Number(T.L[s * N + i]) * pl[i];
+          po[s] = a;
+        }
+      }
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        const Number *c = lds.sw.fc[2 * d + s];
+        fv[s] = c[0] * vo[s] + c[1] * po[s] + c[2] * tv[s] + c[3] * tp[s];
+        fp[s] = c[4] * vo[s] + c[5] * po[s] + c[6] * tv[s] + c[7] * tp[s];
+      }
+    }
+    asm volatile("" ::: "memory");
+    {
+      Number vl[N];
+#pragma unroll
+      for (int i = 0; i < N; ++i) vl[i] = s_u[d][base + i * sd];
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        Number dv = Number(0);
+#pragma unroll
+        for (int m = 0; m < N; ++m) dv += Number(T.DT[i * N + m]) * vl[m];
+        Number k = cvp * dv;
+        k += Number(T.LW[i]) * fp[0];
+        k += Number(T.LW[N + i]) * fp[1];
+        s_kp[d][base + i * sd] = k;
+      }
+    }
+    asm volatile("" ::: "memory");
+    {
+      Number pl[N];
+#pragma unroll
+      for (int i = 0; i < N; ++i) pl[i] = s_u[DIM][base + i * sd];
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        Number gp = Number(0);
+#pragma unroll
+        for (int m = 0; m < N; ++m) gp += Number(T.D[i * N + m]) * pl[m];
+        Number k = cvv * gp;
+        k += Number(T.LW[i]) * fv[0];
+        k += Number(T.LW[N + i]) * fv[1];
+        s_u[d][base + i * sd] = k;
+      }
+    }
+  }
+  __syncthreads();
+  // u_n+1 at the lane's node (b = o1 = u, a1 = 1)
+  const bool node_ok = tid < ND;
+  double un[NF];
+  if (node_ok) {
+    Number k[NF];
+    Number kp = lds.sw.kp[0][tid];
+#pragma unroll
+    for (int dd = 0; dd < DIM; ++dd) {
+      k[dd] = lds.sw.u[dd][tid];
+      if (dd > 0) kp += lds.sw.kp[dd][tid];
+    }
+    k[DIM] = kp;
+    const double ea1 = 1.0;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      un[f] = bv[f] + ea1 * double(k[f]);
+      __builtin_nontemporal_store(un[f], u + cbase + f * ND + tid);
+    }
+  }
+  __syncthreads();  // every lane is done with the sweep's LDS
+  // ---- the CK of step n+1 on the same cell (ader_sweep_ck_kernel's) ----
+  for (int i = tid; i < CS::TAB_SIZE && i < ckt.size; i += blockDim.x)
+    lds.ck.tab[i] = Number(cktab[i]);
+  const int lck = tid / ND;
+  const int q = tid - lck * ND;
+  const bool lane_ok = lck < 1;
+  const int lcc = 0;
+  const bool active = lane_ok;
   Number rho_inv, rc2, ih[DIM];
   ck_factors<DIM, true>(mesh, cell, active, rho_inv, rc2, ih);
   Number uc[NF];

@@ -1,10 +1,12 @@
 """Several global ADER steps at once (exw_ader_steps) against single steps.
 
 On box meshes without hanging faces of 3D degree 3-5 the update sweep of each
-step runs fused with the next step's CK time integral (ader_sweep_ck_kernel);
-both halves are the shared device functions of the separate kernels, so n
-steps must give exactly the bits of n exw_ader_step calls, in FP64 and with
-the FP32 operator, for every step count (the two CK fields alternate).  Meshes
+step runs fused with the next step's CK time integral: ader_sweep_ck_kernel
+(staged neighbour fields; FP32 operator, and FP64 at degree 3) and
+ader_sweepg_ck_kernel (gathered neighbour traces; FP64 at degree 4, 5).  Both
+evaluate the separate kernels' expressions in the same order, so n steps must
+give exactly the bits of n exw_ader_step calls, in FP64 and with the FP32
+operator, for every step count (the two CK fields alternate).  Meshes
 outside the fused kernel (hanging faces, deformed cells, 2D) take one step
 after the other in place, with the same bits.  The oracle's ADER steps anchor
 the chain (<= 1e-12 relative).
