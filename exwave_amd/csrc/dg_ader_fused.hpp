@@ -18,7 +18,7 @@
 // -- the point -- the CK's LDS-bound arithmetic overlaps the sweep's memory
 // traffic of other groups on the same CU instead of running in a kernel of
 // its own.  Cartesian cells whose sweep and CK groups coincide (3D k = 3..5);
-// FP64 at 3D k = 4, 5 runs ader_sweepg_ck_kernel (below), whose sweep half
+// FP64 runs ader_sweepg_ck_kernel (below) instead, whose sweep half
 // gathers the neighbour traces instead of staging the neighbours in LDS.
 #pragma once
 #include "dg_ader.hpp"
@@ -111,15 +111,17 @@ __global__ void __launch_bounds__((AderFusedShape<DIM, K>::BLOCK))
 }
 
 // The same fusion with the sweep half in the gathered-trace form of
-// dg_operator_cartg_kernel, for one-cell groups (3D k = 4, 5; FP64): the line
-// lane of line t of direction d (lanes 0 .. 3 (k+1)^2 - 1, the directions
-// mixed within a wave) sums the neighbour traces of faces 2d, 2d+1 at its face
+// dg_operator_cartg_kernel (FP64, 3D k = 3..5: one cell per group at k = 4, 5,
+// two one-wave cells at k = 3): the line lane of line t of direction d (lanes
+// 0 .. 3 (k+1)^2 - 1 of the cell's, the directions mixed within a wave) sums
+// the neighbour traces of faces 2d, 2d+1 at its face
 // points straight from the neighbours' nodes in memory, instead of the
 // neighbours' fields being staged in LDS.  The LDS of the sweep shrinks to the
 // cell's fields and the lines' dp/dt shares (3D k=4: 7.4 KB; the union is then
 // the CK's 9.4 KB instead of 16 KB) and the registers to 68 (k=4), so a CU
 // holds 14 groups instead of 10 for the CK's latency-bound phases: 3D k=4 64^3
-// 90.1 -> 97.9, k=5 40^3 71.8 -> 84.0 GDoF/s per ADER step (FP64).  The FP32
+// 90.1 -> 97.9, k=5 40^3 71.8 -> 84.0, k=3 64^3 115 -> 127 GDoF/s per ADER
+// step (FP64).  The FP32
 // operator keeps the staged form (117 against 107 GDoF/s at k=4).
 // The arithmetic is dg_operator_cart_kernel's expression for expression (the
 // traces' chains over the same values, the line phase in the three passes of
@@ -129,11 +131,13 @@ template <int DIM, int K>
 struct AderFusedGShape {
   using G = CkGroup<DIM, K>;
   static constexpr int N = K + 1, ND = G::ND, NT = N * N, NLINE = DIM * NT, NF = DIM + 1;
-  static constexpr int BLOCK = G::BLOCK;
-  static constexpr bool ENABLED = DIM == 3 && K >= 4 && K <= 5 && G::CPB == 1 && NLINE <= BLOCK;
+  static constexpr int CPB = G::CPB, BLOCK = G::BLOCK, LANES = BLOCK / CPB;  // lanes per cell
+  // each cell on whole waves, with a lane for each of its nodes and lines
+  static constexpr bool ENABLED = DIM == 3 && K >= 3 && K <= 5 && CPB * LANES == BLOCK &&
+                                  LANES % 64 == 0 && ND <= LANES && NLINE <= LANES;
 };
 
-// u (n_cells cells, one per group) <- u + M^-1 A(w_in); w_out <- CK(u) over
+// u (n_cells cells, CPB per group) <- u + M^-1 A(w_in); w_out <- CK(u) over
 // [0, dt] about 0.  w_in and w_out must differ; u is read and written in place.
 template <int DIM, int K, typename Number>
 __global__ void __launch_bounds__((AderFusedGShape<DIM, K>::BLOCK))
@@ -143,51 +147,64 @@ __global__ void __launch_bounds__((AderFusedGShape<DIM, K>::BLOCK))
   using F = AderFusedGShape<DIM, K>;
   using CS = CkShape<K>;
   constexpr int N = F::N, ND = F::ND, NT = F::NT, NF = F::NF, NFACE = 2 * DIM;
-  constexpr int NLINE = F::NLINE, LOW = AderFusedShape<DIM, K>::LOW;
+  constexpr int NLINE = F::NLINE, CPB = F::CPB, LANES = F::LANES;
+  constexpr int LOW = AderFusedShape<DIM, K>::LOW;
   constexpr const CartConst<N> &T = CartTable<N>::value;
   __shared__ __align__(16) union {
     struct {
-      Number u[NF][ND];
-      Number kp[DIM][ND];
-      Number fc[NFACE][8];
-      Number vc[DIM][2];
+      Number u[CPB][NF][ND];
+      Number kp[CPB][DIM][ND];
+      Number fc[CPB][NFACE][8];
+      Number vc[CPB][DIM][2];
     } sw;
     struct {
-      Number a[1][NF][ND];
-      Number b[1][NF][ND];
-      Number c[1][NF][LOW];
+      Number a[CPB][NF][ND];
+      Number b[CPB][NF][ND];
+      Number c[CPB][NF][LOW];
       Number tab[CS::TAB_SIZE];
     } ck;
   } lds;
 
+  // the lane's cell slot (uniform over a wave) and its lane within the cell;
+  // a slot past the last cell (the last group) computes on the last cell and
+  // stores nothing
   const int tid = threadIdx.x;
-  const int cell = xcd_remap(blockIdx.x, gridDim.x);
+  const int lc = __builtin_amdgcn_readfirstlane(tid / LANES);
+  const int lt = tid - lc * LANES;
+  const int e = xcd_remap(blockIdx.x, gridDim.x) * CPB + lc;
+  const bool swept = e < mesh.n_cells;
+  const int cell = swept ? e : mesh.n_cells - 1;
+  auto &s_u = lds.sw.u[lc];
+  auto &s_kp = lds.sw.kp[lc];
+  auto &s_fc = lds.sw.fc[lc];
+  auto &s_vc = lds.sw.vc[lc];
   int nbr[NFACE];
 #pragma unroll
   for (int f = 0; f < NFACE; ++f) nbr[f] = pipe::sld(mesh.neighbor, (size_t)cell * NFACE + f);
   const size_t cbase = (size_t)cell * NF * ND;
   // ---- the update sweep of step n: the cell's fields (LDS-DMA for a 16-byte
   // aligned FP64 vector, else through registers), the lines' neighbour traces ----
-  const int nl = tid < ND ? tid : ND - 1;
+  const int nl = lt < ND ? lt : ND - 1;
   constexpr int OWN_BYTES = NF * ND * 8, NDMA = (OWN_BYTES + 1023) / 1024;
   const bool dma = std::is_same_v<Number, double> && (uintptr_t)w_in % 16 == 0;
   double uo[NF];
   if (dma) {
-    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const uint32_t lbase = pipe::lds_addr(&lds.sw.u[0][0]);
+    // the cell slot's waves share the instructions
+    const int wave = __builtin_amdgcn_readfirstlane(lt >> 6);
+    const uint32_t lbase = pipe::lds_addr(&s_u[0][0]);
     const uint32_t lane16 = uint32_t(tid & 63) * 16u;
     const double *own = w_in + cbase;
     static_for<NDMA>([&](auto I_) __attribute__((always_inline)) {
       constexpr int I = decltype(I_)::value;
-      if (wave == I % (F::BLOCK / 64) && int(I * 1024 + lane16) < OWN_BYTES)
+      if (wave == I % (LANES / 64) && int(I * 1024 + lane16) < OWN_BYTES)
         pipe::dma_dwordx4(lbase + I * 1024, own, lane16 + I * 1024);
     });
   } else {
 #pragma unroll
     for (int f = 0; f < NF; ++f) uo[f] = w_in[cbase + f * ND + nl];
   }
-  const bool line_ok = tid < NLINE;
-  const int tl = line_ok ? tid : 0;
+  const bool line_ok = lt < NLINE;
+  const int tl = line_ok ? lt : 0;
   const int d = tl / NT, t = tl - d * NT;
   int base, sd;
   {
@@ -220,18 +237,18 @@ __global__ void __launch_bounds__((AderFusedGShape<DIM, K>::BLOCK))
   // (the sums taken by an empty asm, so that the loads' registers are free
   // again before the barrier)
   asm volatile("" : "+v"(tv[0]), "+v"(tv[1]), "+v"(tp[0]), "+v"(tp[1]));
-  if (!dma && tid < ND) {
+  if (!dma && lt < ND) {
 #pragma unroll
-    for (int f = 0; f < NF; ++f) lds.sw.u[f][tid] = Number(uo[f]);
+    for (int f = 0; f < NF; ++f) s_u[f][lt] = Number(uo[f]);
   }
-  if (tid < NFACE) {
+  if (lt < NFACE) {
     // face lane: the linear flux map and the cell term's factors
     // (dg_operator_cart_kernel's face-lane expressions)
-    const int f = tid, fd = f >> 1;
+    const int f = lt, fd = f >> 1;
     int nb = nbr[0];
 #pragma unroll
     for (int g = 1; g < NFACE; ++g)
-      if (tid == g) nb = nbr[g];
+      if (lt == g) nb = nbr[g];
     const double rho = mesh.rho[cell], cc = mesh.speed[cell];
     const double rho_m = nb >= 0 ? mesh.rho[nb] : rho, c_m = nb >= 0 ? mesh.speed[nb] : cc;
     const double ih = recip(mesh.h[(size_t)cell * DIM + fd]);
@@ -242,12 +259,12 @@ __global__ void __launch_bounds__((AderFusedGShape<DIM, K>::BLOCK))
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const double in = (j == 0 || j == 2) ? sg : 1.0;
-      lds.sw.fc[f][j] = Number(a[j] * in * sg * ihf);
-      lds.sw.fc[f][4 + j] = Number(b[j] * in * ihf);
+      s_fc[f][j] = Number(a[j] * in * sg * ihf);
+      s_fc[f][4 + j] = Number(b[j] * in * ihf);
     }
     if ((f & 1) == 0) {
-      lds.sw.vc[fd][0] = Number(-ih * recip(rho));
-      lds.sw.vc[fd][1] = Number(rho * cc * cc * ih);
+      s_vc[fd][0] = Number(-ih * recip(rho));
+      s_vc[fd][1] = Number(rho * cc * cc * ih);
     }
   }
   if (dma) pipe::wait_vm<0>();
@@ -259,9 +276,7 @@ __global__ void __launch_bounds__((AderFusedGShape<DIM, K>::BLOCK))
   // the line phase (dg_operator_cartg_kernel's three passes; the memory
   // clobbers make the second and third pass read the line from LDS again)
   if (line_ok) {
-    auto &s_u = lds.sw.u;
-    auto &s_kp = lds.sw.kp;
-    const Number cvv = lds.sw.vc[d][0], cvp = lds.sw.vc[d][1];
+    const Number cvv = s_vc[d][0], cvp = s_vc[d][1];
     Number fv[2], fp[2];
     {
       Number vo[2], po[2];
@@ -291,7 +306,7 @@ __global__ void __launch_bounds__((AderFusedGShape<DIM, K>::BLOCK))
       }
 #pragma unroll
       for (int s = 0; s < 2; ++s) {
-        const Number *c = lds.sw.fc[2 * d + s];
+        const Number *c = s_fc[2 * d + s];
         fv[s] = c[0] * vo[s] + c[1] * po[s] + c[2] * tv[s] + c[3] * tp[s];
         fp[s] = c[4] * vo[s] + c[5] * po[s] + c[6] * tv[s] + c[7] * tp[s];
       }
@@ -331,33 +346,35 @@ __global__ void __launch_bounds__((AderFusedGShape<DIM, K>::BLOCK))
   }
   __syncthreads();
   // u_n+1 at the lane's node (b = o1 = u, a1 = 1)
-  const bool node_ok = tid < ND;
+  const bool node_ok = swept && lt < ND;
   double un[NF];
   if (node_ok) {
     Number k[NF];
-    Number kp = lds.sw.kp[0][tid];
+    Number kp = s_kp[0][lt];
 #pragma unroll
     for (int dd = 0; dd < DIM; ++dd) {
-      k[dd] = lds.sw.u[dd][tid];
-      if (dd > 0) kp += lds.sw.kp[dd][tid];
+      k[dd] = s_u[dd][lt];
+      if (dd > 0) kp += s_kp[dd][lt];
     }
     k[DIM] = kp;
     const double ea1 = 1.0;
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
       un[f] = bv[f] + ea1 * double(k[f]);
-      __builtin_nontemporal_store(un[f], u + cbase + f * ND + tid);
+      __builtin_nontemporal_store(un[f], u + cbase + f * ND + lt);
     }
   }
   __syncthreads();  // every lane is done with the sweep's LDS
   // ---- the CK of step n+1 on the same cell (ader_sweep_ck_kernel's) ----
   for (int i = tid; i < CS::TAB_SIZE && i < ckt.size; i += blockDim.x)
     lds.ck.tab[i] = Number(cktab[i]);
+  // the CK's lanes: node q of cell slot lck, the sweep's node lane of the same
+  // cell (ND = LANES at 3D k=3; one cell per group at k=4, 5)
   const int lck = tid / ND;
   const int q = tid - lck * ND;
-  const bool lane_ok = lck < 1;
-  const int lcc = 0;
-  const bool active = lane_ok;
+  const bool lane_ok = lck < CPB;
+  const int lcc = lane_ok ? lck : 0;
+  const bool active = lane_ok && swept;
   Number rho_inv, rc2, ih[DIM];
   ck_factors<DIM, true>(mesh, cell, active, rho_inv, rc2, ih);
   Number uc[NF];

@@ -200,9 +200,11 @@ template <int DIM, int K, typename Number>
 hipError_t launch_ader_fused(const DevMesh &m, const double *w_in, double *u, const double *cktab,
                              const CkTab &ck, double dt, double *w_out, hipStream_t s) {
   if constexpr (AderFusedGShape<DIM, K>::ENABLED && std::is_same_v<Number, double>) {
-    // one-cell groups with gathered neighbour traces (FP64, 3D k = 4, 5)
-    if (m.n_cells == 0) return hipSuccess;
-    hipLaunchKernelGGL((ader_sweepg_ck_kernel<DIM, K, Number>), dim3(m.n_cells),
+    // gathered neighbour traces (FP64, 3D k = 3..5)
+    using G = AderFusedGShape<DIM, K>;
+    const int grid = (m.n_cells + G::CPB - 1) / G::CPB;
+    if (grid == 0) return hipSuccess;
+    hipLaunchKernelGGL((ader_sweepg_ck_kernel<DIM, K, Number>), dim3(grid),
                        dim3(AderFusedGShape<DIM, K>::BLOCK), 0, s, m, w_in, u, cktab, ck, dt, w_out);
     return hipGetLastError();
   }

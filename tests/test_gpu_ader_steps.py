@@ -2,8 +2,8 @@
 
 On box meshes without hanging faces of 3D degree 3-5 the update sweep of each
 step runs fused with the next step's CK time integral: ader_sweep_ck_kernel
-(staged neighbour fields; FP32 operator, and FP64 at degree 3) and
-ader_sweepg_ck_kernel (gathered neighbour traces; FP64 at degree 4, 5).  Both
+(staged neighbour fields; the FP32 operator) and ader_sweepg_ck_kernel
+(gathered neighbour traces; FP64, two cells per group at degree 3).  Both
 evaluate the separate kernels' expressions in the same order, so n steps must
 give exactly the bits of n exw_ader_step calls, in FP64 and with the FP32
 operator, for every step count (the two CK fields alternate).  Meshes
